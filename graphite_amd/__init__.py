@@ -1,0 +1,424 @@
+"""graphite_amd: MI355X (gfx950) implementation of GRAPHITE's second-order unstructured
+finite-volume residual path.
+
+The compute and the host runtime live in libgraphite_hip.so (C ABI, include/graphite_hip.h);
+this module is a thin Python binding over that ABI for tests, benchmarks and scripting.
+Host arrays are numpy; device memory is owned by the library.  Layouts follow the
+reference's (Alexsimulation/graphite): scalar fields (n,), vector fields (n, 3), scalar
+gradients (n, 3), vector gradients (n, 9) as tensor columns.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._capi import (GR_BC_FARFIELD, GR_BC_SLIP, GR_F32, GR_F64, GR_HEX, GR_LIM_BARTH_JESPERSEN,
+                    GR_LIM_MICHALAK, GR_LIM_VENKATAKRISHNAN, GR_OK, GR_PYRAMID, GR_QUAD, GR_TET,
+                    GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, load)
+
+__all__ = ["GraphiteError", "Elements", "Mesh", "Field", "Solver", "device_count", "load",
+           "euler_init", "euler_bc_slip", "euler_bc_farfield", "euler_step_prepare", "euler_step",
+           "euler_residual"]
+
+
+class GraphiteError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _check(rc: int) -> None:
+    if rc != GR_OK:
+        raise GraphiteError(rc, load().gr_last_error().decode())
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_count() -> int:
+    return int(load().gr_device_count())
+
+
+# ---- element lists ----------------------------------------------------------------------
+
+class Elements:
+    """su2-style element lists (nodes, volume elements, boundary markers)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().gr_elements_free(h)
+            self._h = None
+
+    @staticmethod
+    def create(nodes, kinds, starts, elem_nodes, markers=()):
+        """markers: iterable of (name, kinds, starts, nodes)."""
+        L = load()
+        xyz = _f64(nodes).reshape(-1)
+        k, s, n = _u32(kinds), _u32(starts), _u32(elem_nodes)
+        h = C.c_void_p()
+        _check(L.gr_elements_create(len(xyz) // 3, _p(xyz, C.c_double), len(k), _p(k, C.c_uint32),
+                                    _p(s, C.c_uint32), _p(n, C.c_uint32), C.byref(h)))
+        e = Elements(h)
+        for name, mk, ms, mn in markers:
+            mk, ms, mn = _u32(mk), _u32(ms), _u32(mn)
+            _check(L.gr_elements_add_marker(h, name.encode(), len(mk), _p(mk, C.c_uint32),
+                                            _p(ms, C.c_uint32), _p(mn, C.c_uint32)))
+        return e
+
+    @staticmethod
+    def box(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, tets=False, jitter=0.0, seed=0):
+        h = C.c_void_p()
+        _check(load().gr_gen_box(nx, ny, nz, lx, ly, lz, int(tets), jitter, seed, C.byref(h)))
+        return Elements(h)
+
+    @staticmethod
+    def ogrid(n_theta, n_r, n_z, r0, r1, ratio, dz):
+        h = C.c_void_p()
+        _check(load().gr_gen_ogrid(n_theta, n_r, n_z, r0, r1, ratio, dz, C.byref(h)))
+        return Elements(h)
+
+    @staticmethod
+    def read_su2(path: str):
+        h = C.c_void_p()
+        _check(load().gr_elements_read_su2(path.encode(), C.byref(h)))
+        return Elements(h)
+
+    def write_su2(self, path: str) -> None:
+        _check(load().gr_elements_write_su2(self._h, path.encode()))
+
+    def info(self) -> ElementsInfo:
+        i = ElementsInfo()
+        _check(load().gr_elements_info_get(self._h, C.byref(i)))
+        return i
+
+    def arrays(self):
+        """(nodes (n,3), kinds, starts, elem_nodes, markers[(name, kinds, starts, nodes)])"""
+        L = load()
+        i = self.info()
+        xyz = np.zeros(3 * i.n_nodes)
+        k = np.zeros(i.n_elems, np.uint32)
+        s = np.zeros(i.n_elems + 1, np.uint32)
+        n = np.zeros(i.n_elem_nodes, np.uint32)
+        _check(L.gr_elements_get(self._h, _p(xyz, C.c_double), _p(k, C.c_uint32), _p(s, C.c_uint32),
+                                 _p(n, C.c_uint32)))
+        markers = []
+        for m in range(i.n_markers):
+            name = C.create_string_buffer(128)
+            cnt, nn = C.c_uint32(), C.c_uint32()
+            _check(L.gr_elements_get_marker(self._h, m, name, C.byref(cnt), C.byref(nn), None, None, None))
+            mk = np.zeros(cnt.value, np.uint32)
+            ms = np.zeros(cnt.value + 1, np.uint32)
+            mn = np.zeros(nn.value, np.uint32)
+            _check(L.gr_elements_get_marker(self._h, m, None, None, None, _p(mk, C.c_uint32),
+                                            _p(ms, C.c_uint32), _p(mn, C.c_uint32)))
+            markers.append((name.value.decode(), mk, ms, mn))
+        return xyz.reshape(-1, 3), k, s, n, markers
+
+
+@dataclass
+class ElementArrays:
+    """Plain arrays of an element list (the form the oracle consumes)."""
+    nodes: np.ndarray
+    elem_types: np.ndarray
+    elem_starts: np.ndarray
+    elem_nodes: np.ndarray
+    markers: list = field(default_factory=list)
+
+    @staticmethod
+    def of(e: Elements) -> "ElementArrays":
+        x, k, s, n, m = e.arrays()
+        return ElementArrays(x, k, s, n, m)
+
+
+# ---- mesh ---------------------------------------------------------------------------------
+
+class Mesh:
+    """cuMesh equivalent: host CSR connectivity + device metrics."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().gr_mesh_free(h)
+            self._h = None
+
+    @staticmethod
+    def build(e: Elements) -> "Mesh":
+        h = C.c_void_p()
+        _check(load().gr_mesh_build(e._h, C.byref(h)))
+        return Mesh(h)
+
+    @staticmethod
+    def read_fvm(path: str, scalar_bytes: int = 8) -> "Mesh":
+        h = C.c_void_p()
+        _check(load().gr_mesh_read_fvm(path.encode(), scalar_bytes, C.byref(h)))
+        return Mesh(h)
+
+    @staticmethod
+    def from_su2(su2_path: str, fvm_path: str, scalar_bytes: int = 8) -> None:
+        _check(load().gr_mesh_from_su2(su2_path.encode(), fvm_path.encode(), scalar_bytes))
+
+    @staticmethod
+    def make_1d(n: int, x0: float = 0.0, x1: float = 1.0) -> "Mesh":
+        h = C.c_void_p()
+        _check(load().gr_mesh_make_1d(n, x0, x1, C.byref(h)))
+        return Mesh(h)
+
+    def write_fvm(self, path: str, scalar_bytes: int = 8) -> None:
+        _check(load().gr_mesh_write_fvm(self._h, path.encode(), scalar_bytes))
+
+    def sizes(self) -> MeshSizes:
+        s = MeshSizes()
+        _check(load().gr_mesh_sizes_get(self._h, C.byref(s)))
+        return s
+
+    @property
+    def n_cells(self) -> int:
+        return self.sizes().n_cells
+
+    @property
+    def n_ghosts(self) -> int:
+        return self.sizes().n_ghosts
+
+    def connectivity(self) -> dict:
+        s = self.sizes()
+        ncg = s.n_cells + s.n_ghosts
+        out = dict(
+            nodes=np.zeros(3 * s.n_nodes), face_nodes_starts=np.zeros(s.n_faces + 1, np.uint32),
+            face_nodes=np.zeros(s.n_face_nodes, np.uint32), cell_faces_starts=np.zeros(ncg + 1, np.uint32),
+            cell_faces=np.zeros(s.n_cell_slots, np.uint32), cell_connects=np.zeros(s.n_cell_slots, np.uint32),
+            ghost_cell_ids=np.zeros(s.n_ghosts, np.uint32), ghost_cell_starts=np.zeros(s.n_regions + 1, np.uint32))
+        a = out
+        _check(load().gr_mesh_get_connectivity(
+            self._h, _p(a["nodes"], C.c_double), _p(a["face_nodes_starts"], C.c_uint32),
+            _p(a["face_nodes"], C.c_uint32), _p(a["cell_faces_starts"], C.c_uint32),
+            _p(a["cell_faces"], C.c_uint32), _p(a["cell_connects"], C.c_uint32),
+            _p(a["ghost_cell_ids"], C.c_uint32), _p(a["ghost_cell_starts"], C.c_uint32)))
+        out["nodes"] = out["nodes"].reshape(-1, 3)
+        return out
+
+    def region_id(self, name: str) -> int:
+        r = C.c_uint32()
+        _check(load().gr_mesh_region_id(self._h, name.encode(), C.byref(r)))
+        return r.value
+
+    def region_names(self) -> list:
+        out = []
+        for r in range(self.sizes().n_regions):
+            b = C.create_string_buffer(128)
+            _check(load().gr_mesh_region_name(self._h, r, b))
+            out.append(b.value.decode())
+        return out
+
+    def renumber_rcm(self):
+        s = self.sizes()
+        c2o = np.zeros(s.n_cells + s.n_ghosts, np.uint32)
+        f2o = np.zeros(s.n_faces, np.uint32)
+        _check(load().gr_mesh_renumber_rcm(self._h, _p(c2o, C.c_uint32), _p(f2o, C.c_uint32)))
+        return c2o, f2o
+
+    def face_colors(self):
+        s = self.sizes()
+        col = np.zeros(s.n_faces, np.uint32)
+        n = C.c_uint32()
+        _check(load().gr_mesh_face_colors(self._h, _p(col, C.c_uint32), C.byref(n)))
+        return col, n.value
+
+    def bandwidth(self) -> int:
+        b = C.c_uint32()
+        _check(load().gr_mesh_bandwidth(self._h, C.byref(b)))
+        return b.value
+
+    def compute_metrics(self) -> None:
+        _check(load().gr_mesh_compute_metrics(self._h))
+
+    def metrics(self) -> dict:
+        s = self.sizes()
+        ncg = s.n_cells + s.n_ghosts
+        out = dict(face_centers=np.zeros(3 * s.n_faces), face_normals=np.zeros(3 * s.n_faces),
+                   face_areas=np.zeros(s.n_faces), cell_centers=np.zeros(3 * ncg),
+                   cell_volumes=np.zeros(ncg), cell_matrices=np.zeros(9 * s.n_cells))
+        _check(load().gr_mesh_get_metrics(self._h, *[_p(out[k], C.c_double) for k in (
+            "face_centers", "face_normals", "face_areas", "cell_centers", "cell_volumes", "cell_matrices")]))
+        out["face_centers"] = out["face_centers"].reshape(-1, 3)
+        out["face_normals"] = out["face_normals"].reshape(-1, 3)
+        out["cell_centers"] = out["cell_centers"].reshape(-1, 3)
+        out["cell_matrices"] = out["cell_matrices"].reshape(-1, 9)
+        return out
+
+    def set_corrected_reconstruction(self, on: bool) -> None:
+        _check(load().gr_euler_set_corrected_reconstruction(self._h, int(on)))
+
+
+# ---- fields -------------------------------------------------------------------------------
+
+class Field:
+    """FvmField equivalent: value/gradient/limiter over cells and ghosts, on the device."""
+
+    def __init__(self, mesh: Mesh, n_comp: int, dtype: int = GR_F64, name: str = "", grads: bool = True):
+        h = C.c_void_p()
+        _check(load().gr_field_create(mesh._h, n_comp, dtype, name.encode(), int(grads), C.byref(h)))
+        self._h, self.mesh, self.n_comp, self.dtype, self.grads = h, mesh, n_comp, dtype, grads
+        s = mesh.sizes()
+        self.n = s.n_cells + s.n_ghosts
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().gr_field_free(h)
+            self._h = None
+
+    def upload(self, value) -> None:
+        v = _f64(value).reshape(-1)
+        assert v.size == self.n * self.n_comp
+        _check(load().gr_field_upload(self._h, _p(v, C.c_double)))
+
+    def download(self, grads: bool = False):
+        v = np.zeros(self.n * self.n_comp)
+        g = np.zeros(self.n * self.n_comp * 3) if grads else None
+        l = np.zeros(self.n * self.n_comp) if grads else None
+        _check(load().gr_field_download(self._h, _p(v, C.c_double), _p(g, C.c_double) if grads else None,
+                                        _p(l, C.c_double) if grads else None))
+        shp = (self.n,) if self.n_comp == 1 else (self.n, 3)
+        if not grads:
+            return v.reshape(shp)
+        return v.reshape(shp), g.reshape(self.n, 3 * self.n_comp), l.reshape(shp)
+
+    def init(self, value) -> None:
+        v = _f64(np.atleast_1d(value))
+        _check(load().gr_field_init(self._h, _p(v, C.c_double)))
+
+    def reduce_sum(self) -> float:
+        out = C.c_double()
+        _check(load().gr_field_reduce_sum(self._h, C.byref(out)))
+        return out.value
+
+    def compute_gradient(self) -> None:
+        _check(load().gr_field_compute_gradient(self._h))
+
+    def compute_limiter(self, limiter: int = GR_LIM_MICHALAK) -> None:
+        _check(load().gr_field_compute_limiter(self._h, limiter))
+
+    def gradlim_bc(self) -> None:
+        _check(load().gr_field_gradlim_bc(self._h))
+
+    def bc_fixed(self, region: int, value) -> None:
+        v = _f64(np.atleast_1d(value))
+        _check(load().gr_field_bc_fixed(self._h, region, _p(v, C.c_double)))
+
+    def bc_zero_grad(self, region: int) -> None:
+        _check(load().gr_field_bc_zero_grad(self._h, region))
+
+
+def euler_init(mesh, rho, rhou, rhoe, mach, p, T):
+    m = _f64(mach)
+    _check(load().gr_euler_init(mesh._h, rho._h, rhou._h, rhoe._h, _p(m, C.c_double), p, T))
+
+
+def euler_bc_slip(mesh, rho, rhou, rhoe, region: str):
+    _check(load().gr_euler_bc_slip(mesh._h, rho._h, rhou._h, rhoe._h, region.encode()))
+
+
+def euler_bc_farfield(mesh, rho, rhou, rhoe, mach, p, T, region: str):
+    m = _f64(mach)
+    _check(load().gr_euler_bc_farfield(mesh._h, rho._h, rhou._h, rhoe._h, _p(m, C.c_double), p, T,
+                                       region.encode()))
+
+
+def euler_step_prepare(mesh, rho, rhou, rhoe, limiter: int = GR_LIM_MICHALAK):
+    _check(load().gr_euler_step_prepare(mesh._h, rho._h, rhou._h, rhoe._h, limiter))
+
+
+def euler_step(mesh, rho, rhou, rhoe, cfl: float, dt: Field | None = None, error: Field | None = None,
+               time_scheme: str = "steady"):
+    _check(load().gr_euler_step(mesh._h, rho._h, rhou._h, rhoe._h, cfl, time_scheme.encode(),
+                                dt._h if dt else None, error._h if error else None))
+
+
+def euler_residual(mesh, rho, rhou, rhoe) -> np.ndarray:
+    out = np.zeros(5 * mesh.n_cells)
+    _check(load().gr_euler_residual(mesh._h, rho._h, rhou._h, rhoe._h, _p(out, C.c_double)))
+    return out.reshape(-1, 5)
+
+
+# ---- solver -------------------------------------------------------------------------------
+
+class Solver:
+    """Device-resident Euler iteration (PhysicsEuler::step protocol)."""
+
+    def __init__(self, mesh: Mesh, dtype: int = GR_F64, cfl: float = 0.5, limiter: int = GR_LIM_MICHALAK,
+                 corrected: bool = False, use_graph: bool = True):
+        cfg = SolverConfig(dtype, cfl, limiter, int(corrected), int(use_graph))
+        h = C.c_void_p()
+        _check(load().gr_solver_create(mesh._h, C.byref(cfg), C.byref(h)))
+        self._h, self.mesh, self.dtype = h, mesh, dtype
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().gr_solver_free(h)
+            self._h = None
+
+    def set_bc(self, region: str, kind: int, mach=(0.0, 0.0, 0.0), p: float = 0.0, T: float = 0.0) -> None:
+        m = _f64(mach)
+        _check(load().gr_solver_set_bc(self._h, region.encode(), kind, _p(m, C.c_double), p, T))
+
+    def init_freestream(self, mach, p: float, T: float) -> None:
+        m = _f64(mach)
+        _check(load().gr_solver_init_freestream(self._h, _p(m, C.c_double), p, T))
+
+    def set_state(self, rho, rhou, rhoe) -> None:
+        r, u, e = _f64(rho), _f64(rhou).reshape(-1), _f64(rhoe)
+        _check(load().gr_solver_set_state(self._h, _p(r, C.c_double), _p(u, C.c_double), _p(e, C.c_double)))
+
+    def get_state(self):
+        s = self.mesh.sizes()
+        n = s.n_cells + s.n_ghosts
+        r, u, e = np.zeros(n), np.zeros(3 * n), np.zeros(n)
+        _check(load().gr_solver_get_state(self._h, _p(r, C.c_double), _p(u, C.c_double), _p(e, C.c_double)))
+        return r, u.reshape(-1, 3), e
+
+    def iterate(self, max_iter: int, tol: float = 0.0, reference: float = 0.0) -> SolveResult:
+        res = SolveResult()
+        _check(load().gr_solver_iterate(self._h, max_iter, tol, reference, C.byref(res)))
+        return res
+
+    def history(self, n: int) -> np.ndarray:
+        h = np.zeros(max(n, 1))
+        _check(load().gr_solver_history(self._h, _p(h, C.c_double), n))
+        return h[:n]
+
+    def residual_norm(self) -> float:
+        out = C.c_double()
+        _check(load().gr_solver_residual_norm(self._h, C.byref(out)))
+        return out.value
+
+    KERNELS = ("bc", "grad_lim", "flux_update", "residual_reduce")
+
+    def profile(self, n_iter: int) -> dict:
+        """Mean per-kernel duration (ms) over n_iter iterations, HIP events on the library's stream."""
+        ms = np.zeros(4)
+        _check(load().gr_solver_profile(self._h, n_iter, _p(ms, C.c_double)))
+        return dict(zip(self.KERNELS, ms.tolist()))
+
+    def kernel_bytes(self) -> dict:
+        b = np.zeros(4)
+        _check(load().gr_solver_kernel_bytes(self._h, _p(b, C.c_double)))
+        return dict(zip(self.KERNELS, b.tolist()))

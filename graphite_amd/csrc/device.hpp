@@ -1,0 +1,116 @@
+// Device-side types of libgraphite_hip.so: HIP error handling, device arrays, the mesh and
+// field handles and the plain-pointer views passed to kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "gr_internal.hpp"
+
+#define GR_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t gr_e_ = (call);                                                           \
+        if (gr_e_ != hipSuccess)                                                             \
+            throw gr::Error(GR_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(gr_e_)); \
+    } while (0)
+
+namespace gr {
+
+constexpr int kBlock = 256;   // threads per block of the per-cell kernels (4 wave64)
+
+inline unsigned blocks_for(size_t n, unsigned b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+// Owning device allocation.
+template <typename T>
+struct DevArray {
+    T *p = nullptr;
+    size_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray &) = delete;
+    DevArray &operator=(const DevArray &) = delete;
+    ~DevArray() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count) GR_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        require(count <= n, "device upload larger than allocation");
+        if (count) GR_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T *h, size_t count, hipStream_t s) const {
+        require(count <= n, "device download larger than allocation");
+        if (count) GR_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+inline size_t pad64(size_t n) { return (n + 63) & ~size_t(63); }
+
+// Mesh arrays as kernels see them.  Geometry is binary64, structure-of-arrays.
+struct MeshView {
+    uint32_t N, G, F;
+    const uint32_t *cfs;          // cell_faces_starts, N+G+1
+    const uint32_t *cf;           // cell_faces
+    const uint32_t *cc;           // cell_connects
+    const double *fcx, *fcy, *fcz;  // face centres
+    const double *fnx, *fny, *fnz;  // unit face normals (orientation as computed)
+    const double *fa;               // face areas
+    const double *cx, *cy, *cz;     // cell centres (N+G)
+    const double *cv;               // cell volumes (N+G)
+    const double *mat;              // LSQ matrices, 6 SoA blocks of N: xx,yy,zz,xy,xz,yz
+    size_t mat_stride;              // block stride of mat
+};
+
+constexpr int kMaxRegions = 32;
+
+}  // namespace gr
+
+struct gr_mesh {
+    gr::HostMesh h;
+    hipStream_t stream = nullptr;
+    bool on_device = false;
+    bool metrics_ready = false;
+    int corrected = 0;   // momentum reconstruction of gr_euler_step
+    gr::DevArray<uint32_t> fns, fn, cfs, cf, cc;
+    gr::DevArray<double> nodes;
+    gr::DevArray<double> face_geo;   // 7 blocks of pad64(F): cx,cy,cz,nx,ny,nz,a
+    gr::DevArray<double> cell_geo;   // 4 blocks of pad64(N+G): cx,cy,cz,v
+    gr::DevArray<double> mat;        // 6 blocks of pad64(N)
+    gr::DevArray<double> partials;   // reduction scratch, one per block of the largest launch
+    gr::DevArray<double> scalar_out; // one double
+    double *host_scalar = nullptr;   // pinned, one double
+    ~gr_mesh();
+    void ensure_device();            // stream + connectivity upload
+    gr::MeshView view() const;
+    size_t fstride() const { return gr::pad64(h.n_faces); }
+    size_t cstride() const { return gr::pad64((size_t)h.n_cells + h.n_ghosts); }
+    size_t mstride() const { return gr::pad64(h.n_cells); }
+};
+
+struct gr_field {
+    gr_mesh *mesh = nullptr;
+    uint32_t ncomp = 1;
+    int dtype = GR_F64;
+    std::string name;
+    bool grads = false;
+    size_t n = 0;        // cells + ghosts
+    size_t stride = 0;   // SoA block stride (elements)
+    gr::DevArray<char> value, next, grad, lim;
+    size_t esize() const { return (size_t)dtype; }
+    template <typename S>
+    S *val(int c) const { return (S *)value.p + (size_t)c * stride; }
+    template <typename S>
+    S *nxt(int c) const { return (S *)next.p + (size_t)c * stride; }
+    template <typename S>
+    S *grd(int c, int k) const { return (S *)grad.p + (size_t)(3 * c + k) * stride; }
+    template <typename S>
+    S *lmt(int c) const { return (S *)lim.p + (size_t)c * stride; }
+    void ensure_next();
+    void swap_next() { std::swap(value.p, next.p); }
+};

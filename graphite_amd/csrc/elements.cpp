@@ -1,0 +1,490 @@
+// Element lists in su2 form: C ABI, su2 text reader/writer, synthetic mesh generators.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <random>
+#include <sstream>
+#include <unordered_map>
+
+#include "gr_internal.hpp"
+
+namespace gr {
+
+bool elem_kind(uint32_t kind, ElemKind &out) {
+    // Same local face tables as the reference (src/meshio.cu:315-475); they decide the slot
+    // order of every cell and so must be identical.
+    static const ElemKind TRI = {3, 1, {{0, 1, 2, -1}}};
+    static const ElemKind QUAD = {4, 1, {{0, 1, 2, 3}}};
+    static const ElemKind TET = {4, 4, {{0, 2, 1, -1}, {0, 1, 3, -1}, {1, 2, 3, -1}, {0, 3, 2, -1}}};
+    static const ElemKind HEX = {8, 6, {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7},
+                                        {3, 2, 1, 0}, {4, 5, 6, 7}}};
+    static const ElemKind WEDGE = {6, 5, {{0, 1, 2, -1}, {3, 5, 4, -1}, {0, 2, 5, 3}, {1, 4, 5, 2},
+                                          {1, 0, 3, 4}}};
+    static const ElemKind PYR = {5, 5, {{0, 1, 2, 3}, {0, 4, 3, -1}, {0, 1, 4, -1}, {1, 2, 4, -1},
+                                        {2, 3, 4, -1}}};
+    switch (kind) {
+    case GR_TRI: out = TRI; return true;
+    case GR_QUAD: out = QUAD; return true;
+    case GR_TET: out = TET; return true;
+    case GR_HEX: out = HEX; return true;
+    case GR_WEDGE: out = WEDGE; return true;
+    case GR_PYRAMID: out = PYR; return true;
+    default: return false;
+    }
+}
+
+static void check_elements(uint32_t n_nodes, uint32_t n, const uint32_t *kinds,
+                           const uint32_t *starts, const uint32_t *nodes, bool surface) {
+    require(starts[0] == 0, "element starts must begin at 0");
+    for (uint32_t i = 0; i < n; ++i) {
+        ElemKind k;
+        require(elem_kind(kinds[i], k), "unknown element kind " + std::to_string(kinds[i]));
+        bool is_surface = kinds[i] == GR_TRI || kinds[i] == GR_QUAD;
+        require(is_surface == surface, surface ? "marker elements must be triangles or quads"
+                                               : "volume elements must be tet/hex/wedge/pyramid");
+        require(starts[i + 1] >= starts[i] && starts[i + 1] - starts[i] == (uint32_t)k.n_nodes,
+                "element " + std::to_string(i) + " has the wrong node count for its kind");
+        for (uint32_t q = starts[i]; q < starts[i + 1]; ++q)
+            require(nodes[q] < n_nodes, "element node index out of range");
+    }
+}
+
+// ---- boundary extraction used by the generators ----------------------------------------
+
+struct FaceKey {
+    uint32_t k[4];
+    bool operator==(const FaceKey &o) const { return std::memcmp(k, o.k, sizeof k) == 0; }
+};
+struct FaceKeyHash {
+    size_t operator()(const FaceKey &f) const {
+        uint64_t h = 1469598103934665603ull;
+        for (uint32_t v : f.k) { h ^= v; h *= 1099511628211ull; }
+        return (size_t)h;
+    }
+};
+
+static FaceKey sorted_key(const uint32_t *n, int cnt) {
+    FaceKey key;
+    for (int i = 0; i < 4; ++i) key.k[i] = i < cnt ? n[i] : 0xFFFFFFFFu;
+    std::sort(key.k, key.k + 4);
+    return key;
+}
+
+// Adds one marker per name; classify(face nodes, count) returns the marker index of a
+// boundary face (faces that belong to a single element).  Faces are emitted in element order.
+template <typename Classify>
+static void add_boundary_markers(gr_elements &e, const std::vector<std::string> &names,
+                                 Classify classify) {
+    std::unordered_map<FaceKey, uint32_t, FaceKeyHash> count;
+    count.reserve(e.kinds.size() * 6);
+    for (uint32_t el = 0; el < e.n_elems(); ++el) {
+        ElemKind k;
+        elem_kind(e.kinds[el], k);
+        for (int f = 0; f < k.n_faces; ++f) {
+            uint32_t fn[4];
+            int c = 0;
+            for (int j = 0; j < 4; ++j)
+                if (k.f[f][j] >= 0) fn[c++] = e.nodes[e.starts[el] + k.f[f][j]];
+            count[sorted_key(fn, c)]++;
+        }
+    }
+    e.markers.clear();
+    for (const auto &n : names) {
+        Marker m;
+        m.name = n;
+        m.starts.push_back(0);
+        e.markers.push_back(m);
+    }
+    for (uint32_t el = 0; el < e.n_elems(); ++el) {
+        ElemKind k;
+        elem_kind(e.kinds[el], k);
+        for (int f = 0; f < k.n_faces; ++f) {
+            uint32_t fn[4];
+            int c = 0;
+            for (int j = 0; j < 4; ++j)
+                if (k.f[f][j] >= 0) fn[c++] = e.nodes[e.starts[el] + k.f[f][j]];
+            if (count[sorted_key(fn, c)] != 1) continue;
+            int mk = classify(fn, c);
+            require(mk >= 0 && mk < (int)names.size(), "generator: unclassified boundary face");
+            Marker &m = e.markers[mk];
+            m.kinds.push_back(c == 3 ? GR_TRI : GR_QUAD);
+            for (int j = 0; j < c; ++j) m.nodes.push_back(fn[j]);
+            m.starts.push_back((uint32_t)m.nodes.size());
+        }
+    }
+}
+
+// Uniform double in [0,1) from the top 53 bits of a 64-bit draw (platform independent).
+static double u01(std::mt19937_64 &g) { return (double)(g() >> 11) * (1.0 / 9007199254740992.0); }
+
+void gen_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly, double lz, int tets,
+             double jitter, uint64_t seed, gr_elements &e) {
+    require(nx > 0 && ny > 0 && nz > 0, "box: cell counts must be positive");
+    require(lx > 0 && ly > 0 && lz > 0, "box: lengths must be positive");
+    require(jitter >= 0 && jitter < 0.25, "box: jitter must be in [0, 0.25)");
+    const uint64_t nn = (uint64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    const uint64_t ne = (uint64_t)nx * ny * nz * (tets ? 6 : 1);
+    require(nn < 0xFFFFFFF0ull && ne * 8 < 0xFFFFFFF0ull, "box: mesh too large for 32-bit indices");
+    auto nid = [&](uint32_t i, uint32_t j, uint32_t k) { return i + (nx + 1) * (j + (ny + 1) * k); };
+    const double hx = lx / nx, hy = ly / ny, hz = lz / nz;
+    std::mt19937_64 rng(seed);
+    e = gr_elements();
+    e.xyz.resize(3 * nn);
+    for (uint32_t k = 0; k <= nz; ++k)
+        for (uint32_t j = 0; j <= ny; ++j)
+            for (uint32_t i = 0; i <= nx; ++i) {
+                double x = i * hx, y = j * hy, z = k * hz;
+                bool interior = i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz;
+                if (interior && jitter > 0) {
+                    x += (2.0 * u01(rng) - 1.0) * jitter * hx;
+                    y += (2.0 * u01(rng) - 1.0) * jitter * hy;
+                    z += (2.0 * u01(rng) - 1.0) * jitter * hz;
+                }
+                uint32_t n = nid(i, j, k);
+                e.xyz[3 * n] = x;
+                e.xyz[3 * n + 1] = y;
+                e.xyz[3 * n + 2] = z;
+            }
+    e.starts.push_back(0);
+    for (uint32_t k = 0; k < nz; ++k)
+        for (uint32_t j = 0; j < ny; ++j)
+            for (uint32_t i = 0; i < nx; ++i) {
+                // hex corners in su2 order
+                uint32_t h[8] = {nid(i, j, k),         nid(i + 1, j, k),     nid(i + 1, j + 1, k),
+                                 nid(i, j + 1, k),     nid(i, j, k + 1),     nid(i + 1, j, k + 1),
+                                 nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)};
+                if (!tets) {
+                    e.kinds.push_back(GR_HEX);
+                    e.nodes.insert(e.nodes.end(), h, h + 8);
+                    e.starts.push_back((uint32_t)e.nodes.size());
+                    continue;
+                }
+                // Kuhn split along the diagonal h[0]-h[6]: one tet per axis permutation,
+                // path 0 -> e_a -> e_a+e_b -> (1,1,1).  Corner of offset (a,b,c) is
+                // corner[a + 2b + 4c] below.
+                uint32_t corner[8] = {h[0], h[1], h[3], h[2], h[4], h[5], h[7], h[6]};
+                static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2},
+                                                {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+                for (const auto &p : perms) {
+                    int o1 = 1 << p[0], o2 = o1 | (1 << p[1]);
+                    uint32_t t[4] = {corner[0], corner[o1], corner[o2], corner[7]};
+                    // orient so that the su2 volume is positive
+                    const double *a = &e.xyz[3 * t[0]], *b = &e.xyz[3 * t[1]];
+                    const double *c = &e.xyz[3 * t[2]], *d = &e.xyz[3 * t[3]];
+                    double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+                    double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+                    double w[3] = {d[0] - a[0], d[1] - a[1], d[2] - a[2]};
+                    double det = u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) +
+                                 u[2] * (v[0] * w[1] - v[1] * w[0]);
+                    if (det < 0) std::swap(t[1], t[2]);
+                    e.kinds.push_back(GR_TET);
+                    e.nodes.insert(e.nodes.end(), t, t + 4);
+                    e.starts.push_back((uint32_t)e.nodes.size());
+                }
+            }
+    const uint32_t sx = nx + 1, sxy = (nx + 1) * (ny + 1);
+    add_boundary_markers(e, {"xmin", "xmax", "ymin", "ymax", "zmin", "zmax"},
+                         [&](const uint32_t *fn, int c) {
+                             auto all = [&](auto pred) {
+                                 for (int q = 0; q < c; ++q)
+                                     if (!pred(fn[q])) return false;
+                                 return true;
+                             };
+                             if (all([&](uint32_t n) { return n % sx == 0; })) return 0;
+                             if (all([&](uint32_t n) { return n % sx == nx; })) return 1;
+                             if (all([&](uint32_t n) { return (n % sxy) / sx == 0; })) return 2;
+                             if (all([&](uint32_t n) { return (n % sxy) / sx == ny; })) return 3;
+                             if (all([&](uint32_t n) { return n / sxy == 0; })) return 4;
+                             if (all([&](uint32_t n) { return n / sxy == nz; })) return 5;
+                             return -1;
+                         });
+}
+
+void gen_ogrid(uint32_t nt, uint32_t nr, uint32_t nz, double r0, double r1, double ratio,
+               double dz, gr_elements &e) {
+    require(nt >= 3 && nr > 0 && nz > 0, "ogrid: need n_theta >= 3, n_r > 0, n_z > 0");
+    require(r0 > 0 && r1 > r0 && ratio > 0 && dz > 0, "ogrid: bad radii, ratio or dz");
+    const uint64_t nn = (uint64_t)nt * (nr + 1) * (nz + 1);
+    require(nn < 0xFFFFFFF0ull && (uint64_t)nt * nr * nz * 8 < 0xFFFFFFF0ull, "ogrid: too large");
+    std::vector<double> r(nr + 1);
+    if (std::fabs(ratio - 1.0) < 1e-14) {
+        for (uint32_t j = 0; j <= nr; ++j) r[j] = r0 + (r1 - r0) * j / nr;
+    } else {
+        double d0 = (r1 - r0) * (ratio - 1.0) / (std::pow(ratio, (double)nr) - 1.0);
+        for (uint32_t j = 0; j <= nr; ++j) r[j] = r0 + d0 * (std::pow(ratio, (double)j) - 1.0) / (ratio - 1.0);
+        r[nr] = r1;
+    }
+    auto nid = [&](uint32_t t, uint32_t j, uint32_t k) { return (t % nt) + nt * (j + (nr + 1) * k); };
+    e = gr_elements();
+    e.xyz.resize(3 * nn);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (uint32_t k = 0; k <= nz; ++k)
+        for (uint32_t j = 0; j <= nr; ++j)
+            for (uint32_t t = 0; t < nt; ++t) {
+                double th = two_pi * t / nt;
+                uint32_t n = nid(t, j, k);
+                e.xyz[3 * n] = r[j] * std::cos(th);
+                e.xyz[3 * n + 1] = r[j] * std::sin(th);
+                e.xyz[3 * n + 2] = k * dz;
+            }
+    e.starts.push_back(0);
+    for (uint32_t k = 0; k < nz; ++k)
+        for (uint32_t j = 0; j < nr; ++j)
+            for (uint32_t t = 0; t < nt; ++t) {
+                uint32_t h[8] = {nid(t, j, k),         nid(t + 1, j, k),     nid(t + 1, j + 1, k),
+                                 nid(t, j + 1, k),     nid(t, j, k + 1),     nid(t + 1, j, k + 1),
+                                 nid(t + 1, j + 1, k + 1), nid(t, j + 1, k + 1)};
+                e.kinds.push_back(GR_HEX);
+                e.nodes.insert(e.nodes.end(), h, h + 8);
+                e.starts.push_back((uint32_t)e.nodes.size());
+            }
+    const uint32_t per_layer = nt * (nr + 1);
+    add_boundary_markers(e, {"wall", "farfield", "zmin", "zmax"}, [&](const uint32_t *fn, int c) {
+        auto all = [&](auto pred) {
+            for (int q = 0; q < c; ++q)
+                if (!pred(fn[q])) return false;
+            return true;
+        };
+        if (all([&](uint32_t n) { return n / per_layer == 0; })) return 2;
+        if (all([&](uint32_t n) { return n / per_layer == nz; })) return 3;
+        if (all([&](uint32_t n) { return (n % per_layer) / nt == 0; })) return 0;
+        if (all([&](uint32_t n) { return (n % per_layer) / nt == nr; })) return 1;
+        return -1;
+    });
+}
+
+// ---- su2 text format (src/meshio.cu:771-897) -----------------------------------------------
+
+void read_su2(const char *path, gr_elements &e) {
+    std::ifstream in(path);
+    require(in.good(), std::string("cannot open su2 file ") + path, GR_ERR_IO);
+    e = gr_elements();
+    std::string tok;
+    int dim = 3;
+    auto read_elems = [&](uint32_t count, bool with_id, std::vector<uint32_t> &kinds,
+                          std::vector<uint32_t> &starts, std::vector<uint32_t> &nodes) {
+        starts.assign(1, 0);
+        for (uint32_t i = 0; i < count; ++i) {
+            uint32_t kind;
+            require((bool)(in >> kind), "su2: truncated element list", GR_ERR_IO);
+            ElemKind k;
+            require(elem_kind(kind, k), "su2: unknown element kind " + std::to_string(kind), GR_ERR_IO);
+            kinds.push_back(kind);
+            for (int j = 0; j < k.n_nodes; ++j) {
+                uint32_t n;
+                require((bool)(in >> n), "su2: truncated element", GR_ERR_IO);
+                nodes.push_back(n);
+            }
+            starts.push_back((uint32_t)nodes.size());
+            if (with_id) {
+                uint64_t id;
+                require((bool)(in >> id) && id == i, "su2: element id out of sequence", GR_ERR_IO);
+            }
+        }
+    };
+    while (in >> tok) {
+        if (tok == "NDIME=") {
+            require((bool)(in >> dim) && dim == 3, "su2: mesh must be 3D", GR_ERR_IO);
+        } else if (tok == "NPOIN=") {
+            uint64_t n;
+            require((bool)(in >> n), "su2: bad NPOIN", GR_ERR_IO);
+            // NPOIN= may carry a second count on the same line (gmsh writes one)
+            std::string rest;
+            std::getline(in, rest);
+            e.xyz.resize(3 * n);
+            for (uint64_t i = 0; i < n; ++i) {
+                std::string line;
+                require((bool)std::getline(in, line), "su2: truncated point list", GR_ERR_IO);
+                std::istringstream ls(line);
+                double x = 0, y = 0, z = 0;
+                require((bool)(ls >> x >> y >> z), "su2: bad point line", GR_ERR_IO);
+                e.xyz[3 * i] = x;
+                e.xyz[3 * i + 1] = y;
+                e.xyz[3 * i + 2] = z;
+            }
+        } else if (tok == "NELEM=") {
+            uint32_t n;
+            require((bool)(in >> n), "su2: bad NELEM", GR_ERR_IO);
+            read_elems(n, true, e.kinds, e.starts, e.nodes);
+        } else if (tok == "NMARK=") {
+            uint32_t nm;
+            require((bool)(in >> nm), "su2: bad NMARK", GR_ERR_IO);
+            for (uint32_t m = 0; m < nm; ++m) {
+                std::string t1, name, t2;
+                uint32_t cnt;
+                require((bool)(in >> t1 >> name) && t1 == "MARKER_TAG=", "su2: expected MARKER_TAG=", GR_ERR_IO);
+                require((bool)(in >> t2 >> cnt) && t2 == "MARKER_ELEMS=", "su2: expected MARKER_ELEMS=", GR_ERR_IO);
+                Marker mk;
+                mk.name = name;
+                read_elems(cnt, false, mk.kinds, mk.starts, mk.nodes);
+                e.markers.push_back(std::move(mk));
+            }
+        }
+    }
+    require(!e.kinds.empty(), "su2: no elements", GR_ERR_IO);
+    check_elements(e.n_nodes(), e.n_elems(), e.kinds.data(), e.starts.data(), e.nodes.data(), false);
+    for (const auto &m : e.markers)
+        check_elements(e.n_nodes(), (uint32_t)m.kinds.size(), m.kinds.data(), m.starts.data(),
+                       m.nodes.data(), true);
+}
+
+void write_su2(const gr_elements &e, const char *path) {
+    FILE *f = std::fopen(path, "w");
+    require(f != nullptr, std::string("cannot open for writing ") + path, GR_ERR_IO);
+    std::fprintf(f, "NDIME= 3\nNELEM= %u\n", e.n_elems());
+    for (uint32_t i = 0; i < e.n_elems(); ++i) {
+        std::fprintf(f, "%u", e.kinds[i]);
+        for (uint32_t q = e.starts[i]; q < e.starts[i + 1]; ++q) std::fprintf(f, " %u", e.nodes[q]);
+        std::fprintf(f, " %u\n", i);
+    }
+    std::fprintf(f, "NPOIN= %u\n", e.n_nodes());
+    for (uint32_t i = 0; i < e.n_nodes(); ++i)
+        std::fprintf(f, "%.17g %.17g %.17g %u\n", e.xyz[3 * i], e.xyz[3 * i + 1], e.xyz[3 * i + 2], i);
+    std::fprintf(f, "NMARK= %u\n", (uint32_t)e.markers.size());
+    for (const auto &m : e.markers) {
+        std::fprintf(f, "MARKER_TAG= %s\nMARKER_ELEMS= %u\n", m.name.c_str(), (uint32_t)m.kinds.size());
+        for (size_t i = 0; i < m.kinds.size(); ++i) {
+            std::fprintf(f, "%u", m.kinds[i]);
+            for (uint32_t q = m.starts[i]; q < m.starts[i + 1]; ++q) std::fprintf(f, " %u", m.nodes[q]);
+            std::fprintf(f, "\n");
+        }
+    }
+    bool ok = std::ferror(f) == 0;
+    ok = (std::fclose(f) == 0) && ok;
+    require(ok, std::string("write failed: ") + path, GR_ERR_IO);
+}
+
+}  // namespace gr
+
+// ---- C ABI -------------------------------------------------------------------------------
+
+using gr::guarded;
+using gr::require;
+
+extern "C" {
+
+int gr_elements_create(uint32_t n_nodes, const double *xyz, uint32_t n_elems,
+                       const uint32_t *kinds, const uint32_t *starts, const uint32_t *nodes,
+                       gr_elements **out) {
+    return guarded([&] {
+        require(out && xyz && kinds && starts && nodes, "null argument");
+        require(n_elems > 0, "no elements");
+        gr::check_elements(n_nodes, n_elems, kinds, starts, nodes, false);
+        auto *e = new gr_elements();
+        e->xyz.assign(xyz, xyz + 3 * (size_t)n_nodes);
+        e->kinds.assign(kinds, kinds + n_elems);
+        e->starts.assign(starts, starts + n_elems + 1);
+        e->nodes.assign(nodes, nodes + starts[n_elems]);
+        *out = e;
+    });
+}
+
+int gr_elements_add_marker(gr_elements *e, const char *name, uint32_t n, const uint32_t *kinds,
+                           const uint32_t *starts, const uint32_t *nodes) {
+    return guarded([&] {
+        require(e && name && kinds && starts && nodes, "null argument");
+        require(std::strlen(name) > 0 && std::strlen(name) < 128, "marker name must have 1..127 chars");
+        gr::check_elements(e->n_nodes(), n, kinds, starts, nodes, true);
+        gr::Marker m;
+        m.name = name;
+        m.kinds.assign(kinds, kinds + n);
+        m.starts.assign(starts, starts + n + 1);
+        m.nodes.assign(nodes, nodes + starts[n]);
+        e->markers.push_back(std::move(m));
+    });
+}
+
+int gr_elements_read_su2(const char *path, gr_elements **out) {
+    return guarded([&] {
+        require(path && out, "null argument");
+        auto *e = new gr_elements();
+        try {
+            gr::read_su2(path, *e);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+int gr_elements_write_su2(const gr_elements *e, const char *path) {
+    return guarded([&] {
+        require(e && path, "null argument");
+        gr::write_su2(*e, path);
+    });
+}
+
+int gr_elements_info_get(const gr_elements *e, gr_elements_info *out) {
+    return guarded([&] {
+        require(e && out, "null argument");
+        out->n_nodes = e->n_nodes();
+        out->n_elems = e->n_elems();
+        out->n_elem_nodes = (uint32_t)e->nodes.size();
+        out->n_markers = (uint32_t)e->markers.size();
+    });
+}
+
+int gr_elements_get(const gr_elements *e, double *xyz, uint32_t *kinds, uint32_t *starts,
+                    uint32_t *nodes) {
+    return guarded([&] {
+        require(e != nullptr, "null argument");
+        if (xyz) std::copy(e->xyz.begin(), e->xyz.end(), xyz);
+        if (kinds) std::copy(e->kinds.begin(), e->kinds.end(), kinds);
+        if (starts) std::copy(e->starts.begin(), e->starts.end(), starts);
+        if (nodes) std::copy(e->nodes.begin(), e->nodes.end(), nodes);
+    });
+}
+
+int gr_elements_get_marker(const gr_elements *e, uint32_t m, char name[128], uint32_t *n,
+                           uint32_t *n_nodes, uint32_t *kinds, uint32_t *starts, uint32_t *nodes) {
+    return guarded([&] {
+        require(e != nullptr, "null argument");
+        require(m < e->markers.size(), "marker index out of range");
+        const auto &mk = e->markers[m];
+        if (name) {
+            std::memset(name, 0, 128);
+            std::strncpy(name, mk.name.c_str(), 127);
+        }
+        if (n) *n = (uint32_t)mk.kinds.size();
+        if (n_nodes) *n_nodes = (uint32_t)mk.nodes.size();
+        if (kinds) std::copy(mk.kinds.begin(), mk.kinds.end(), kinds);
+        if (starts) std::copy(mk.starts.begin(), mk.starts.end(), starts);
+        if (nodes) std::copy(mk.nodes.begin(), mk.nodes.end(), nodes);
+    });
+}
+
+void gr_elements_free(gr_elements *e) { delete e; }
+
+int gr_gen_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly, double lz, int tets,
+               double jitter, uint64_t seed, gr_elements **out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        auto *e = new gr_elements();
+        try {
+            gr::gen_box(nx, ny, nz, lx, ly, lz, tets, jitter, seed, *e);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+int gr_gen_ogrid(uint32_t n_theta, uint32_t n_r, uint32_t n_z, double r0, double r1,
+                 double ratio, double dz, gr_elements **out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        auto *e = new gr_elements();
+        try {
+            gr::gen_ogrid(n_theta, n_r, n_z, r0, r1, ratio, dz, *e);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+}  // extern "C"

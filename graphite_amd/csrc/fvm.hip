@@ -1,0 +1,910 @@
+// Fields, FV operators, the Euler model entry points and the device-resident Euler solver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <utility>
+
+#include "fvm_kernels.hpp"
+
+namespace gr {
+
+__global__ __launch_bounds__(kBlock) void k_finalize(const double *partials, uint32_t n, double *out, SolverCtl *ctl,
+                                                     double *hist) {
+    if (ctl && ctl->stop) return;
+    double acc = 0;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) acc += partials[k];
+    const double tot = block_sum(acc);
+    if (threadIdx.x != 0) return;
+    if (!ctl) {
+        if (out) out[0] = tot;
+        return;
+    }
+    const int it = ctl->iter;
+    if (it < ctl->hist_cap) hist[it] = tot;
+    if (it == 0 && ctl->ref <= 0) ctl->ref = tot;
+    ctl->last = tot;
+    ctl->ratio = tot / ctl->ref;
+    ctl->iter = it + 1;
+    if (!isfinite(tot)) {
+        ctl->diverged = 1;
+        ctl->stop = 1;
+    } else if (ctl->tol > 0 && ctl->ratio < ctl->tol) {
+        ctl->converged = 1;
+        ctl->stop = 1;
+    }
+    if (ctl->iter >= ctl->max_iter) ctl->stop = 1;
+}
+
+template <typename S>
+__global__ void k_fill(S *p, size_t n, S v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// fvmScalarBcFixed / fvmScalarBcZeroGrad / fvmVectorBcZeroGrad over ghosts [g0, g0+count).
+template <typename S>
+__global__ void k_bc_generic(MeshView m, S *val, size_t stride, int ncomp, uint32_t g0, uint32_t count, int zero_grad,
+                             double v0, double v1, double v2) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t c = m.N + g0 + t;
+    const uint32_t o = m.cc[m.cfs[c]];
+    const double v[3] = {v0, v1, v2};
+    for (int k = 0; k < ncomp; ++k) val[k * stride + c] = zero_grad ? val[k * stride + o] : (S)v[k];
+}
+
+}  // namespace gr
+
+using gr::guarded;
+using gr::require;
+
+void gr_field::ensure_next() {
+    if (!next.p) {
+        next.alloc(value.n);
+        GR_HIP(hipMemsetAsync(next.p, 0, next.n, mesh->stream));
+    }
+}
+
+namespace {
+
+void check_ready(gr_mesh *m) {
+    require(m != nullptr, "null mesh");
+    require(m->metrics_ready, "mesh metrics not computed (gr_mesh_compute_metrics)");
+}
+
+void check_euler_fields(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, bool need_grads) {
+    check_ready(m);
+    require(rho && rhou && rhoe, "null field");
+    require(rho->mesh == m && rhou->mesh == m && rhoe->mesh == m, "fields belong to another mesh");
+    require(rho->ncomp == 1 && rhou->ncomp == 3 && rhoe->ncomp == 1, "rho, rhoe must be scalar and rhou vector fields");
+    require(rho->dtype == rhou->dtype && rho->dtype == rhoe->dtype, "rho, rhou, rhoe must share a dtype");
+    if (need_grads) require(rho->grads && rhou->grads && rhoe->grads, "fields need gradients and limiters");
+}
+
+template <typename S>
+gr::EulerArgs<S> euler_args(gr_field *rho, gr_field *rhou, gr_field *rhoe) {
+    gr::EulerArgs<S> a{};
+    gr_field *f[5] = {rho, rhou, rhou, rhou, rhoe};
+    int comp[5] = {0, 0, 1, 2, 0};
+    for (int c = 0; c < 5; ++c) {
+        a.u[c] = f[c]->val<S>(comp[c]);
+        a.un[c] = f[c]->next.p ? f[c]->nxt<S>(comp[c]) : nullptr;
+        if (f[c]->grads) {
+            a.l[c] = f[c]->lmt<S>(comp[c]);
+            for (int k = 0; k < 3; ++k) a.g[3 * c + k] = f[c]->grd<S>(comp[c], k);
+        }
+    }
+    return a;
+}
+
+template <typename S>
+gr::GradLimArgs<S, 5> euler_gradlim_args(gr_field *rho, gr_field *rhou, gr_field *rhoe) {
+    gr::GradLimArgs<S, 5> a{};
+    gr_field *f[5] = {rho, rhou, rhou, rhou, rhoe};
+    int comp[5] = {0, 0, 1, 2, 0};
+    for (int c = 0; c < 5; ++c) {
+        a.u[c] = f[c]->val<S>(comp[c]);
+        a.l[c] = f[c]->lmt<S>(comp[c]);
+        for (int k = 0; k < 3; ++k) a.g[3 * c + k] = f[c]->grd<S>(comp[c], k);
+    }
+    return a;
+}
+
+template <typename S, int NC>
+gr::GradLimArgs<S, NC> field_gradlim_args(gr_field *f) {
+    gr::GradLimArgs<S, NC> a{};
+    for (int c = 0; c < NC; ++c) {
+        a.u[c] = f->val<S>(c);
+        if (f->grads) {
+            a.l[c] = f->lmt<S>(c);
+            for (int k = 0; k < 3; ++k) a.g[3 * c + k] = f->grd<S>(c, k);
+        }
+    }
+    return a;
+}
+
+template <typename S>
+void fill(S *p, size_t n, S v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(gr::k_fill<S>, dim3(gr::blocks_for(n)), dim3(gr::kBlock), 0, s, p, n, v);
+}
+
+// Freestream conservative state (euler_init, src/models/euler.cu:385-409), binary64.
+void freestream(const double mach[3], double p, double T, double out[5]) {
+    const double rho_v = p / T;
+    const double c = std::sqrt((1.4 * p) / rho_v);
+    const double vel[3] = {mach[0] * c, mach[1] * c, mach[2] * c};
+    const double rhoe_v = p / 0.4 + (0.5 * rho_v) * ((vel[0] * vel[0] + vel[1] * vel[1]) + vel[2] * vel[2]);
+    out[0] = rho_v;
+    for (int k = 0; k < 3; ++k) out[1 + k] = vel[k] * rho_v;
+    out[4] = rhoe_v;
+}
+
+gr::BcTable one_region_table(const gr_mesh *m, uint32_t region, int kind, const double *mach, double p, double T) {
+    gr::BcTable t{};
+    t.n = 1;
+    t.r[0].begin = m->h.ghost_cell_starts[region];
+    t.r[0].end = m->h.ghost_cell_starts[region + 1];
+    t.r[0].kind = kind;
+    for (int k = 0; k < 3; ++k) t.r[0].mach[k] = mach ? mach[k] : 0.0;
+    t.r[0].p = p;
+    t.r[0].T = T;
+    return t;
+}
+
+uint32_t region_of(const gr_mesh *m, const char *name) {
+    require(name != nullptr, "null region name");
+    for (uint32_t r = 0; r < m->h.n_regions; ++r)
+        if (m->h.region_names[r] == name) return r;
+    throw gr::Error(GR_ERR_NOT_FOUND, std::string("region '") + name + "' not found in mesh");
+}
+
+template <typename S>
+void launch_bc(gr_mesh *m, gr::EulerArgs<S> a, const gr::BcTable &t, uint32_t g0, uint32_t count,
+               const gr::SolverCtl *ctl) {
+    if (!count) return;
+    hipLaunchKernelGGL(gr::k_euler_bc<S>, dim3(gr::blocks_for(count)), dim3(gr::kBlock), 0, m->stream, m->view(), a, t,
+                       g0, count, ctl);
+}
+
+template <typename S>
+void launch_flux(gr_mesh *m, int corrected, gr::EulerArgs<S> a, double cfl, S *err, S *dt, double *resid,
+                 double *partials, int copy_ghosts, const gr::SolverCtl *ctl) {
+    const size_t n = copy_ghosts ? (size_t)m->h.n_cells + m->h.n_ghosts : m->h.n_cells;
+    if (corrected)
+        hipLaunchKernelGGL((gr::k_euler_flux<S, true>), dim3(gr::blocks_for(n)), dim3(gr::kBlock), 0, m->stream,
+                           m->view(), a, cfl, err, dt, resid, partials, copy_ghosts, ctl);
+    else
+        hipLaunchKernelGGL((gr::k_euler_flux<S, false>), dim3(gr::blocks_for(n)), dim3(gr::kBlock), 0, m->stream,
+                           m->view(), a, cfl, err, dt, resid, partials, copy_ghosts, ctl);
+}
+
+template <typename S>
+void launch_gradlim5(gr_mesh *m, const gr::GradLimArgs<S, 5> &a, int limiter, const gr::SolverCtl *ctl) {
+    hipLaunchKernelGGL((gr::k_grad_lim<S, 5, true, true>), dim3(gr::blocks_for(m->h.n_cells)), dim3(gr::kBlock), 0,
+                       m->stream, m->view(), a, limiter, ctl);
+}
+
+double finalize_sum(gr_mesh *m, uint32_t nparts) {
+    hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, m->partials.p, nparts,
+                       m->scalar_out.p, (gr::SolverCtl *)nullptr, (double *)nullptr);
+    GR_HIP(hipGetLastError());
+    GR_HIP(hipMemcpyAsync(m->host_scalar, m->scalar_out.p, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    GR_HIP(hipStreamSynchronize(m->stream));
+    return *m->host_scalar;
+}
+
+}  // namespace
+
+// ---- fields --------------------------------------------------------------------------------
+
+extern "C" {
+
+int gr_field_create(gr_mesh *m, uint32_t n_comp, int dtype, const char *name, int alloc_grads, gr_field **out) {
+    return guarded([&] {
+        require(m && out, "null argument");
+        require(n_comp == 1 || n_comp == 3, "fields have 1 or 3 components");
+        require(dtype == GR_F64 || dtype == GR_F32, "dtype must be GR_F64 or GR_F32");
+        m->ensure_device();
+        auto *f = new gr_field();
+        try {
+            f->mesh = m;
+            f->ncomp = n_comp;
+            f->dtype = dtype;
+            f->name = name ? name : "";
+            f->grads = alloc_grads != 0;
+            f->n = (size_t)m->h.n_cells + m->h.n_ghosts;
+            f->stride = gr::pad64(f->n);
+            const size_t blk = f->stride * f->esize();
+            f->value.alloc(n_comp * blk);
+            GR_HIP(hipMemsetAsync(f->value.p, 0, f->value.n, m->stream));
+            if (f->grads) {
+                f->grad.alloc(3 * n_comp * blk);
+                f->lim.alloc(n_comp * blk);
+                GR_HIP(hipMemsetAsync(f->grad.p, 0, f->grad.n, m->stream));
+                GR_HIP(hipMemsetAsync(f->lim.p, 0, f->lim.n, m->stream));
+            }
+            GR_HIP(hipStreamSynchronize(m->stream));
+        } catch (...) {
+            delete f;
+            throw;
+        }
+        *out = f;
+    });
+}
+
+void gr_field_free(gr_field *f) {
+    if (f && f->mesh && f->mesh->stream) (void)hipStreamSynchronize(f->mesh->stream);
+    delete f;
+}
+
+int gr_field_upload(gr_field *f, const double *value) {
+    return guarded([&] {
+        require(f && value, "null argument");
+        hipStream_t s = f->mesh->stream;
+        if (f->dtype == GR_F64) {
+            std::vector<double> soa(f->ncomp * f->stride, 0.0);
+            for (size_t i = 0; i < f->n; ++i)
+                for (uint32_t c = 0; c < f->ncomp; ++c) soa[c * f->stride + i] = value[f->ncomp * i + c];
+            GR_HIP(hipMemcpyAsync(f->value.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice, s));
+            GR_HIP(hipStreamSynchronize(s));
+        } else {
+            std::vector<float> soa(f->ncomp * f->stride, 0.0f);
+            for (size_t i = 0; i < f->n; ++i)
+                for (uint32_t c = 0; c < f->ncomp; ++c) soa[c * f->stride + i] = (float)value[f->ncomp * i + c];
+            GR_HIP(hipMemcpyAsync(f->value.p, soa.data(), soa.size() * 4, hipMemcpyHostToDevice, s));
+            GR_HIP(hipStreamSynchronize(s));
+        }
+    });
+}
+
+int gr_field_download(const gr_field *f, double *value, double *gradient, double *limiter) {
+    return guarded([&] {
+        require(f != nullptr, "null argument");
+        require(f->grads || (!gradient && !limiter), "field has no gradient/limiter arrays");
+        hipStream_t s = f->mesh->stream;
+        auto get = [&](const gr::DevArray<char> &arr, int blocks, double *dst, int per_cell) {
+            if (!dst) return;
+            std::vector<char> raw(arr.n);
+            arr.download(raw.data(), raw.size(), s);
+            GR_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < f->n; ++i)
+                for (int b = 0; b < blocks; ++b) {
+                    size_t idx = (size_t)b * f->stride + i;
+                    dst[(size_t)per_cell * i + b] =
+                        f->dtype == GR_F64 ? ((const double *)raw.data())[idx] : (double)((const float *)raw.data())[idx];
+                }
+        };
+        get(f->value, (int)f->ncomp, value, (int)f->ncomp);
+        get(f->grad, 3 * (int)f->ncomp, gradient, 3 * (int)f->ncomp);
+        get(f->lim, (int)f->ncomp, limiter, (int)f->ncomp);
+    });
+}
+
+int gr_field_init(gr_field *f, const double *v) {
+    return guarded([&] {
+        require(f && v, "null argument");
+        hipStream_t s = f->mesh->stream;
+        for (uint32_t c = 0; c < f->ncomp; ++c) {
+            if (f->dtype == GR_F64) {
+                fill<double>(f->val<double>(c), f->n, v[c], s);
+                if (f->grads) {
+                    fill<double>(f->lmt<double>(c), f->n, 1.0, s);
+                    for (int k = 0; k < 3; ++k) fill<double>(f->grd<double>(c, k), f->n, 0.0, s);
+                }
+            } else {
+                fill<float>(f->val<float>(c), f->n, (float)v[c], s);
+                if (f->grads) {
+                    fill<float>(f->lmt<float>(c), f->n, 1.0f, s);
+                    for (int k = 0; k < 3; ++k) fill<float>(f->grd<float>(c, k), f->n, 0.0f, s);
+                }
+            }
+        }
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int gr_field_reduce_sum(const gr_field *f, double *out) {
+    return guarded([&] {
+        require(f && out, "null argument");
+        require(f->ncomp == 1, "reduce_sum is defined for scalar fields");
+        gr_mesh *m = f->mesh;
+        const uint32_t N = m->h.n_cells;
+        const uint32_t nb = gr::blocks_for(N);
+        require(nb <= m->partials.n, "reduction scratch too small");
+        if (f->dtype == GR_F64)
+            hipLaunchKernelGGL(gr::k_field_partial_sum<double>, dim3(nb), dim3(gr::kBlock), 0, m->stream,
+                               f->val<double>(0), N, m->partials.p);
+        else
+            hipLaunchKernelGGL(gr::k_field_partial_sum<float>, dim3(nb), dim3(gr::kBlock), 0, m->stream,
+                               f->val<float>(0), N, m->partials.p);
+        *out = finalize_sum(m, nb);
+    });
+}
+
+static int field_gradlim(gr_field *f, bool grad, bool lim, int kind) {
+    return guarded([&] {
+        require(f != nullptr, "null argument");
+        check_ready(f->mesh);
+        require(f->grads, "field has no gradient/limiter arrays");
+        require(kind >= 0 && kind <= 2, "unknown limiter");
+        gr_mesh *m = f->mesh;
+        const dim3 grid(gr::blocks_for(m->h.n_cells)), blk(gr::kBlock);
+        const gr::SolverCtl *nc = nullptr;
+        if (!m->h.n_cells) return;
+#define GR_GL(S, NC)                                                                                         \
+    do {                                                                                                     \
+        auto a = field_gradlim_args<S, NC>(f);                                                               \
+        if (grad)                                                                                            \
+            hipLaunchKernelGGL((gr::k_grad_lim<S, NC, true, false>), grid, blk, 0, m->stream, m->view(), a, kind, nc); \
+        if (lim)                                                                                             \
+            hipLaunchKernelGGL((gr::k_grad_lim<S, NC, false, true>), grid, blk, 0, m->stream, m->view(), a, kind, nc); \
+    } while (0)
+        if (f->dtype == GR_F64) {
+            if (f->ncomp == 1) GR_GL(double, 1);
+            else GR_GL(double, 3);
+        } else {
+            if (f->ncomp == 1) GR_GL(float, 1);
+            else GR_GL(float, 3);
+        }
+#undef GR_GL
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+int gr_field_compute_gradient(gr_field *f) { return field_gradlim(f, true, false, 0); }
+int gr_field_compute_limiter(gr_field *f, int limiter) { return field_gradlim(f, false, true, limiter); }
+
+int gr_field_gradlim_bc(gr_field *f) {
+    return guarded([&] {
+        require(f != nullptr, "null argument");
+        check_ready(f->mesh);
+        require(f->grads, "field has no gradient/limiter arrays");
+        gr_mesh *m = f->mesh;
+        if (!m->h.n_ghosts) return;
+        const dim3 grid(gr::blocks_for(m->h.n_ghosts)), blk(gr::kBlock);
+        if (f->dtype == GR_F64) {
+            if (f->ncomp == 1)
+                hipLaunchKernelGGL((gr::k_gradlim_ghost<double, 1>), grid, blk, 0, m->stream, m->view(),
+                                   field_gradlim_args<double, 1>(f));
+            else
+                hipLaunchKernelGGL((gr::k_gradlim_ghost<double, 3>), grid, blk, 0, m->stream, m->view(),
+                                   field_gradlim_args<double, 3>(f));
+        } else {
+            if (f->ncomp == 1)
+                hipLaunchKernelGGL((gr::k_gradlim_ghost<float, 1>), grid, blk, 0, m->stream, m->view(),
+                                   field_gradlim_args<float, 1>(f));
+            else
+                hipLaunchKernelGGL((gr::k_gradlim_ghost<float, 3>), grid, blk, 0, m->stream, m->view(),
+                                   field_gradlim_args<float, 3>(f));
+        }
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+static int field_bc(gr_field *f, uint32_t region, const double *v, int zero_grad) {
+    return guarded([&] {
+        require(f != nullptr, "null argument");
+        check_ready(f->mesh);
+        gr_mesh *m = f->mesh;
+        require(region < m->h.n_regions, "region index out of range");
+        require(zero_grad || v, "null value");
+        const uint32_t g0 = m->h.ghost_cell_starts[region], cnt = m->h.ghost_cell_starts[region + 1] - g0;
+        if (!cnt) return;
+        double vv[3] = {0, 0, 0};
+        if (v)
+            for (uint32_t k = 0; k < f->ncomp; ++k) vv[k] = v[k];
+        if (f->dtype == GR_F64)
+            hipLaunchKernelGGL(gr::k_bc_generic<double>, dim3(gr::blocks_for(cnt)), dim3(gr::kBlock), 0, m->stream,
+                               m->view(), f->val<double>(0), f->stride, (int)f->ncomp, g0, cnt, zero_grad, vv[0],
+                               vv[1], vv[2]);
+        else
+            hipLaunchKernelGGL(gr::k_bc_generic<float>, dim3(gr::blocks_for(cnt)), dim3(gr::kBlock), 0, m->stream,
+                               m->view(), f->val<float>(0), f->stride, (int)f->ncomp, g0, cnt, zero_grad, vv[0],
+                               vv[1], vv[2]);
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+int gr_field_bc_fixed(gr_field *f, uint32_t region, const double *v) { return field_bc(f, region, v, 0); }
+int gr_field_bc_zero_grad(gr_field *f, uint32_t region) { return field_bc(f, region, nullptr, 1); }
+
+// ---- Euler model --------------------------------------------------------------------------
+
+int gr_euler_init(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, const double mach[3], double pressure,
+                  double temperature) {
+    return guarded([&] {
+        require(mach != nullptr, "null mach");
+        check_euler_fields(m, rho, rhou, rhoe, false);
+        double fs[5];
+        freestream(mach, pressure, temperature, fs);
+        gr_field *f[5] = {rho, rhou, rhou, rhou, rhoe};
+        int comp[5] = {0, 0, 1, 2, 0};
+        for (int c = 0; c < 5; ++c) {
+            if (rho->dtype == GR_F64) fill<double>(f[c]->val<double>(comp[c]), f[c]->n, fs[c], m->stream);
+            else fill<float>(f[c]->val<float>(comp[c]), f[c]->n, (float)fs[c], m->stream);
+        }
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+static int euler_bc(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, int kind, const double *mach, double p,
+                    double T, const char *region) {
+    return guarded([&] {
+        check_euler_fields(m, rho, rhou, rhoe, false);
+        if (kind == GR_BC_FARFIELD) require(mach != nullptr, "null mach");
+        const uint32_t r = region_of(m, region);
+        gr::BcTable t = one_region_table(m, r, kind, mach, p, T);
+        const uint32_t g0 = t.r[0].begin, cnt = t.r[0].end - t.r[0].begin;
+        if (rho->dtype == GR_F64) launch_bc<double>(m, euler_args<double>(rho, rhou, rhoe), t, g0, cnt, nullptr);
+        else launch_bc<float>(m, euler_args<float>(rho, rhou, rhoe), t, g0, cnt, nullptr);
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+int gr_euler_bc_slip(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, const char *region) {
+    return euler_bc(m, rho, rhou, rhoe, GR_BC_SLIP, nullptr, 0, 0, region);
+}
+
+int gr_euler_bc_farfield(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, const double mach[3],
+                         double pressure, double temperature, const char *region) {
+    return euler_bc(m, rho, rhou, rhoe, GR_BC_FARFIELD, mach, pressure, temperature, region);
+}
+
+int gr_euler_step_prepare(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, int limiter) {
+    return guarded([&] {
+        check_euler_fields(m, rho, rhou, rhoe, true);
+        require(limiter >= 0 && limiter <= 2, "unknown limiter");
+        if (!m->h.n_cells) return;
+        auto ghost = [&](auto a, auto tag) {
+            using S = decltype(tag);
+            if (m->h.n_ghosts)
+                hipLaunchKernelGGL((gr::k_gradlim_ghost<S, 5>), dim3(gr::blocks_for(m->h.n_ghosts)), dim3(gr::kBlock),
+                                   0, m->stream, m->view(), a);
+        };
+        if (rho->dtype == GR_F64) {
+            auto a = euler_gradlim_args<double>(rho, rhou, rhoe);
+            launch_gradlim5<double>(m, a, limiter, nullptr);
+            ghost(a, 0.0);
+        } else {
+            auto a = euler_gradlim_args<float>(rho, rhou, rhoe);
+            launch_gradlim5<float>(m, a, limiter, nullptr);
+            ghost(a, 0.0f);
+        }
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+int gr_euler_step(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, double cfl, const char *time_scheme,
+                  gr_field *dt, gr_field *error) {
+    return guarded([&] {
+        check_euler_fields(m, rho, rhou, rhoe, true);
+        require(time_scheme != nullptr, "null time scheme");
+        // The reference only implements "steady" (src/models/euler.cu:661); other schemes
+        // leave the state unchanged there, and are rejected here.
+        require(std::strcmp(time_scheme, "steady") == 0, "only the 'steady' time scheme exists");
+        for (gr_field *o : {dt, error})
+            if (o) require(o->mesh == m && o->ncomp == 1 && o->dtype == rho->dtype, "dt/error must be scalar fields of the state dtype");
+        rho->ensure_next();
+        rhou->ensure_next();
+        rhoe->ensure_next();
+        if (rho->dtype == GR_F64)
+            launch_flux<double>(m, m->corrected, euler_args<double>(rho, rhou, rhoe), cfl,
+                                error ? error->val<double>(0) : nullptr, dt ? dt->val<double>(0) : nullptr, nullptr,
+                                nullptr, 1, nullptr);
+        else
+            launch_flux<float>(m, m->corrected, euler_args<float>(rho, rhou, rhoe), cfl,
+                               error ? error->val<float>(0) : nullptr, dt ? dt->val<float>(0) : nullptr, nullptr,
+                               nullptr, 1, nullptr);
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+        rho->swap_next();
+        rhou->swap_next();
+        rhoe->swap_next();
+    });
+}
+
+int gr_euler_residual(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, double *resid) {
+    return guarded([&] {
+        check_euler_fields(m, rho, rhou, rhoe, true);
+        require(resid != nullptr, "null output");
+        rho->ensure_next();
+        rhou->ensure_next();
+        rhoe->ensure_next();
+        gr::DevArray<double> r;
+        r.alloc(5 * std::max<size_t>(1, m->h.n_cells));
+        if (rho->dtype == GR_F64)
+            launch_flux<double>(m, m->corrected, euler_args<double>(rho, rhou, rhoe), 1.0, nullptr, nullptr, r.p,
+                                nullptr, 0, nullptr);
+        else
+            launch_flux<float>(m, m->corrected, euler_args<float>(rho, rhou, rhoe), 1.0, nullptr, nullptr, r.p,
+                               nullptr, 0, nullptr);
+        GR_HIP(hipGetLastError());
+        r.download(resid, 5 * (size_t)m->h.n_cells, m->stream);
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+int gr_euler_set_corrected_reconstruction(gr_mesh *m, int on) {
+    return guarded([&] {
+        require(m != nullptr, "null mesh");
+        m->corrected = on ? 1 : 0;
+    });
+}
+
+}  // extern "C"
+
+// ---- solver --------------------------------------------------------------------------------
+
+struct gr_solver {
+    gr_mesh *mesh = nullptr;
+    gr_solver_config cfg{};
+    size_t stride = 0;
+    gr::DevArray<char> ubuf[2], grad, lim;
+    int cur = 0;         // buffer holding the current interior state
+    int ghost_src = 0;   // buffer holding the latest ghost values
+    struct Bc { bool set = false; int kind = 0; double mach[3] = {0, 0, 0}, p = 0, T = 0; };
+    std::vector<Bc> bcs;
+    gr::DevArray<gr::SolverCtl> ctl;
+    gr::DevArray<double> hist;
+    int hist_cap = 0;
+    gr::SolverCtl *ctl_host = nullptr;   // pinned: [0] upload, [1],[2] polling
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int graph_iters = 10;   // iterations per captured graph (even)
+    int last_iters = 0;
+
+    ~gr_solver() {
+        if (mesh && mesh->stream) (void)hipStreamSynchronize(mesh->stream);
+        drop_graphs();
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (ctl_host) (void)hipHostFree(ctl_host);
+    }
+    void drop_graphs() {
+        for (auto &g : exec) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+    size_t es() const { return (size_t)cfg.dtype; }
+    template <typename S> S *u(int b, int c) const { return (S *)ubuf[b].p + (size_t)c * stride; }
+    template <typename S> S *g(int k) const { return (S *)grad.p + (size_t)k * stride; }
+    template <typename S> S *l(int c) const { return (S *)lim.p + (size_t)c * stride; }
+
+    template <typename S> gr::EulerArgs<S> args(int b) const {
+        gr::EulerArgs<S> a{};
+        for (int c = 0; c < 5; ++c) {
+            a.u[c] = u<S>(b, c);
+            a.un[c] = u<S>(1 - b, c);
+            a.l[c] = l<S>(c);
+        }
+        for (int k = 0; k < 15; ++k) a.g[k] = g<S>(k);
+        return a;
+    }
+    template <typename S> gr::GradLimArgs<S, 5> gl(int b) const {
+        gr::GradLimArgs<S, 5> a{};
+        for (int c = 0; c < 5; ++c) {
+            a.u[c] = u<S>(b, c);
+            a.l[c] = l<S>(c);
+        }
+        for (int k = 0; k < 15; ++k) a.g[k] = g<S>(k);
+        return a;
+    }
+    gr::BcTable table() const {
+        gr::BcTable t{};
+        for (uint32_t r = 0; r < mesh->h.n_regions; ++r) {
+            if (!bcs[r].set) continue;
+            auto &R = t.r[t.n++];
+            R.begin = mesh->h.ghost_cell_starts[r];
+            R.end = mesh->h.ghost_cell_starts[r + 1];
+            R.kind = bcs[r].kind;
+            for (int k = 0; k < 3; ++k) R.mach[k] = bcs[r].mach[k];
+            R.p = bcs[r].p;
+            R.T = bcs[r].T;
+        }
+        return t;
+    }
+    // One iteration reading buffer b: BCs, gradient+limiter, flux+update into 1-b, norm.
+    // ev (optional) receives 5 events recorded before, between and after the kernels.
+    template <typename S> void enqueue_iteration(int b, gr::SolverCtl *c, hipEvent_t *ev) {
+        gr_mesh *m = mesh;
+        const gr::BcTable t = table();
+        if (ev) GR_HIP(hipEventRecord(ev[0], m->stream));
+        launch_bc<S>(m, args<S>(b), t, 0, m->h.n_ghosts, c);
+        if (ev) GR_HIP(hipEventRecord(ev[1], m->stream));
+        launch_gradlim5<S>(m, gl<S>(b), cfg.limiter, c);
+        if (ev) GR_HIP(hipEventRecord(ev[2], m->stream));
+        launch_flux<S>(m, cfg.corrected, args<S>(b), cfg.cfl, nullptr, nullptr, nullptr, m->partials.p, 0, c);
+        if (ev) GR_HIP(hipEventRecord(ev[3], m->stream));
+        if (c)   // without a control block the caller reduces the partials itself
+            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, m->partials.p,
+                               gr::blocks_for(m->h.n_cells), (double *)nullptr, c, hist.p);
+        if (ev) GR_HIP(hipEventRecord(ev[4], m->stream));
+    }
+    void enqueue(int b, gr::SolverCtl *c, hipEvent_t *ev = nullptr) {
+        if (cfg.dtype == GR_F64) enqueue_iteration<double>(b, c, ev);
+        else enqueue_iteration<float>(b, c, ev);
+    }
+    // Algorithmic bytes per launch: each array element a kernel reads or writes, once.
+    void kernel_bytes(double out[4]) const {
+        const double N = mesh->h.n_cells, G = mesh->h.n_ghosts, F = mesh->h.n_faces;
+        const double slots = mesh->h.cell_faces_starts[mesh->h.n_cells];   // interior cells' face slots
+        const double su = (double)cfg.dtype, sg = 8.0;
+        // BC: ghost slot (face, owner) and starts, face centre+normal, owner centre, owner U
+        // read, ghost U written.
+        out[0] = 4.0 * (3 * G) + sg * (6 * G + 3 * G) + su * (5 * G + 5 * G);
+        // gradient+limiter: U of cells and ghosts, centres of cells and ghosts, LSQ matrix
+        // (6), face centres, CSR (starts, faces, neighbours); gradient (15) and limiter (5) out.
+        out[1] = su * (5 * (N + G)) + sg * (3 * (N + G) + 6 * N + 3 * F) + 4.0 * (N + 1 + 2 * slots) +
+                 su * (20 * N);
+        // flux+update: U of cells and ghosts, gradient and limiter of cells, centres of cells
+        // and ghosts, volumes, face centre+normal+area, CSR; new U out.
+        out[2] = su * (5 * (N + G) + 20 * N) + sg * (3 * (N + G) + N + 7 * F) + 4.0 * (N + 1 + 2 * slots) +
+                 su * (5 * N);
+        out[3] = 8.0 * std::ceil(N / gr::kBlock);
+    }
+    hipGraphExec_t graph(int b) {
+        if (exec[b]) return exec[b];
+        hipGraph_t g = nullptr;
+        GR_HIP(hipStreamBeginCapture(mesh->stream, hipStreamCaptureModeRelaxed));
+        try {
+            for (int k = 0; k < graph_iters; ++k) enqueue((b + k) % 2, ctl.p, nullptr);
+        } catch (...) {
+            (void)hipStreamEndCapture(mesh->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        GR_HIP(hipStreamEndCapture(mesh->stream, &g));
+        hipError_t e = hipGraphInstantiate(&exec[b], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        GR_HIP(e);
+        return exec[b];
+    }
+};
+
+namespace {
+
+template <typename S>
+void upload_state(gr_solver *s, int b, const double *rho, const double *rhou, const double *rhoe) {
+    const size_t n = (size_t)s->mesh->h.n_cells + s->mesh->h.n_ghosts;
+    std::vector<S> soa(5 * s->stride, S(0));
+    for (size_t i = 0; i < n; ++i) {
+        soa[i] = (S)rho[i];
+        for (int k = 0; k < 3; ++k) soa[(1 + k) * s->stride + i] = (S)rhou[3 * i + k];
+        soa[4 * s->stride + i] = (S)rhoe[i];
+    }
+    GR_HIP(hipMemcpyAsync(s->ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice, s->mesh->stream));
+    GR_HIP(hipStreamSynchronize(s->mesh->stream));
+}
+
+template <typename S>
+void download_state(const gr_solver *s, double *rho, double *rhou, double *rhoe) {
+    const gr_mesh *m = s->mesh;
+    const size_t N = m->h.n_cells, n = N + m->h.n_ghosts;
+    std::vector<S> a(5 * s->stride), b(5 * s->stride);
+    GR_HIP(hipMemcpyAsync(a.data(), s->ubuf[s->cur].p, a.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
+    GR_HIP(hipMemcpyAsync(b.data(), s->ubuf[s->ghost_src].p, b.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
+    GR_HIP(hipStreamSynchronize(m->stream));
+    for (size_t i = 0; i < n; ++i) {
+        const std::vector<S> &src = i < N ? a : b;
+        if (rho) rho[i] = (double)src[i];
+        if (rhou)
+            for (int k = 0; k < 3; ++k) rhou[3 * i + k] = (double)src[(1 + k) * s->stride + i];
+        if (rhoe) rhoe[i] = (double)src[4 * s->stride + i];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int gr_solver_create(gr_mesh *m, const gr_solver_config *cfg, gr_solver **out) {
+    return guarded([&] {
+        require(m && cfg && out, "null argument");
+        check_ready(m);
+        require(cfg->dtype == GR_F64 || cfg->dtype == GR_F32, "solver dtype must be GR_F64 or GR_F32");
+        require(cfg->cfl > 0, "cfl must be positive");
+        require(cfg->limiter >= 0 && cfg->limiter <= 2, "unknown limiter");
+        require(m->h.n_regions <= (uint32_t)gr::kMaxRegions, "too many boundary regions");
+        require(!m->h.one_dimensional, "the Euler solver needs a 3-D mesh");
+        auto *s = new gr_solver();
+        try {
+            s->mesh = m;
+            s->cfg = *cfg;
+            s->stride = gr::pad64((size_t)m->h.n_cells + m->h.n_ghosts);
+            const size_t blk = s->stride * s->es();
+            for (auto &b : s->ubuf) {
+                b.alloc(5 * blk);
+                GR_HIP(hipMemsetAsync(b.p, 0, b.n, m->stream));
+            }
+            s->grad.alloc(15 * blk);
+            s->lim.alloc(5 * blk);
+            GR_HIP(hipMemsetAsync(s->grad.p, 0, s->grad.n, m->stream));
+            GR_HIP(hipMemsetAsync(s->lim.p, 0, s->lim.n, m->stream));
+            s->bcs.resize(m->h.n_regions);
+            s->ctl.alloc(1);
+            s->hist_cap = 1 << 17;
+            s->hist.alloc(s->hist_cap);
+            GR_HIP(hipHostMalloc((void **)&s->ctl_host, 3 * sizeof(gr::SolverCtl)));
+            for (auto &e : s->ev) GR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            GR_HIP(hipStreamSynchronize(m->stream));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+void gr_solver_free(gr_solver *s) { delete s; }
+
+int gr_solver_set_bc(gr_solver *s, const char *region, int kind, const double mach[3], double pressure,
+                     double temperature) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        require(kind == GR_BC_SLIP || kind == GR_BC_FARFIELD, "unknown boundary kind");
+        if (kind == GR_BC_FARFIELD) require(mach != nullptr && pressure > 0 && temperature > 0, "farfield needs mach, p > 0, T > 0");
+        const uint32_t r = region_of(s->mesh, region);
+        auto &b = s->bcs[r];
+        b.set = true;
+        b.kind = kind;
+        for (int k = 0; k < 3; ++k) b.mach[k] = mach ? mach[k] : 0.0;
+        b.p = pressure;
+        b.T = temperature;
+        s->drop_graphs();   // the table is baked into captured launches
+    });
+}
+
+int gr_solver_init_freestream(gr_solver *s, const double mach[3], double pressure, double temperature) {
+    return guarded([&] {
+        require(s && mach, "null argument");
+        double fs[5];
+        freestream(mach, pressure, temperature, fs);
+        const size_t n = (size_t)s->mesh->h.n_cells + s->mesh->h.n_ghosts;
+        for (int c = 0; c < 5; ++c) {
+            if (s->cfg.dtype == GR_F64) fill<double>(s->u<double>(s->cur, c), n, fs[c], s->mesh->stream);
+            else fill<float>(s->u<float>(s->cur, c), n, (float)fs[c], s->mesh->stream);
+        }
+        s->ghost_src = s->cur;
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(s->mesh->stream));
+    });
+}
+
+int gr_solver_set_state(gr_solver *s, const double *rho, const double *rhou, const double *rhoe) {
+    return guarded([&] {
+        require(s && rho && rhou && rhoe, "null argument");
+        if (s->cfg.dtype == GR_F64) upload_state<double>(s, s->cur, rho, rhou, rhoe);
+        else upload_state<float>(s, s->cur, rho, rhou, rhoe);
+        s->ghost_src = s->cur;
+    });
+}
+
+int gr_solver_get_state(const gr_solver *s, double *rho, double *rhou, double *rhoe) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        if (s->cfg.dtype == GR_F64) download_state<double>(s, rho, rhou, rhoe);
+        else download_state<float>(s, rho, rhou, rhoe);
+    });
+}
+
+int gr_solver_iterate(gr_solver *s, int max_iter, double tol, double reference, gr_solve_result *out) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        require(max_iter > 0, "max_iter must be positive");
+        gr_mesh *m = s->mesh;
+        gr::SolverCtl &c0 = s->ctl_host[0];
+        std::memset(&c0, 0, sizeof c0);
+        c0.ref = reference;
+        c0.tol = tol;
+        c0.max_iter = max_iter;
+        c0.hist_cap = s->hist_cap;
+        GR_HIP(hipMemcpyAsync(s->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
+        const int K = s->graph_iters;
+        const int launches = (max_iter + K - 1) / K;
+        for (int L = 0; L < launches; ++L) {
+            if (s->cfg.use_graph) {
+                GR_HIP(hipGraphLaunch(s->graph(s->cur), m->stream));
+            } else {
+                for (int k = 0; k < K; ++k) s->enqueue((s->cur + k) % 2, s->ctl.p, nullptr);
+                GR_HIP(hipGetLastError());
+            }
+            gr::SolverCtl *poll = &s->ctl_host[1 + (L & 1)];
+            GR_HIP(hipMemcpyAsync(poll, s->ctl.p, sizeof *poll, hipMemcpyDeviceToHost, m->stream));
+            GR_HIP(hipEventRecord(s->ev[L & 1], m->stream));
+            if (L >= 1) {
+                GR_HIP(hipEventSynchronize(s->ev[(L - 1) & 1]));
+                if (s->ctl_host[1 + ((L - 1) & 1)].stop) break;
+            }
+        }
+        GR_HIP(hipStreamSynchronize(m->stream));
+        gr::SolverCtl fin;
+        GR_HIP(hipMemcpy(&fin, s->ctl.p, sizeof fin, hipMemcpyDeviceToHost));
+        if (fin.iter > 0) {
+            s->ghost_src = (s->cur + fin.iter - 1) % 2;
+            s->cur = (s->cur + fin.iter) % 2;
+        }
+        s->last_iters = std::min(fin.iter, s->hist_cap);
+        if (out) {
+            out->iterations = fin.iter;
+            out->converged = fin.converged;
+            out->residual = fin.last;
+            out->ratio = fin.ratio;
+            out->reference = fin.ref;
+        }
+        if (fin.diverged) throw gr::Error(GR_ERR_DIVERGED, "residual is not finite (diverged)");
+    });
+}
+
+int gr_solver_history(const gr_solver *s, double *hist, int n) {
+    return guarded([&] {
+        require(s && hist, "null argument");
+        require(n >= 0 && n <= s->last_iters, "history request longer than the last run");
+        if (n) GR_HIP(hipMemcpy(hist, s->hist.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int gr_solver_residual_norm(gr_solver *s, double *out) {
+    return guarded([&] {
+        require(s && out, "null argument");
+        s->enqueue(s->cur, nullptr, nullptr);   // BCs, gradient+limiter, flux partials; state not swapped
+        GR_HIP(hipGetLastError());
+        *out = finalize_sum(s->mesh, gr::blocks_for(s->mesh->h.n_cells));
+        s->ghost_src = s->cur;
+    });
+}
+
+int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]) {
+    return guarded([&] {
+        require(s && ms, "null argument");
+        require(n_iter > 0 && n_iter <= 100000, "n_iter must be in 1..100000");
+        gr_mesh *m = s->mesh;
+        std::vector<hipEvent_t> ev(5 * (size_t)n_iter, nullptr);
+        struct Guard {
+            std::vector<hipEvent_t> &e;
+            ~Guard() {
+                for (auto x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } guard{ev};
+        for (auto &e : ev) GR_HIP(hipEventCreate(&e));
+        gr::SolverCtl &c0 = s->ctl_host[0];
+        std::memset(&c0, 0, sizeof c0);
+        c0.max_iter = n_iter;
+        c0.hist_cap = s->hist_cap;
+        GR_HIP(hipMemcpyAsync(s->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
+        for (int k = 0; k < n_iter; ++k) s->enqueue((s->cur + k) % 2, s->ctl.p, &ev[5 * (size_t)k]);
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+        double acc[4] = {0, 0, 0, 0};
+        for (int k = 0; k < n_iter; ++k)
+            for (int j = 0; j < 4; ++j) {
+                float t = 0;
+                GR_HIP(hipEventElapsedTime(&t, ev[5 * (size_t)k + j], ev[5 * (size_t)k + j + 1]));
+                acc[j] += t;
+            }
+        for (int j = 0; j < 4; ++j) ms[j] = acc[j] / n_iter;
+        s->ghost_src = (s->cur + n_iter - 1) % 2;
+        s->cur = (s->cur + n_iter) % 2;
+        s->last_iters = 0;
+    });
+}
+
+int gr_solver_kernel_bytes(const gr_solver *s, double bytes[4]) {
+    return guarded([&] {
+        require(s && bytes, "null argument");
+        s->kernel_bytes(bytes);
+    });
+}
+
+}  // extern "C"

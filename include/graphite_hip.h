@@ -1,0 +1,265 @@
+/*
+ * graphite_hip.h -- C ABI of libgraphite_hip.so, the MI355X (gfx950) implementation of
+ * GRAPHITE's second-order unstructured finite-volume residual path.
+ *
+ * Every entry point returns GR_OK (0) or an error code and never exits the process; the
+ * message of the last error on the calling thread is gr_last_error().  Handles are opaque and
+ * owned by the caller (free with the matching *_free).  Vectors are passed as double[3].
+ * All device work is enqueued on the library's stream; entry points that return data to the
+ * host synchronise that stream before returning.
+ *
+ * Citations are Alexsimulation/graphite file:line of the interface each entry point replaces.
+ */
+#ifndef GRAPHITE_HIP_H
+#define GRAPHITE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GR_ABI_VERSION 1
+
+enum gr_status {
+    GR_OK = 0,
+    GR_ERR_INVALID = 1,      /* bad argument or state */
+    GR_ERR_TOPOLOGY = 2,     /* inconsistent mesh (face not shared by exactly two cells, ...) */
+    GR_ERR_IO = 3,           /* file cannot be opened, read or written */
+    GR_ERR_NOT_FOUND = 4,    /* unknown region name */
+    GR_ERR_DEVICE = 5,       /* HIP runtime error (no device, out of memory, launch failure) */
+    GR_ERR_DIVERGED = 6      /* non-finite residual */
+};
+
+/* Data types of fields; mesh metrics are always binary64. */
+enum gr_dtype { GR_F32 = 4, GR_F64 = 8 };
+
+/* su2 element kinds, src/meshio.cu:297-313 */
+enum gr_elem_kind { GR_TRI = 5, GR_QUAD = 9, GR_TET = 10, GR_HEX = 12, GR_WEDGE = 13, GR_PYRAMID = 14 };
+
+/* Limiter functions, src/fvmops.cu:118-127 (MICHALAK is the reference's active one). */
+enum gr_limiter { GR_LIM_MICHALAK = 0, GR_LIM_BARTH_JESPERSEN = 1, GR_LIM_VENKATAKRISHNAN = 2 };
+
+/* Boundary kinds of PhysicsEuler's "bcs" entries, src/models/euler.cu:103-135. */
+enum gr_bc_kind { GR_BC_SLIP = 0, GR_BC_FARFIELD = 1 };
+
+const char *gr_last_error(void);
+int gr_abi_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int gr_device_count(void);
+
+/* ---- element lists (su2 form) ------------------------------------------------------------ */
+typedef struct gr_elements gr_elements;
+
+typedef struct gr_elements_info {
+    uint32_t n_nodes, n_elems, n_elem_nodes, n_markers;
+} gr_elements_info;
+
+/* Volume elements: kinds GR_TET/GR_HEX/GR_WEDGE/GR_PYRAMID, node lists concatenated with
+ * starts[n_elems+1].  Nodes are xyz triples.  (su2ReadElements, src/meshio.cu:771-811) */
+int gr_elements_create(uint32_t n_nodes, const double *xyz, uint32_t n_elems,
+                       const uint32_t *kinds, const uint32_t *starts, const uint32_t *nodes,
+                       gr_elements **out);
+/* Boundary marker (MARKER_TAG) of GR_TRI/GR_QUAD elements, src/meshio.cu:877-896. */
+int gr_elements_add_marker(gr_elements *e, const char *name, uint32_t n, const uint32_t *kinds,
+                           const uint32_t *starts, const uint32_t *nodes);
+/* su2 text reader (NDIME/NPOIN/NELEM/NMARK), src/meshio.cu:817-897, and writer. */
+int gr_elements_read_su2(const char *path, gr_elements **out);
+int gr_elements_write_su2(const gr_elements *e, const char *path);
+int gr_elements_info_get(const gr_elements *e, gr_elements_info *out);
+int gr_elements_get(const gr_elements *e, double *xyz, uint32_t *kinds, uint32_t *starts,
+                    uint32_t *nodes);
+/* Marker m: name (up to 127 chars + NUL into name[128]) and element count; arrays may be
+ * NULL to query the sizes first (n_nodes receives the node-list length). */
+int gr_elements_get_marker(const gr_elements *e, uint32_t m, char name[128], uint32_t *n,
+                           uint32_t *n_nodes, uint32_t *kinds, uint32_t *starts, uint32_t *nodes);
+void gr_elements_free(gr_elements *e);
+
+/* Synthetic meshes (no reference counterpart; they stand in for the gmsh .su2 inputs).
+ * Box [0,lx]x[0,ly]x[0,lz] of nx*ny*nz hexes, each split into 6 tets along the main diagonal
+ * when tets != 0; interior nodes moved by U[-jitter*h, +jitter*h] per axis (seeded
+ * mt19937_64); markers xmin,xmax,ymin,ymax,zmin,zmax. */
+int gr_gen_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly, double lz,
+               int tets, double jitter, uint64_t seed, gr_elements **out);
+/* O-grid hexes around a cylinder of radius r0 along z: n_theta x n_r x n_z cells, radial
+ * spacing geometric with ratio from r0 to r1, span n_z*dz; markers wall, farfield, zmin,
+ * zmax. */
+int gr_gen_ogrid(uint32_t n_theta, uint32_t n_r, uint32_t n_z, double r0, double r1,
+                 double ratio, double dz, gr_elements **out);
+
+/* ---- mesh ------------------------------------------------------------------------------ */
+typedef struct gr_mesh gr_mesh;
+
+typedef struct gr_mesh_sizes {
+    uint32_t n_nodes, n_faces, n_cells, n_ghosts, n_regions;
+    uint32_t n_face_nodes;   /* face_nodes_starts[n_faces] */
+    uint32_t n_cell_slots;   /* cell_faces_starts[n_cells+n_ghosts] */
+} gr_mesh_sizes;
+
+/* Unique faces, cell/face/ghost CSR, in cuMeshFromSu2's numbering (src/meshio.cu:902-1138).
+ * GR_ERR_TOPOLOGY when a face is not shared by exactly two cells or a marker face is not a
+ * face of the volume mesh. */
+int gr_mesh_build(const gr_elements *e, gr_mesh **out);
+/* cuMeshFromSu2 (src/meshio.cu:817-1250): su2 file -> binary .fvm file. */
+int gr_mesh_from_su2(const char *su2_path, const char *fvm_path, int scalar_bytes);
+/* cuMeshFromFile (src/meshio.cu:166-292): read a .fvm file whose node coordinates are
+ * scalar_bytes wide (4 as the reference's default FP32 build writes, or 8).  Metrics are
+ * computed by gr_mesh_compute_metrics. */
+int gr_mesh_read_fvm(const char *path, int scalar_bytes, gr_mesh **out);
+int gr_mesh_write_fvm(const gr_mesh *m, const char *path, int scalar_bytes);
+/* cuMeshMake1D (src/mesh.cu:540-634): n cells on [x0,x1] with unit cross-section. */
+int gr_mesh_make_1d(size_t n, double x0, double x1, gr_mesh **out);
+void gr_mesh_free(gr_mesh *m);   /* cuMeshFree, src/mesh.cu:401-411 */
+
+/* cuMeshGetnCells / GetnGhosts / GetVarSize ..., src/mesh.cu:5-31 */
+int gr_mesh_sizes_get(const gr_mesh *m, gr_mesh_sizes *out);
+/* Host copies of the CSR arrays (any pointer may be NULL).  nodes: 3*n_nodes. */
+int gr_mesh_get_connectivity(const gr_mesh *m, double *nodes, uint32_t *face_nodes_starts,
+                             uint32_t *face_nodes, uint32_t *cell_faces_starts,
+                             uint32_t *cell_faces, uint32_t *cell_connects,
+                             uint32_t *ghost_cell_ids, uint32_t *ghost_cell_starts);
+/* Region names (cuMeshGetRegionId src/mesh.cu:17-23): GR_ERR_NOT_FOUND for an unknown name
+ * (the reference prints and falls back to region 0). */
+int gr_mesh_region_id(const gr_mesh *m, const char *name, uint32_t *id);
+int gr_mesh_region_name(const gr_mesh *m, uint32_t id, char name[128]);
+/* cuMeshGetnRegionCells, src/mesh.cu:13-15 */
+int gr_mesh_region_size(const gr_mesh *m, uint32_t id, uint32_t *n);
+
+/* Reverse Cuthill-McKee renumbering of interior cells, ghosts sorted by their interior cell
+ * within each region, faces ordered by their lower-numbered cell.  Each cell keeps its face
+ * slot order, so every per-cell sum runs in the same order as before.  new2old_cells
+ * (n_cells+n_ghosts) and new2old_faces (n_faces) may be NULL.  Must precede
+ * gr_mesh_compute_metrics. */
+int gr_mesh_renumber_rcm(gr_mesh *m, uint32_t *new2old_cells, uint32_t *new2old_faces);
+/* Greedy face colouring: no two faces of one colour share a cell (interior or ghost).
+ * colors[n_faces] may be NULL; n_colors receives the count. */
+int gr_mesh_face_colors(const gr_mesh *m, uint32_t *colors, uint32_t *n_colors);
+/* Matrix bandwidth max |i-j| over interior neighbour pairs. */
+int gr_mesh_bandwidth(const gr_mesh *m, uint32_t *bw);
+
+/* cuMeshPassCpuToGpu + the four metric kernels cuMeshComputeFaces/Cells/GhostCells/
+ * CellMatrices (src/meshio.cu:277-286, src/mesh.cu:44-219), binary64 on the device. */
+int gr_mesh_compute_metrics(gr_mesh *m);
+/* cuMeshPassGpuToCpu of the metric arrays (src/mesh.cu:467-517), reference AoS layouts:
+ * face_centers/face_normals 3*n_faces, face_areas n_faces, cell_centers 3*(N+G),
+ * cell_volumes N+G, cell_matrices 9*N (tensor columns u,v,w).  Pointers may be NULL. */
+int gr_mesh_get_metrics(const gr_mesh *m, double *face_centers, double *face_normals,
+                        double *face_areas, double *cell_centers, double *cell_volumes,
+                        double *cell_matrices);
+
+/* ---- fields (FvmField, include/fvmfields.cuh:16-43) ---------------------------------------- */
+typedef struct gr_field gr_field;
+
+/* FvmFieldNew (src/fvmfields.cu:7-46): n_comp 1 (scalar) or 3 (vector) over the mesh's
+ * cells and ghosts, values of dtype GR_F64/GR_F32; gradients and limiters when alloc_grads. */
+int gr_field_create(gr_mesh *m, uint32_t n_comp, int dtype, const char *name, int alloc_grads,
+                    gr_field **out);
+void gr_field_free(gr_field *f);   /* FvmFieldFree, src/fvmfields.cu:49-64 */
+/* Host <-> device in the reference's AoS layout (value n*n_comp; gradient n*n_comp*3 with
+ * vector gradients as tensor columns; limiter n*n_comp).  VarGpuToCpu / VarAllGpuToCpu
+ * (src/fvmfields.cu:85-101). */
+int gr_field_upload(gr_field *f, const double *value);
+int gr_field_download(const gr_field *f, double *value, double *gradient, double *limiter);
+/* ScalarVarInit / VectorVarInit (src/fvmops.cu:8-43): value = v, gradient 0, limiter 1. */
+int gr_field_init(gr_field *f, const double *v);
+/* FvmFieldReduceSum (src/fvmops.cu:369-383): sum over interior cells, device reduction in
+ * binary64 (one 8-byte copy to the host). */
+int gr_field_reduce_sum(const gr_field *f, double *out);
+
+/* Scalar/VectorVarComputeGradient (src/fvmops.cu:51-111). */
+int gr_field_compute_gradient(gr_field *f);
+/* Scalar/VectorVarComputeLimiters (src/fvmops.cu:175-266). */
+int gr_field_compute_limiter(gr_field *f, int limiter);
+/* fvmScalGradLimBc / fvmVecGradLimBc (src/fvmops.cu:272-304). */
+int gr_field_gradlim_bc(gr_field *f);
+/* fvmScalarBcFixed (src/fvmops.cu:308-323); v has n_comp entries. */
+int gr_field_bc_fixed(gr_field *f, uint32_t region, const double *v);
+/* fvmScalarBcZeroGrad / fvmVectorBcZeroGrad (src/fvmops.cu:326-365). */
+int gr_field_bc_zero_grad(gr_field *f, uint32_t region);
+
+/* ---- Euler model (src/models/euler.cu, include/models/euler.cuh:45-94) ------------------- */
+
+/* FvmEulerInit (src/models/euler.cu:529-549). */
+int gr_euler_init(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe,
+                  const double mach[3], double pressure, double temperature);
+/* FvmEulerSlipBc (src/models/euler.cu:552-568). */
+int gr_euler_bc_slip(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe,
+                     const char *region);
+/* FvmEulerInletOutletBc (src/models/euler.cu:571-593). */
+int gr_euler_bc_farfield(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe,
+                         const double mach[3], double pressure, double temperature,
+                         const char *region);
+/* FvmEulerStepPrepare (src/models/euler.cu:597-647): gradients, limiters and their ghost
+ * copies of rho, rhou, rhoe (one fused kernel plus the ghost copy). */
+int gr_euler_step_prepare(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe,
+                          int limiter);
+/* FvmEulerStep (src/models/euler.cu:649-673) with time_scheme "steady": flux residual and
+ * local-time-step update.  Jacobi: the update reads the old state of every cell (the
+ * reference updates in place while neighbours read).  dt (scalar field, may be NULL)
+ * receives cfl/sum(eig*ds); error (scalar field, may be NULL) receives V*|R|^2 per cell. */
+int gr_euler_step(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, double cfl,
+                  const char *time_scheme, gr_field *dt, gr_field *error);
+/* Per-cell flux residual R (5 per interior cell: rho, rhou xyz, rhoe) of the current state,
+ * without updating it (the sums fvmEulerSteadyOp forms, src/models/euler.cu:236-339). */
+int gr_euler_residual(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, double *resid);
+/* Momentum reconstruction: 0 = the reference's transposed product (default), 1 = grad(u).d
+ * per component. */
+int gr_euler_set_corrected_reconstruction(gr_mesh *m, int on);
+
+/* ---- Euler solver: the whole iteration on the device ----------------------------------------
+ * PhysicsEuler::step (src/models/euler.cu:99-175) as one device pipeline per iteration:
+ * all boundary regions in one launch, fused gradient+limiter, fused flux+update+residual,
+ * device reduction of the residual norm, captured in a HIP graph; convergence is decided on
+ * the device so the host does not synchronise per iteration. */
+typedef struct gr_solver gr_solver;
+
+typedef struct gr_solver_config {
+    int dtype;         /* GR_F64: binary64 throughout; GR_F32: binary32 state, gradient,
+                          limiter and flux with binary64 metrics and norms */
+    double cfl;
+    int limiter;       /* gr_limiter */
+    int corrected;     /* momentum reconstruction, see gr_euler_set_corrected_reconstruction */
+    int use_graph;     /* capture iterations in a HIP graph */
+} gr_solver_config;
+
+typedef struct gr_solve_result {
+    int iterations;        /* iterations run by this call */
+    int converged;         /* residual ratio fell below tol */
+    double residual;       /* last un-normalised residual sum */
+    double ratio;          /* last residual / reference */
+    double reference;      /* normalisation used */
+} gr_solve_result;
+
+int gr_solver_create(gr_mesh *m, const gr_solver_config *cfg, gr_solver **out);
+void gr_solver_free(gr_solver *s);
+/* Boundary condition of a region (slip, or farfield with mach/p/T). */
+int gr_solver_set_bc(gr_solver *s, const char *region, int kind, const double mach[3],
+                     double pressure, double temperature);
+/* Uniform state over cells and ghosts (euler_init). */
+int gr_solver_init_freestream(gr_solver *s, const double mach[3], double pressure,
+                              double temperature);
+/* Set/get the conservative state, reference layouts: rho (N+G), rhou 3(N+G), rhoe (N+G). */
+int gr_solver_set_state(gr_solver *s, const double *rho, const double *rhou, const double *rhoe);
+int gr_solver_get_state(const gr_solver *s, double *rho, double *rhou, double *rhoe);
+/* Runs up to max_iter iterations; stops after the iteration whose residual / reference
+ * falls below tol (tol <= 0: never).  reference <= 0 uses the residual of the first
+ * iteration of this call, as the reference's err0 (src/models/euler.cu:158-169). */
+int gr_solver_iterate(gr_solver *s, int max_iter, double tol, double reference,
+                      gr_solve_result *out);
+/* Residual sums of the iterations of the last gr_solver_iterate call (n <= iterations). */
+int gr_solver_history(const gr_solver *s, double *hist, int n);
+/* Residual of the current state without updating it (one iteration's BC+prepare+flux). */
+int gr_solver_residual_norm(gr_solver *s, double *out);
+/* Runs n_iter iterations (advancing the state) as stream launches with HIP events recorded
+ * around every kernel on the library's stream; ms receives the mean duration in milliseconds
+ * of [0] boundary conditions, [1] gradient+limiter, [2] flux+update, [3] residual reduction. */
+int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]);
+/* Algorithmic bytes one launch of each kernel moves (every array element it reads or writes
+ * counted once), same order as gr_solver_profile. */
+int gr_solver_kernel_bytes(const gr_solver *s, double bytes[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,330 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle on the same
+seeded inputs.  Tolerance (binary64 path): normwise relative difference <= 1e-12 per array;
+connectivity-derived integer arrays are compared exactly.  The binary64 kernels evaluate
+the oracle's expressions in the same order without contraction, so most arrays also match
+bit for bit; test_*_bit_exact records that stronger property where it holds.
+"""
+import numpy as np
+import pytest
+
+from conftest import normwise
+from flowcases import P_INF, T_INF, c1_elements, ogrid_elements, perturbed_state
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _case(gr, orc, e, renumber=False):
+    m = gr.Mesh.build(e)
+    perm = None
+    if renumber:
+        perm = m.renumber_rcm()
+    m.compute_metrics()
+    om = orc.OracleMesh(gr.ElementArrays.of(e))
+    return m, om, perm
+
+
+@pytest.fixture(scope="module")
+def c1(gr, orc):
+    print("case: C1 6-tet jittered box 20^3", flush=True)
+    return _case(gr, orc, c1_elements(gr))
+
+
+def test_metrics_match_oracle(c1):
+    m, om, _ = c1
+    mt = m.metrics()
+    pairs = {"face_centers": om.face_centers, "face_normals": om.face_normals, "face_areas": om.face_areas,
+             "cell_centers": om.cell_centers, "cell_volumes": om.cell_volumes, "cell_matrices": om.cell_matrices}
+    for k, ref in pairs.items():
+        assert normwise(mt[k], ref) <= TOL, k
+
+
+def test_metrics_bit_exact(c1):
+    m, om, _ = c1
+    mt = m.metrics()
+    for k in ("face_centers", "face_normals", "face_areas", "cell_centers", "cell_volumes", "cell_matrices"):
+        assert np.array_equal(mt[k], getattr(om, k)), k
+
+
+def _fields(gr, m, dtype):
+    rho = gr.Field(m, 1, dtype, "rho")
+    rhou = gr.Field(m, 3, dtype, "rhou")
+    rhoe = gr.Field(m, 1, dtype, "rhoe")
+    return rho, rhou, rhoe
+
+
+def _oracle_state(orc, om, rho, rhou, rhoe):
+    n = om.n_cells + om.n_ghosts
+    s = orc.EulerState.zeros(n)
+    s.rho[:] = rho
+    s.rhou[:] = rhou
+    s.rhoe[:] = rhoe
+    return s
+
+
+def test_field_gradient_limiter_match_oracle(gr, orc, c1):
+    m, om, _ = c1
+    n = om.n_cells + om.n_ghosts
+    rho, rhou, rhoe = perturbed_state(om.cell_centers)
+    for kind in (gr.GR_LIM_MICHALAK, gr.GR_LIM_BARTH_JESPERSEN, gr.GR_LIM_VENKATAKRISHNAN):
+        fs = gr.Field(m, 1, gr.GR_F64, "s")
+        fv = gr.Field(m, 3, gr.GR_F64, "v")
+        fs.upload(rho)
+        fv.upload(rhou)
+        for f in (fs, fv):
+            f.compute_gradient()
+            f.compute_limiter(kind)
+            f.gradlim_bc()
+        vs, gs, ls = fs.download(grads=True)
+        vv, gv, lv = fv.download(grads=True)
+        og_s = orc.scalar_gradient(om, rho)
+        og_v = orc.vector_gradient(om, rhou)
+        ol_s = orc.scalar_limiter(om, rho, og_s, kind)
+        ol_v = orc.vector_limiter(om, rhou, og_v, kind)
+        N = om.n_cells
+        assert normwise(gs[:N], og_s[:N]) <= TOL
+        assert normwise(gv[:N], og_v[:N]) <= TOL
+        assert normwise(ls[:N], ol_s[:N]) <= TOL
+        assert normwise(lv[:N], ol_v[:N]) <= TOL
+        # ghost copies (fvmScalGradLimBc)
+        owner = m.connectivity()["cell_connects"][m.connectivity()["cell_faces_starts"][N:n]]
+        assert np.array_equal(gs[N:], gs[owner]) and np.array_equal(lv[N:], lv[owner])
+
+
+def test_reduce_sum_and_init(gr, orc, c1):
+    m, om, _ = c1
+    f = gr.Field(m, 1, gr.GR_F64, "e")
+    v = np.random.default_rng(5).random(om.n_cells + om.n_ghosts)
+    f.upload(v)
+    assert abs(f.reduce_sum() - orc.reduce_sum(v[:om.n_cells])) <= 1e-13 * abs(v[:om.n_cells].sum())
+    f.init(2.5)
+    val, g, l = f.download(grads=True)
+    assert np.all(val == 2.5) and np.all(g == 0) and np.all(l == 1)
+
+
+def test_generic_bcs(gr, orc, c1):
+    m, om, _ = c1
+    n = om.n_cells + om.n_ghosts
+    v = np.random.default_rng(1).random((n, 3))
+    f = gr.Field(m, 3, gr.GR_F64, "v")
+    f.upload(v)
+    f.bc_fixed(0, [1.0, 2.0, 3.0])
+    f.bc_zero_grad(1)
+    exp = v.copy().reshape(-1)
+    orc.lib().orc_bc_fixed(om._p, 0, 3, np.array([1.0, 2.0, 3.0]), exp)
+    orc.lib().orc_bc_zero_grad(om._p, 1, 3, exp)
+    assert np.array_equal(f.download(), exp.reshape(n, 3))
+
+
+@pytest.mark.parametrize("mach", [(0.5, 0.0, 0.0), (2.0, 0.3, -0.1)])
+def test_euler_bcs_match_oracle(gr, orc, c1, mach):
+    m, om, _ = c1
+    rho, rhou, rhoe = perturbed_state(om.cell_centers, mach=mach)
+    f = _fields(gr, m, gr.GR_F64)
+    for x, v in zip(f, (rho, rhou, rhoe)):
+        x.upload(v)
+    s = _oracle_state(orc, om, rho, rhou, rhoe)
+    names = m.region_names()
+    for r, name in enumerate(names):
+        if r % 2 == 0:
+            gr.euler_bc_farfield(m, *f, mach, P_INF, T_INF, name)
+            orc.farfield_bc(om, r, s, mach, P_INF, T_INF)
+        else:
+            gr.euler_bc_slip(m, *f, name)
+            orc.slip_bc(om, r, s)
+    assert np.array_equal(f[0].download(), s.rho)
+    assert np.array_equal(f[1].download(), s.rhou)
+    assert np.array_equal(f[2].download(), s.rhoe)
+
+
+@pytest.mark.parametrize("corrected", [False, True])
+def test_euler_prepare_and_step_match_oracle(gr, orc, c1, corrected):
+    m, om, _ = c1
+    mach = (0.5, 0.0, 0.0)
+    rho, rhou, rhoe = perturbed_state(om.cell_centers, mach=mach)
+    f = _fields(gr, m, gr.GR_F64)
+    for x, v in zip(f, (rho, rhou, rhoe)):
+        x.upload(v)
+    s = _oracle_state(orc, om, rho, rhou, rhoe)
+    for r, name in enumerate(m.region_names()):
+        gr.euler_bc_farfield(m, *f, mach, P_INF, T_INF, name)
+        orc.farfield_bc(om, r, s, mach, P_INF, T_INF)
+    gr.euler_step_prepare(m, *f)
+    orc.step_prepare(om, s)
+    N = om.n_cells
+    _, g_r, l_r = f[0].download(grads=True)
+    _, g_u, l_u = f[1].download(grads=True)
+    assert normwise(g_r, s.grad_rho) <= TOL and normwise(l_r, s.lim_rho) <= TOL
+    assert normwise(g_u, s.grad_rhou) <= TOL and normwise(l_u, s.lim_rhou) <= TOL
+    m.set_corrected_reconstruction(corrected)
+    resid = gr.euler_residual(m, *f)
+    dt = gr.Field(m, 1, gr.GR_F64, "dt", grads=False)
+    err = gr.Field(m, 1, gr.GR_F64, "err", grads=False)
+    gr.euler_step(m, *f, 0.5, dt=dt, error=err)
+    m.set_corrected_reconstruction(False)
+    nr, nu, ne, oerr, ores, odt = orc.step(om, s, 0.5, corrected=corrected)
+    assert normwise(resid, ores) <= TOL
+    assert normwise(f[0].download(), nr) <= TOL
+    assert normwise(f[1].download(), nu) <= TOL
+    assert normwise(f[2].download(), ne) <= TOL
+    assert normwise(err.download()[:N], oerr) <= TOL
+    assert normwise(dt.download()[:N], odt) <= TOL
+    assert abs(err.reduce_sum() - orc.reduce_sum(oerr)) <= 1e-12 * orc.reduce_sum(oerr)
+
+
+def test_euler_step_bit_exact(gr, orc, c1):
+    m, om, _ = c1
+    mach = (0.5, 0.0, 0.0)
+    rho, rhou, rhoe = perturbed_state(om.cell_centers, mach=mach)
+    f = _fields(gr, m, gr.GR_F64)
+    for x, v in zip(f, (rho, rhou, rhoe)):
+        x.upload(v)
+    s = _oracle_state(orc, om, rho, rhou, rhoe)
+    for r, name in enumerate(m.region_names()):
+        gr.euler_bc_farfield(m, *f, mach, P_INF, T_INF, name)
+        orc.farfield_bc(om, r, s, mach, P_INF, T_INF)
+    gr.euler_step_prepare(m, *f)
+    orc.step_prepare(om, s)
+    resid = gr.euler_residual(m, *f)
+    _, _, _, _, ores, _ = orc.step(om, s, 0.5)
+    assert np.array_equal(resid, ores)
+
+
+def _solver_run(gr, m, dtype, bcs, state, n_iter, cfl=0.5, use_graph=True, limiter=0):
+    sv = gr.Solver(m, dtype=dtype, cfl=cfl, use_graph=use_graph, limiter=limiter)
+    for name, kind, mach in bcs:
+        sv.set_bc(name, gr.GR_BC_FARFIELD if kind == "farfield" else gr.GR_BC_SLIP, mach, P_INF, T_INF)
+    sv.set_state(*state)
+    res = sv.iterate(n_iter)
+    return sv, res
+
+
+def test_solver_iterations_match_oracle(gr, orc, c1):
+    m, om, _ = c1
+    mach = (0.5, 0.0, 0.0)
+    state = perturbed_state(om.cell_centers, mach=mach)
+    names = m.region_names()
+    bcs = [(nm, "farfield" if r < 4 else "slip", mach) for r, nm in enumerate(names)]
+    n_iter = 30
+    sv, res = _solver_run(gr, m, gr.GR_F64, bcs, state, n_iter)
+    assert res.iterations == n_iter
+    hist = sv.history(n_iter)
+    s = _oracle_state(orc, om, *state)
+    obcs = [(r, orc.BC_FARFIELD if k == "farfield" else orc.BC_SLIP, mc, P_INF, T_INF)
+            for r, (_, k, mc) in enumerate(bcs)]
+    n, ohist = orc.iterate(om, s, obcs, 0.5, n_iter, threads=8)
+    assert n == n_iter
+    assert normwise(hist, ohist) <= TOL
+    r, u, e = sv.get_state()
+    assert normwise(r, s.rho) <= TOL and normwise(u, s.rhou) <= TOL and normwise(e, s.rhoe) <= TOL
+
+
+def test_solver_graph_equals_stream_and_is_deterministic(gr, orc, c1):
+    m, om, _ = c1
+    mach = (0.5, 0.0, 0.0)
+    state = perturbed_state(om.cell_centers, mach=mach)
+    bcs = [(nm, "farfield", mach) for nm in m.region_names()]
+    a, _ = _solver_run(gr, m, gr.GR_F64, bcs, state, 17, use_graph=True)
+    b, _ = _solver_run(gr, m, gr.GR_F64, bcs, state, 17, use_graph=False)
+    c, _ = _solver_run(gr, m, gr.GR_F64, bcs, state, 17, use_graph=True)
+    sa, sb, sc = a.get_state(), b.get_state(), c.get_state()
+    for x, y, z in zip(sa, sb, sc):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    assert np.array_equal(a.history(17), b.history(17))
+
+
+def test_solver_tolerance_stop_and_resume(gr, orc, c1):
+    m, om, _ = c1
+    mach = (0.5, 0.0, 0.0)
+    state = perturbed_state(om.cell_centers, mach=mach)
+    bcs = [(nm, "farfield", mach) for nm in m.region_names()]
+    sv, res = _solver_run(gr, m, gr.GR_F64, bcs, state, 400)
+    hist = sv.history(res.iterations)
+    # stop rule of PhysicsEuler::step: err/err0 < tol after the iteration that crosses it
+    tol = float(hist[7] / hist[0]) * 1.0000001
+    sv2, r2 = _solver_run(gr, m, gr.GR_F64, bcs, state, 400)
+    sv2.set_state(*state)
+    r2 = sv2.iterate(400, tol=tol)
+    k = int(np.argmax(hist / hist[0] < tol))
+    assert r2.converged == 1 and r2.iterations == k + 1
+    # continuing an odd number of iterations keeps the state consistent with one long run
+    sv3, _ = _solver_run(gr, m, gr.GR_F64, bcs, state, 13)
+    sv3.iterate(8)
+    sv4, _ = _solver_run(gr, m, gr.GR_F64, bcs, state, 21)
+    for x, y in zip(sv3.get_state(), sv4.get_state()):
+        assert np.array_equal(x, y)
+
+
+def test_renumbered_mesh_gives_same_results(gr, orc):
+    e = c1_elements(gr, n=10, seed=7)
+    m0 = gr.Mesh.build(e)
+    m0.compute_metrics()
+    m1 = gr.Mesh.build(e)
+    c2o, _ = m1.renumber_rcm()
+    m1.compute_metrics()
+    om = orc.OracleMesh(gr.ElementArrays.of(e))
+    mach = (0.5, 0.0, 0.0)
+    state = perturbed_state(om.cell_centers, mach=mach)
+    bcs = [(nm, "farfield", mach) for nm in m0.region_names()]
+    a, _ = _solver_run(gr, m0, gr.GR_F64, bcs, state, 12)
+    b, _ = _solver_run(gr, m1, gr.GR_F64, bcs, tuple(x[c2o] for x in state), 12)
+    sa, sb = a.get_state(), b.get_state()
+    N = om.n_cells
+    for x, y in zip(sa, sb):
+        back = np.empty_like(y)
+        back[c2o] = y
+        assert np.array_equal(x[:N], back[:N])
+    assert np.array_equal(a.history(12), b.history(12))
+
+
+def test_freestream_preservation_c1(gr, orc, c1):
+    m, om, _ = c1
+    mach = (0.5, 0.0, 0.0)
+    sv = gr.Solver(m, gr.GR_F64, cfl=0.5)
+    for nm in m.region_names():
+        sv.set_bc(nm, gr.GR_BC_FARFIELD, mach, P_INF, T_INF)
+    sv.init_freestream(mach, P_INF, T_INF)
+    sv.iterate(5)
+    r, u, e = sv.get_state()
+    assert np.max(np.abs(r - 1.4)) < 1e-12
+    assert np.max(np.abs(u[:, 0] - 1.4 * 0.5)) < 1e-12 and np.max(np.abs(u[:, 1:])) < 1e-12
+
+
+def test_mixed_precision_error_vs_fp64(gr, orc, c1):
+    m, om, _ = c1
+    mach = (0.5, 0.0, 0.0)
+    state = perturbed_state(om.cell_centers, mach=mach)
+    bcs = [(nm, "farfield", mach) for nm in m.region_names()]
+    a, ra = _solver_run(gr, m, gr.GR_F64, bcs, state, 50)
+    b, rb = _solver_run(gr, m, gr.GR_F32, bcs, state, 50)
+    N = om.n_cells
+    errs = {}
+    for name, x, y in zip(("rho", "rhou", "rhoe"), a.get_state(), b.get_state()):
+        errs[name] = normwise(y[:N], x[:N])
+    print("mixed-precision normwise error vs FP64 after 50 iterations:", errs)
+    for k, v in errs.items():
+        assert v < 1e-5, (k, v)
+    assert abs(rb.residual / ra.residual - 1) < 1e-3
+
+
+def test_ogrid_c2_residual_matches_oracle(gr, orc):
+    print("case: C2 O-grid 256x128x4 (3 iterations vs oracle)", flush=True)
+    e = ogrid_elements(gr)
+    m = gr.Mesh.build(e)
+    m.compute_metrics()
+    om = orc.OracleMesh(gr.ElementArrays.of(e))
+    mach = (0.3, 0.0, 0.0)
+    bcs = [("wall", "slip", mach), ("farfield", "farfield", mach), ("zmin", "slip", mach), ("zmax", "slip", mach)]
+    sv = gr.Solver(m, gr.GR_F64, cfl=0.5)
+    for nm, k, mc in bcs:
+        sv.set_bc(nm, gr.GR_BC_FARFIELD if k == "farfield" else gr.GR_BC_SLIP, mc, P_INF, T_INF)
+    sv.init_freestream(mach, P_INF, T_INF)
+    res = sv.iterate(3)
+    s = orc.EulerState.zeros(om.n_cells + om.n_ghosts)
+    orc.euler_init(om, s, mach, P_INF, T_INF)
+    obcs = [(om.region(nm), orc.BC_FARFIELD if k == "farfield" else orc.BC_SLIP, mc, P_INF, T_INF)
+            for nm, k, mc in bcs]
+    n, ohist = orc.iterate(om, s, obcs, 0.5, 3, threads=8)
+    assert normwise(sv.history(3), ohist) <= TOL
+    r, u, e_ = sv.get_state()
+    assert normwise(r, s.rho) <= TOL and normwise(u, s.rhou) <= TOL and normwise(e_, s.rhoe) <= TOL
