@@ -93,6 +93,12 @@ class Elements:
         return Elements(h)
 
     @staticmethod
+    def cgrid_naca(n_i=384, n_j=96, n_wake=64, thickness=0.12, radius=20.0, first_height=1e-3, dz=0.1):
+        h = C.c_void_p()
+        _check(load().gr_gen_cgrid_naca(n_i, n_j, n_wake, thickness, radius, first_height, dz, C.byref(h)))
+        return Elements(h)
+
+    @staticmethod
     def read_su2(path: str):
         h = C.c_void_p()
         _check(load().gr_elements_read_su2(path.encode(), C.byref(h)))

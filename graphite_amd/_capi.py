@@ -54,6 +54,7 @@ SIGNATURES = {
     "gr_elements_free": (None, [_vp]),
     "gr_gen_box": (_i, [_u32, _u32, _u32, _d, _d, _d, _i, _d, C.c_uint64, C.POINTER(_vp)]),
     "gr_gen_ogrid": (_i, [_u32, _u32, _u32, _d, _d, _d, _d, C.POINTER(_vp)]),
+    "gr_gen_cgrid_naca": (_i, [_u32, _u32, _u32, _d, _d, _d, _d, C.POINTER(_vp)]),
     "gr_mesh_build": (_i, [_vp, C.POINTER(_vp)]),
     "gr_mesh_from_su2": (_i, [_s, _s, _i]),
     "gr_mesh_read_fvm": (_i, [_s, _i, C.POINTER(_vp)]),

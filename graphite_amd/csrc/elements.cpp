@@ -254,6 +254,133 @@ void gen_ogrid(uint32_t nt, uint32_t nr, uint32_t nz, double r0, double r1, doub
     });
 }
 
+// NACA 4-digit symmetric thickness, closed trailing edge.
+static double naca_half_thickness(double x, double t) {
+    return 5.0 * t * (0.2969 * std::sqrt(x) - 0.1260 * x - 0.3516 * x * x + 0.2843 * x * x * x - 0.1036 * x * x * x * x);
+}
+
+// Ratio r with h0 * (r^n - 1)/(r - 1) = 1 (geometric spacing from h0 over unit length).
+static double stretch_ratio(double h0, uint32_t n) {
+    if (h0 * n >= 1.0) return 1.0;
+    double lo = 1.0 + 1e-12, hi = 2.0;
+    for (int it = 0; it < 200; ++it) {
+        double r = 0.5 * (lo + hi);
+        double len = h0 * (std::pow(r, (double)n) - 1.0) / (r - 1.0);
+        (len > 1.0 ? hi : lo) = r;
+    }
+    return 0.5 * (lo + hi);
+}
+
+void gen_cgrid(uint32_t ni, uint32_t nj, uint32_t nwake, double thick, double radius, double h0, double dz,
+               gr_elements &e) {
+    require(ni >= 16 && nj >= 2 && nwake >= 2 && 2 * nwake + 8 <= ni && (ni - 2 * nwake) % 2 == 0,
+            "cgrid: need ni >= 2*n_wake + 8 with an even airfoil count");
+    require(thick > 0 && thick < 0.5 && radius > 2 && h0 > 0 && h0 < 0.5 && dz > 0, "cgrid: bad parameters");
+    const uint32_t naf = ni - 2 * nwake, nhalf = naf / 2;
+    const double pi = 3.14159265358979323846;
+    // inner line (j = 0): lower wake (outflow -> TE), lower surface (TE -> LE), upper surface
+    // (LE -> TE), upper wake (TE -> outflow).  Outer line (j = nj): the matching C boundary.
+    std::vector<double> px(ni + 1), py(ni + 1), qx(ni + 1), qy(ni + 1);
+    const double x_out = 1.0 + radius;
+    const double te_spacing = 0.5 * (1 - std::cos(pi / nhalf)) ;   // ~ chordwise spacing at the TE
+    const double wr = stretch_ratio(std::max(te_spacing, 1e-4) / radius, nwake);
+    std::vector<double> wx(nwake + 1);
+    {
+        double acc = 0, h = std::max(te_spacing, 1e-4) / radius;
+        wx[0] = 0;
+        for (uint32_t k = 1; k <= nwake; ++k) {
+            acc += h;
+            h *= wr;
+            wx[k] = acc;
+        }
+        for (uint32_t k = 0; k <= nwake; ++k) wx[k] = 1.0 + radius * wx[k] / wx[nwake];
+    }
+    for (uint32_t i = 0; i <= ni; ++i) {
+        if (i <= nwake) {                       // lower wake, x decreasing to the TE
+            px[i] = wx[nwake - i];
+            py[i] = 0;
+            qx[i] = px[i];
+            qy[i] = -radius;
+        } else if (i >= ni - nwake) {           // upper wake
+            px[i] = wx[i - (ni - nwake)];
+            py[i] = 0;
+            qx[i] = px[i];
+            qy[i] = radius;
+        } else {                                // airfoil, cosine spacing; beta: 0 at TE lower .. pi at LE .. 2pi at TE upper
+            const double beta = pi * (double)(i - nwake) / nhalf;
+            const double x = 0.5 * (1 + std::cos(beta));
+            const double yt = naca_half_thickness(std::min(1.0, std::max(0.0, x)), thick);
+            px[i] = x;
+            py[i] = (i - nwake < nhalf) ? -yt : yt;
+            // outer C: half circle of the same radius around (1, 0), angle matching beta
+            const double phi = -0.5 * pi - pi * (double)(i - nwake) / naf;   // -pi/2 .. -3pi/2
+            qx[i] = 1.0 + radius * std::cos(phi);
+            qy[i] = radius * std::sin(phi);
+        }
+    }
+    px[nwake] = 1.0; py[nwake] = 0.0;           // trailing edge shared by surface and wake
+    px[ni - nwake] = 1.0; py[ni - nwake] = 0.0;
+    const double rj = stretch_ratio(h0 / radius, nj);
+    std::vector<double> s(nj + 1);
+    {
+        double acc = 0, h = h0 / radius;
+        s[0] = 0;
+        for (uint32_t j = 1; j <= nj; ++j) {
+            acc += h;
+            h *= rj;
+            s[j] = acc;
+        }
+        for (uint32_t j = 0; j <= nj; ++j) s[j] /= s[nj];
+    }
+    // node ids; the upper wake's inner nodes are the lower wake's (wake cut)
+    const uint32_t per_layer = (ni + 1) * (nj + 1);
+    auto raw = [&](uint32_t i, uint32_t j, uint32_t k) { return i + (ni + 1) * (j + (nj + 1) * k); };
+    std::vector<uint32_t> remap(2 * per_layer);
+    std::vector<uint32_t> node_i(2 * per_layer), node_j(2 * per_layer), node_k(2 * per_layer);
+    uint32_t next = 0;
+    std::vector<double> xyz;
+    for (uint32_t k = 0; k < 2; ++k)
+        for (uint32_t j = 0; j <= nj; ++j)
+            for (uint32_t i = 0; i <= ni; ++i) {
+                uint32_t r = raw(i, j, k);
+                if (j == 0 && i >= ni - nwake) {
+                    remap[r] = remap[raw(ni - i, 0, k)];
+                    continue;
+                }
+                remap[r] = next;
+                node_i[next] = i;
+                node_j[next] = j;
+                node_k[next] = k;
+                ++next;
+                xyz.push_back(px[i] + s[j] * (qx[i] - px[i]));
+                xyz.push_back(py[i] + s[j] * (qy[i] - py[i]));
+                xyz.push_back(k * dz);
+            }
+    e = gr_elements();
+    e.xyz = std::move(xyz);
+    e.starts.push_back(0);
+    for (uint32_t j = 0; j < nj; ++j)
+        for (uint32_t i = 0; i < ni; ++i) {
+            uint32_t h[8] = {remap[raw(i, j, 0)],     remap[raw(i + 1, j, 0)], remap[raw(i + 1, j + 1, 0)],
+                             remap[raw(i, j + 1, 0)], remap[raw(i, j, 1)],     remap[raw(i + 1, j, 1)],
+                             remap[raw(i + 1, j + 1, 1)], remap[raw(i, j + 1, 1)]};
+            e.kinds.push_back(GR_HEX);
+            e.nodes.insert(e.nodes.end(), h, h + 8);
+            e.starts.push_back((uint32_t)e.nodes.size());
+        }
+    add_boundary_markers(e, {"airfoil", "farfield", "zmin", "zmax"}, [&](const uint32_t *fn, int c) {
+        auto all = [&](auto pred) {
+            for (int q = 0; q < c; ++q)
+                if (!pred(fn[q])) return false;
+            return true;
+        };
+        if (all([&](uint32_t n) { return node_k[n] == 0; })) return 2;
+        if (all([&](uint32_t n) { return node_k[n] == 1; })) return 3;
+        if (all([&](uint32_t n) { return node_j[n] == 0; })) return 0;
+        return 1;   // outer C boundary and the two outflow planes
+    });
+}
+
 // ---- su2 text format (src/meshio.cu:771-897) -----------------------------------------------
 
 void read_su2(const char *path, gr_elements &e) {
@@ -479,6 +606,21 @@ int gr_gen_ogrid(uint32_t n_theta, uint32_t n_r, uint32_t n_z, double r0, double
         auto *e = new gr_elements();
         try {
             gr::gen_ogrid(n_theta, n_r, n_z, r0, r1, ratio, dz, *e);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+int gr_gen_cgrid_naca(uint32_t n_i, uint32_t n_j, uint32_t n_wake, double thickness, double radius,
+                      double first_height, double dz, gr_elements **out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        auto *e = new gr_elements();
+        try {
+            gr::gen_cgrid(n_i, n_j, n_wake, thickness, radius, first_height, dz, *e);
         } catch (...) {
             delete e;
             throw;

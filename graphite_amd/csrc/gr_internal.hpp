@@ -96,5 +96,7 @@ void gen_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly, double
              double jitter, uint64_t seed, gr_elements &e);
 void gen_ogrid(uint32_t nt, uint32_t nr, uint32_t nz, double r0, double r1, double ratio,
                double dz, gr_elements &e);
+void gen_cgrid(uint32_t ni, uint32_t nj, uint32_t nwake, double thick, double radius, double h0, double dz,
+               gr_elements &e);
 
 }  // namespace gr

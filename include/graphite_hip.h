@@ -87,6 +87,13 @@ int gr_gen_box(uint32_t nx, uint32_t ny, uint32_t nz, double lx, double ly, doub
  * zmax. */
 int gr_gen_ogrid(uint32_t n_theta, uint32_t n_r, uint32_t n_z, double r0, double r1,
                  double ratio, double dz, gr_elements **out);
+/* C-grid hexes around a symmetric NACA 4-digit airfoil (chord 1, leading edge at the origin,
+ * closed trailing edge) extruded one cell of depth dz: n_i cells along the C (n_wake in each
+ * wake part, the rest cosine-spaced around the airfoil), n_j cells outward with geometric
+ * growth from first_height to an outer C of the given radius around the trailing edge; the
+ * wake cut is merged.  Markers airfoil, farfield, zmin, zmax. */
+int gr_gen_cgrid_naca(uint32_t n_i, uint32_t n_j, uint32_t n_wake, double thickness, double radius,
+                      double first_height, double dz, gr_elements **out);
 
 /* ---- mesh ------------------------------------------------------------------------------ */
 typedef struct gr_mesh gr_mesh;
