@@ -164,6 +164,9 @@ def main():
                                 s.n_cells * args.steps / dt32 * iteration_bytes(s, 4) / s.n_cells / 1e9},
     }
 
+    # achievable HBM rate of a plain copy kernel (2 x 1 GiB per launch, far beyond the caches)
+    out["stream_copy_GBps"] = gr.stream_copy_gbps(1 << 30, 20)
+
     if not args.no_converge:
         log("bench: time to a 1e-3 residual drop (binary64, cold start)")
         for name, dtype in (("f64", gr.GR_F64), ("mixed", gr.GR_F32)):

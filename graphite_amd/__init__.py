@@ -50,6 +50,13 @@ def device_count() -> int:
     return int(load().gr_device_count())
 
 
+def stream_copy_gbps(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Read+write GB/s of a plain device copy kernel over nbytes (bandwidth reference)."""
+    out = C.c_double()
+    _check(load().gr_stream_copy_probe(nbytes, reps, C.byref(out)))
+    return out.value
+
+
 # ---- element lists ----------------------------------------------------------------------
 
 class Elements:

@@ -100,6 +100,7 @@ SIGNATURES = {
     "gr_solver_residual_norm": (_i, [_vp, _dp]),
     "gr_solver_profile": (_i, [_vp, _i, _dp]),
     "gr_solver_kernel_bytes": (_i, [_vp, _dp]),
+    "gr_stream_copy_probe": (_i, [C.c_size_t, _i, _dp]),
 }
 
 _lib = None

@@ -7,7 +7,7 @@
 #include <cstring>
 #include <utility>
 
-#include "fvm_kernels.hpp"
+#include "fvm_tiles.hpp"
 
 namespace gr {
 
@@ -591,10 +591,11 @@ struct gr_solver {
         for (int k = 0; k < 15; ++k) a.g[k] = g<S>(k);
         return a;
     }
-    template <typename S> gr::GradLimArgs<S, 5> gl(int b) const {
-        gr::GradLimArgs<S, 5> a{};
+    template <typename S> gr::SolverArgs<S> sargs(int b) const {
+        gr::SolverArgs<S> a{};
         for (int c = 0; c < 5; ++c) {
             a.u[c] = u<S>(b, c);
+            a.un[c] = u<S>(1 - b, c);
             a.l[c] = l<S>(c);
         }
         for (int k = 0; k < 15; ++k) a.g[k] = g<S>(k);
@@ -614,44 +615,59 @@ struct gr_solver {
         }
         return t;
     }
-    // One iteration reading buffer b: BCs, gradient+limiter, flux+update into 1-b, norm.
-    // ev (optional) receives 5 events recorded before, between and after the kernels.
+    // One iteration reading buffer b: gradient+limiter, flux+update into 1-b, norm, with the
+    // boundary conditions evaluated inside both kernels.  ev (optional) receives 5 events:
+    // [0]-[1] the (fused, empty) BC stage, [1]-[2] gradient+limiter, [2]-[3] flux, [3]-[4] norm.
     template <typename S> void enqueue_iteration(int b, gr::SolverCtl *c, hipEvent_t *ev) {
         gr_mesh *m = mesh;
         const gr::BcTable t = table();
+        const gr::SolverArgs<S> a = sargs<S>(b);
+        const dim3 grid(m->ntiles), blk(gr::kBlock);
         if (ev) GR_HIP(hipEventRecord(ev[0], m->stream));
-        launch_bc<S>(m, args<S>(b), t, 0, m->h.n_ghosts, c);
         if (ev) GR_HIP(hipEventRecord(ev[1], m->stream));
-        launch_gradlim5<S>(m, gl<S>(b), cfg.limiter, c);
+        hipLaunchKernelGGL(gr::k_gradlim_tile<S>, grid, blk, 0, m->stream, m->view(), a, t, m->tiles.p, m->ntiles,
+                           cfg.limiter, c);
         if (ev) GR_HIP(hipEventRecord(ev[2], m->stream));
-        launch_flux<S>(m, cfg.corrected, args<S>(b), cfg.cfl, nullptr, nullptr, nullptr, m->partials.p, 0, c);
+        if (cfg.corrected)
+            hipLaunchKernelGGL((gr::k_flux_tile<S, true>), grid, blk, 0, m->stream, m->view(), a, t, m->tiles.p,
+                               m->ntiles, cfg.cfl, m->partials.p, c);
+        else
+            hipLaunchKernelGGL((gr::k_flux_tile<S, false>), grid, blk, 0, m->stream, m->view(), a, t, m->tiles.p,
+                               m->ntiles, cfg.cfl, m->partials.p, c);
         if (ev) GR_HIP(hipEventRecord(ev[3], m->stream));
         if (c)   // without a control block the caller reduces the partials itself
-            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, m->partials.p,
-                               gr::blocks_for(m->h.n_cells), (double *)nullptr, c, hist.p);
+            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, m->partials.p, m->ntiles,
+                               (double *)nullptr, c, hist.p);
         if (ev) GR_HIP(hipEventRecord(ev[4], m->stream));
     }
     void enqueue(int b, gr::SolverCtl *c, hipEvent_t *ev = nullptr) {
         if (cfg.dtype == GR_F64) enqueue_iteration<double>(b, c, ev);
         else enqueue_iteration<float>(b, c, ev);
     }
+    // Writes the boundary-condition ghost states of buffer b's interior into buffer b (the
+    // iteration kernels evaluate them on the fly and never store them).
+    void materialise_ghosts(int b) {
+        gr_mesh *m = mesh;
+        const gr::BcTable t = table();
+        if (cfg.dtype == GR_F64) launch_bc<double>(m, args<double>(b), t, 0, m->h.n_ghosts, nullptr);
+        else launch_bc<float>(m, args<float>(b), t, 0, m->h.n_ghosts, nullptr);
+        GR_HIP(hipGetLastError());
+    }
     // Algorithmic bytes per launch: each array element a kernel reads or writes, once.
     void kernel_bytes(double out[4]) const {
         const double N = mesh->h.n_cells, G = mesh->h.n_ghosts, F = mesh->h.n_faces;
         const double slots = mesh->h.cell_faces_starts[mesh->h.n_cells];   // interior cells' face slots
         const double su = (double)cfg.dtype, sg = 8.0;
-        // BC: ghost slot (face, owner) and starts, face centre+normal, owner centre, owner U
-        // read, ghost U written.
-        out[0] = 4.0 * (3 * G) + sg * (6 * G + 3 * G) + su * (5 * G + 5 * G);
-        // gradient+limiter: U of cells and ghosts, centres of cells and ghosts, LSQ matrix
-        // (6), face centres, CSR (starts, faces, neighbours); gradient (15) and limiter (5) out.
-        out[1] = su * (5 * (N + G)) + sg * (3 * (N + G) + 6 * N + 3 * F) + 4.0 * (N + 1 + 2 * slots) +
+        out[0] = 0.0;   // boundary conditions are evaluated inside the two kernels below
+        // gradient+limiter: U of cells, centres of cells and ghosts, LSQ matrix (6), face
+        // centres, normals of ghost faces (fused BC), CSR (starts, faces, neighbours);
+        // gradient (15) and limiter (5) written.
+        out[1] = su * (5 * N) + sg * (3 * (N + G) + 6 * N + 3 * F + 3 * G) + 4.0 * (N + 1 + 2 * slots) +
                  su * (20 * N);
-        // flux+update: U of cells and ghosts, gradient and limiter of cells, centres of cells
-        // and ghosts, volumes, face centre+normal+area, CSR; new U out.
-        out[2] = su * (5 * (N + G) + 20 * N) + sg * (3 * (N + G) + N + 7 * F) + 4.0 * (N + 1 + 2 * slots) +
-                 su * (5 * N);
-        out[3] = 8.0 * std::ceil(N / gr::kBlock);
+        // flux+update: U, gradient, limiter of cells, centres of cells and ghosts, volumes,
+        // face centre+normal+area, CSR; new U written.
+        out[2] = su * (25 * N) + sg * (3 * (N + G) + N + 7 * F) + 4.0 * (N + 1 + 2 * slots) + su * (5 * N);
+        out[3] = 8.0 * mesh->ntiles;
     }
     hipGraphExec_t graph(int b) {
         if (exec[b]) return exec[b];
@@ -674,8 +690,10 @@ struct gr_solver {
 
 namespace {
 
+// Writes the state into both Jacobi buffers: ghosts of regions without a boundary condition
+// are read from memory by whichever buffer is current.
 template <typename S>
-void upload_state(gr_solver *s, int b, const double *rho, const double *rhou, const double *rhoe) {
+void upload_state(gr_solver *s, const double *rho, const double *rhou, const double *rhoe) {
     const size_t n = (size_t)s->mesh->h.n_cells + s->mesh->h.n_ghosts;
     std::vector<S> soa(5 * s->stride, S(0));
     for (size_t i = 0; i < n; ++i) {
@@ -683,14 +701,19 @@ void upload_state(gr_solver *s, int b, const double *rho, const double *rhou, co
         for (int k = 0; k < 3; ++k) soa[(1 + k) * s->stride + i] = (S)rhou[3 * i + k];
         soa[4 * s->stride + i] = (S)rhoe[i];
     }
-    GR_HIP(hipMemcpyAsync(s->ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice, s->mesh->stream));
+    for (int b = 0; b < 2; ++b)
+        GR_HIP(hipMemcpyAsync(s->ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice,
+                              s->mesh->stream));
     GR_HIP(hipStreamSynchronize(s->mesh->stream));
 }
 
+// Interior cells from the current buffer; ghosts as the reference leaves them after a step:
+// the boundary-condition values computed from the state the last iteration read.
 template <typename S>
-void download_state(const gr_solver *s, double *rho, double *rhou, double *rhoe) {
+void download_state(gr_solver *s, double *rho, double *rhou, double *rhoe) {
     const gr_mesh *m = s->mesh;
     const size_t N = m->h.n_cells, n = N + m->h.n_ghosts;
+    if (s->ghost_src != s->cur) s->materialise_ghosts(s->ghost_src);
     std::vector<S> a(5 * s->stride), b(5 * s->stride);
     GR_HIP(hipMemcpyAsync(a.data(), s->ubuf[s->cur].p, a.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
     GR_HIP(hipMemcpyAsync(b.data(), s->ubuf[s->ghost_src].p, b.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
@@ -732,6 +755,7 @@ int gr_solver_create(gr_mesh *m, const gr_solver_config *cfg, gr_solver **out) {
             GR_HIP(hipMemsetAsync(s->grad.p, 0, s->grad.n, m->stream));
             GR_HIP(hipMemsetAsync(s->lim.p, 0, s->lim.n, m->stream));
             s->bcs.resize(m->h.n_regions);
+            m->ensure_tiles(gr::kTileSlots, gr::kTileCells);
             s->ctl.alloc(1);
             s->hist_cap = 1 << 17;
             s->hist.alloc(s->hist_cap);
@@ -771,10 +795,11 @@ int gr_solver_init_freestream(gr_solver *s, const double mach[3], double pressur
         double fs[5];
         freestream(mach, pressure, temperature, fs);
         const size_t n = (size_t)s->mesh->h.n_cells + s->mesh->h.n_ghosts;
-        for (int c = 0; c < 5; ++c) {
-            if (s->cfg.dtype == GR_F64) fill<double>(s->u<double>(s->cur, c), n, fs[c], s->mesh->stream);
-            else fill<float>(s->u<float>(s->cur, c), n, (float)fs[c], s->mesh->stream);
-        }
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 5; ++c) {
+                if (s->cfg.dtype == GR_F64) fill<double>(s->u<double>(b, c), n, fs[c], s->mesh->stream);
+                else fill<float>(s->u<float>(b, c), n, (float)fs[c], s->mesh->stream);
+            }
         s->ghost_src = s->cur;
         GR_HIP(hipGetLastError());
         GR_HIP(hipStreamSynchronize(s->mesh->stream));
@@ -784,8 +809,8 @@ int gr_solver_init_freestream(gr_solver *s, const double mach[3], double pressur
 int gr_solver_set_state(gr_solver *s, const double *rho, const double *rhou, const double *rhoe) {
     return guarded([&] {
         require(s && rho && rhou && rhoe, "null argument");
-        if (s->cfg.dtype == GR_F64) upload_state<double>(s, s->cur, rho, rhou, rhoe);
-        else upload_state<float>(s, s->cur, rho, rhou, rhoe);
+        if (s->cfg.dtype == GR_F64) upload_state<double>(s, rho, rhou, rhoe);
+        else upload_state<float>(s, rho, rhou, rhoe);
         s->ghost_src = s->cur;
     });
 }
@@ -793,8 +818,11 @@ int gr_solver_set_state(gr_solver *s, const double *rho, const double *rhou, con
 int gr_solver_get_state(const gr_solver *s, double *rho, double *rhou, double *rhoe) {
     return guarded([&] {
         require(s != nullptr, "null solver");
-        if (s->cfg.dtype == GR_F64) download_state<double>(s, rho, rhou, rhoe);
-        else download_state<float>(s, rho, rhou, rhoe);
+        // materialising the ghost states writes only ghost entries that the iteration never
+        // reads, so the logical state is unchanged
+        gr_solver *ms = const_cast<gr_solver *>(s);
+        if (s->cfg.dtype == GR_F64) download_state<double>(ms, rho, rhou, rhoe);
+        else download_state<float>(ms, rho, rhou, rhoe);
     });
 }
 
@@ -859,7 +887,7 @@ int gr_solver_residual_norm(gr_solver *s, double *out) {
         require(s && out, "null argument");
         s->enqueue(s->cur, nullptr, nullptr);   // BCs, gradient+limiter, flux partials; state not swapped
         GR_HIP(hipGetLastError());
-        *out = finalize_sum(s->mesh, gr::blocks_for(s->mesh->h.n_cells));
+        *out = finalize_sum(s->mesh, s->mesh->ntiles);
         s->ghost_src = s->cur;
     });
 }

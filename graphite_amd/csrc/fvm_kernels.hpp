@@ -33,15 +33,24 @@ template <typename S> struct Tol;
 template <> struct Tol<double> { static constexpr double v = 1e-12; };   // FVM_TOL, include/datatypes.cuh:45
 template <> struct Tol<float> { static constexpr float v = 1e-6f; };     // FVM_TOL, include/datatypes.cuh:39
 
+// Division and square root of the state type.  binary64: correctly rounded (the parity
+// path).  binary32 (mixed precision, no exactness claim): the hardware reciprocal and square
+// root (1 ulp), which replace the multi-instruction IEEE sequences.
 __device__ inline double dsqrt(double x) { return sqrt(x); }
-__device__ inline float dsqrt(float x) { return sqrtf(x); }
+__device__ inline float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline double fdiv(double a, double b) { return a / b; }
+__device__ inline float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// a/b of binary64 operands (geometry) rounded to the state type.
+template <typename S> __device__ inline S geo_ratio(double a, double b);
+template <> __device__ inline double geo_ratio<double>(double a, double b) { return a / b; }
+template <> __device__ inline float geo_ratio<float>(double a, double b) { return fdiv((float)a, (float)b); }
 template <typename S> __device__ inline S sgn_t(double x) { return (S)((double)(x > 0) - (double)(x < 0)); }
 
 // LIMITER_FUNC of src/fvmops.cu:118-125.
 template <typename S>
 __device__ inline S limiter_func(S y, int kind) {
     if (kind == GR_LIM_BARTH_JESPERSEN) return fmax(S(0), fmin(S(1), y));
-    if (kind == GR_LIM_VENKATAKRISHNAN) return (y * y + S(2) * y) / (y * y + y + S(2));
+    if (kind == GR_LIM_VENKATAKRISHNAN) return fdiv(y * y + S(2) * y, y * y + y + S(2));
     return (y < S(2)) ? (y - (S(0.25) * y) * y) : S(1);
 }
 
@@ -59,14 +68,14 @@ __device__ inline S limiter_face(S u, S gx, S gy, S gz, S umin, S umax, S dx, S 
     if (dmm2 <= k3v) {
         sig = S(1);
     } else if (dmm2 <= S(2) * k3v) {
-        const S y = dmm2 / k3v - S(1);
+        const S y = fdiv(dmm2, k3v) - S(1);
         sig = ((S(2) * y) * y) * y - (S(3) * y) * y + S(1);
     } else {
         sig = S(0);
     }
     S lim = S(1);
-    if (delta > Tol<S>::v) lim = limiter_func<S>(d_u / delta, kind);
-    else if (delta < -Tol<S>::v) lim = limiter_func<S>(d_d / delta, kind);
+    if (delta > Tol<S>::v) lim = limiter_func<S>(fdiv(d_u, delta), kind);
+    else if (delta < -Tol<S>::v) lim = limiter_func<S>(fdiv(d_d, delta), kind);
     lim *= S(0.8);
     return sig + (S(1) - sig) * lim;
 }

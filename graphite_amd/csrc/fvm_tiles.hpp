@@ -1,0 +1,361 @@
+// Slot-parallel tile kernels of the solver's iteration.
+//
+// A tile is a run of consecutive interior cells whose face slots fit one block (at most
+// kTileSlots); one thread evaluates one face slot of one cell, the per-slot terms go to LDS,
+// and one thread per cell then accumulates its slots in slot order -- the order of the
+// reference's per-cell loops -- so the binary64 results equal the one-thread-per-cell
+// kernels bit for bit while the launch has one thread per slot instead of one per cell.
+//
+// Boundary conditions are fused: a ghost's only neighbour is the cell whose face it lies on,
+// so its state (euler_slip_bc / euler_inletoutlet_bc, src/models/euler.cu:412-520) is a
+// function of that cell's state and the face normal, evaluated where a kernel needs it.
+// Ghost states are therefore never stored during the iteration.
+//
+// Tiles are mapped to blocks so that each of the 8 XCDs of the MI355X walks a contiguous
+// range of tiles (workgroups are dispatched round-robin over XCDs): neighbours in the
+// reverse Cuthill-McKee band then hit the XCD's own L2.
+#pragma once
+
+#include "fvm_kernels.hpp"
+
+namespace gr {
+
+constexpr int kTileSlots = 248;   // 20 binary64 rows x 248 slots = 39.7 KB LDS: 4 blocks per CU
+constexpr int kTileCells = 51;    // one thread per (component, cell) in the per-cell phases
+static_assert(5 * kTileCells <= kBlock, "per-cell phases need 5 threads per cell");
+
+// Bijection workgroup id -> tile such that XCD x (= w mod 8) gets a contiguous tile range.
+__device__ inline uint32_t xcd_tile(uint32_t w, uint32_t T) {
+    const uint32_t x = w & 7u, k = w >> 3;
+    return x * (T >> 3) + min(x, T & 7u) + k;
+}
+
+// Ghost state from the state of its cell, outward normal n (src/models/euler.cu:436-441,
+// :473-518).  Same arithmetic, in the same order, as k_euler_bc.
+template <typename S>
+__device__ inline void ghost_state(const BcRegion &R, const S n[3], const S u1[5], S g[5]) {
+    const S r1 = u1[0];
+    const S ru1[3] = {u1[1], u1[2], u1[3]};
+    const S e1 = u1[4];
+    if (R.kind == GR_BC_SLIP) {
+        const S un = (n[0] * ru1[0] + n[1] * ru1[1]) + n[2] * ru1[2];
+        g[0] = r1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g[1 + k] = ru1[k] - (n[k] * S(2)) * un;
+        g[4] = e1;
+        return;
+    }
+    const S u[3] = {fdiv(ru1[0], r1), fdiv(ru1[1], r1), fdiv(ru1[2], r1)};
+    const S p = S(0.4) * (e1 - fdiv(S(0.5), r1) * ((ru1[0] * ru1[0] + ru1[1] * ru1[1]) + ru1[2] * ru1[2]));
+    const S tt = fdiv(p, S(1) * r1);
+    const S mach_i = dsqrt(fdiv((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2], fdiv(S(1.4) * p, r1)));
+    const S p_in = (S)R.p, T_in = (S)R.T;
+    const S mach[3] = {(S)R.mach[0], (S)R.mach[1], (S)R.mach[2]};
+    const S rho_in = fdiv(p_in, S(1) * T_in);
+    const S cin = dsqrt(fdiv(S(1.4) * p_in, rho_in));
+    if ((u[0] * n[0] + u[1] * n[1]) + u[2] * n[2] < S(0)) {
+        if (mach_i > S(1)) {
+            const S vin[3] = {mach[0] * cin, mach[1] * cin, mach[2] * cin};
+            g[0] = rho_in;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g[1 + k] = (mach[k] * rho_in) * cin;
+            g[4] = fdiv(p_in, S(0.4)) + (S(0.5) * rho_in) * ((vin[0] * vin[0] + vin[1] * vin[1]) + vin[2] * vin[2]);
+        } else {
+            const S r0 = fdiv(p, S(1) * T_in);
+            const S c = dsqrt(fdiv(S(1.4) * p, r0));
+            const S v[3] = {mach[0] * c, mach[1] * c, mach[2] * c};
+            g[0] = r0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g[1 + k] = v[k] * r0;
+            g[4] = fdiv(p, S(0.4)) + (S(0.5) * r0) * ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        }
+    } else {
+        if (mach_i > S(1)) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) g[k] = u1[k];
+        } else {
+            const S r0 = fdiv(p_in, S(1) * tt);
+            g[0] = r0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g[1 + k] = u[k] * r0;
+            g[4] = fdiv(p_in, S(0.4)) + (S(0.5) * r0) * ((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+        }
+    }
+}
+
+// Index of the table region holding ghost g, or -1 (ghost state then read from memory).
+__device__ inline int bc_region(const BcTable &t, uint32_t g) {
+    int r = -1;
+    for (int k = 0; k < t.n; ++k)
+        if (g >= t.r[k].begin && g < t.r[k].end) r = k;
+    return r;
+}
+
+// Outward unit normal of face f seen from the cell centred at (xi,yi,zi): the reference's
+// outer_normal(n_f, fc - x_i) (include/datatypes.cuh:213-215).
+template <typename S>
+__device__ inline void outward_normal(const MeshView &m, uint32_t f, double xi, double yi, double zi, S n[3]) {
+    const double dx = m.fcx[f] - xi, dy = m.fcy[f] - yi, dz = m.fcz[f] - zi;
+    const double nfx = m.fnx[f], nfy = m.fny[f], nfz = m.fnz[f];
+    const double dn = (nfx * dx + nfy * dy) + nfz * dz;
+    const double sg = (double)(dn > 0) - (double)(dn < 0);
+    n[0] = (S)(nfx * sg);
+    n[1] = (S)(nfy * sg);
+    n[2] = (S)(nfz * sg);
+}
+
+// Neighbour state across slot q of cell i: interior cells from memory, ghosts from the fused
+// boundary condition of their region (or from memory when their region has none).
+template <typename S>
+__device__ inline void neighbour_state(const MeshView &m, const BcTable &t, const S *const u[5], uint32_t i, uint32_t j,
+                                       uint32_t f, double xi, double yi, double zi, const S ui[5], S uj[5]) {
+    int r = j >= m.N ? bc_region(t, j - m.N) : -1;
+    if (r < 0) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) uj[c] = u[c][j];
+        return;
+    }
+    S n[3];
+    outward_normal<S>(m, f, xi, yi, zi, n);
+    ghost_state<S>(t.r[r], n, ui, uj);
+}
+
+// Locates the local cell of local slot s among the tile's starts (ncell+1 entries).
+__device__ inline int tile_cell_of(const uint32_t *starts, int ncell, uint32_t s) {
+    int lo = 0, hi = ncell;   // starts[lo] <= s < starts[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (starts[mid] <= s) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename S>
+struct SolverArgs {
+    const S *u[5];    // current state (cells; ghosts only for regions without a BC)
+    S *un[5];         // next state
+    S *g[15];         // gradients
+    S *l[5];          // limiters
+};
+
+// Gradient (LSQ) + neighbour min/max + limiter of the five conservative components
+// (FvmEulerStepPrepare, src/models/euler.cu:597-647).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<S> a, BcTable bc,
+                                                         const uint32_t *__restrict__ tiles, uint32_t ntiles,
+                                                         int kind, const SolverCtl *ctl) {
+    if (ctl && ctl->stop) return;
+    __shared__ uint32_t s_start[kTileCells + 1];
+    __shared__ S s_buf[20][kTileSlots];
+    const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
+    const uint32_t c0 = tiles[tile], c1 = tiles[tile + 1];
+    const int ncell = (int)(c1 - c0);
+    const uint32_t q0 = m.cfs[c0];
+    const int nslot = (int)(m.cfs[c1] - q0);
+    const int t = threadIdx.x;
+    for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = m.cfs[c0 + k] - q0;
+    __syncthreads();
+
+    // A: per-slot gradient terms d*(u_j-u_i)/|d|^2 (rows 0..14) and u_j (rows 15..19)
+    int lc = 0;
+    uint32_t i = 0, q = 0;
+    double xi = 0, yi = 0, zi = 0;
+    S ui[5];
+    if (t < nslot) {
+        lc = tile_cell_of(s_start, ncell, (uint32_t)t);
+        i = c0 + lc;
+        q = q0 + t;
+        xi = m.cx[i];
+        yi = m.cy[i];
+        zi = m.cz[i];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) ui[c] = a.u[c][i];
+        const uint32_t j = m.cc[q];
+        S uj[5];
+        neighbour_state<S>(m, bc, a.u, i, j, m.cf[q], xi, yi, zi, ui, uj);
+        const S d[3] = {(S)(m.cx[j] - xi), (S)(m.cy[j] - yi), (S)(m.cz[j] - zi)};
+        const S w2 = fdiv(S(1), (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const S du = uj[c] - ui[c];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s_buf[3 * c + k][t] = (d[k] * du) * w2;
+            s_buf[15 + c][t] = uj[c];
+        }
+    }
+    __syncthreads();
+
+    // B: one thread per (component, cell): sum in slot order, LSQ apply, min/max.  The
+    // gradient goes to the cell's first slot column (rows 3c..3c+2), min/max to its second
+    // (rows c and 5+c) -- rows other threads of the cell still read, hence the barrier between
+    // the reads and the writes.
+    const bool in_b = t < 5 * ncell;
+    const int bc_c = in_b ? t / ncell : 0, bc_l = in_b ? t - bc_c * ncell : 0;
+    S bg[3] = {S(0), S(0), S(0)}, bmin = S(0), bmax = S(0);
+    int bsb = 0;
+    if (in_b) {
+        const uint32_t ic = c0 + bc_l;
+        bsb = (int)s_start[bc_l];
+        const int se = (int)s_start[bc_l + 1];
+        S g3[3] = {S(0), S(0), S(0)};
+        bmin = bmax = a.u[bc_c][ic];
+        for (int s = bsb; s < se; ++s) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g3[k] += s_buf[3 * bc_c + k][s];
+            bmin = fmin(bmin, s_buf[15 + bc_c][s]);
+            bmax = fmax(bmax, s_buf[15 + bc_c][s]);
+        }
+        lsq_apply<S>(m, ic, g3, bg);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a.g[3 * bc_c + k][ic] = bg[k];
+    }
+    __syncthreads();
+    if (in_b) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s_buf[3 * bc_c + k][bsb] = bg[k];
+        s_buf[bc_c][bsb + 1] = bmin;
+        s_buf[5 + bc_c][bsb + 1] = bmax;
+    }
+    __syncthreads();
+
+    // C: per-slot limiter of each component at this slot's face centre.  Reads rows 0..14 of
+    // the cell's first two slot columns, writes rows 15..19 of its own column (u_j, consumed
+    // in B): disjoint, so no barrier is needed between the reads and the writes.
+    if (t < nslot) {
+        const int sb = (int)s_start[lc];
+        const uint32_t f = m.cf[q];
+        const S dx = (S)(m.fcx[f] - xi), dy = (S)(m.fcy[f] - yi), dz = (S)(m.fcz[f] - zi);
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+            s_buf[15 + c][t] = limiter_face<S>(ui[c], s_buf[3 * c][sb], s_buf[3 * c + 1][sb], s_buf[3 * c + 2][sb],
+                                               s_buf[c][sb + 1], s_buf[5 + c][sb + 1], dx, dy, dz, kind);
+    }
+    __syncthreads();
+
+    // D: one thread per (component, cell): minimum over the cell's faces, starting from 2
+    if (in_b) {
+        const int se = (int)s_start[bc_l + 1];
+        S l = S(2);
+        for (int s = bsb; s < se; ++s) l = fmin(l, s_buf[15 + bc_c][s]);
+        a.l[bc_c][c0 + bc_l] = l;
+    }
+}
+
+// Flux + local-time-step update (fvmEulerSteadyOp + euler_steady_step,
+// src/models/euler.cu:236-381), Jacobi, with V*|R|^2 block partials.
+template <typename S, bool CORR>
+__global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
+                                                      const uint32_t *__restrict__ tiles, uint32_t ntiles, double cfl,
+                                                      double *partials, const SolverCtl *ctl) {
+    if (ctl && ctl->stop) return;
+    __shared__ uint32_t s_start[kTileCells + 1];
+    __shared__ S s_c[5][kTileSlots];   // central flux term ((F0+F1)*ds)*0.5
+    __shared__ S s_d[5][kTileSlots];   // dissipation term ((eig*(U1-U0))*0.5)*ds
+    __shared__ S s_e[kTileSlots];      // eig*ds
+    const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
+    const uint32_t c0 = tiles[tile], c1 = tiles[tile + 1];
+    const int ncell = (int)(c1 - c0);
+    const uint32_t q0 = m.cfs[c0];
+    const int nslot = (int)(m.cfs[c1] - q0);
+    const int t = threadIdx.x;
+    for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = m.cfs[c0 + k] - q0;
+    __syncthreads();
+
+    if (t < nslot) {
+        const int lc = tile_cell_of(s_start, ncell, (uint32_t)t);
+        const uint32_t i = c0 + lc, q = q0 + t;
+        const uint32_t f = m.cf[q], j = m.cc[q];
+        const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+        const double fcx = m.fcx[f], fcy = m.fcy[f], fcz = m.fcz[f];
+        S n[3];
+        outward_normal<S>(m, f, xi, yi, zi, n);
+        const S ds = geo_ratio<S>(m.fa[f], m.cv[i]);
+        const S d0[3] = {(S)(fcx - xi), (S)(fcy - yi), (S)(fcz - zi)};
+        const S d1[3] = {(S)(fcx - m.cx[j]), (S)(fcy - m.cy[j]), (S)(fcz - m.cz[j])};
+        S ui[5], gi[15], li[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            ui[c] = a.u[c][i];
+            li[c] = a.l[c][i];
+        }
+#pragma unroll
+        for (int k = 0; k < 15; ++k) gi[k] = a.g[k][i];
+        S r0, ru0[3], e0, r1, ru1[3], e1;
+        reconstruct<S, CORR>(ui, gi, li, d0, r0, ru0, e0);
+        if (j < m.N) {
+            S uj[5], gj[15], lj[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                uj[c] = a.u[c][j];
+                lj[c] = a.l[c][j];
+            }
+#pragma unroll
+            for (int k = 0; k < 15; ++k) gj[k] = a.g[k][j];
+            reconstruct<S, CORR>(uj, gj, lj, d1, r1, ru1, e1);
+        } else {   // ghost: its state from the BC, gradient and limiter of cell i
+            S ug[5];
+            const int r = bc_region(bc, j - m.N);
+            if (r < 0) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) ug[c] = a.u[c][j];
+            } else {
+                ghost_state<S>(bc.r[r], n, ui, ug);
+            }
+            reconstruct<S, CORR>(ug, gi, li, d1, r1, ru1, e1);
+        }
+        const S p0 = S(0.4) * (e0 - fdiv(S(0.5), r0) * ((ru0[0] * ru0[0] + ru0[1] * ru0[1]) + ru0[2] * ru0[2]));
+        const S p1 = S(0.4) * (e1 - fdiv(S(0.5), r1) * ((ru1[0] * ru1[0] + ru1[1] * ru1[1]) + ru1[2] * ru1[2]));
+        const S V0 = fdiv((ru0[0] * n[0] + ru0[1] * n[1]) + ru0[2] * n[2], r0);
+        const S V1 = fdiv((ru1[0] * n[0] + ru1[1] * n[1]) + ru1[2] * n[2], r1);
+        s_c[0][t] = (((V0 * r0) + (V1 * r1)) * ds) * S(0.5);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const S fu0 = ru0[k] * V0 + n[k] * p0;
+            const S fu1 = ru1[k] * V1 + n[k] * p1;
+            s_c[1 + k][t] = ((fu0 + fu1) * ds) * S(0.5);
+        }
+        s_c[4][t] = (((V0 * (e0 + p0)) + (V1 * (e1 + p1))) * ds) * S(0.5);
+        const S v0[3] = {fdiv(ru0[0], r0), fdiv(ru0[1], r0), fdiv(ru0[2], r0)};
+        const S v1[3] = {fdiv(ru1[0], r1), fdiv(ru1[1], r1), fdiv(ru1[2], r1)};
+        const S cs0 = dsqrt(fmax(S(1e-12), fmin(S(1e30), fdiv(S(1.4) * p0, r0))));
+        const S cs1 = dsqrt(fmax(S(1e-12), fmin(S(1e30), fdiv(S(1.4) * p1, r1))));
+        const S eig = fmax(dsqrt((v0[0] * v0[0] + v0[1] * v0[1]) + v0[2] * v0[2]) + cs0,
+                           dsqrt((v1[0] * v1[0] + v1[1] * v1[1]) + v1[2] * v1[2]) + cs1);
+        s_d[0][t] = ((eig * (r1 - r0)) * S(0.5)) * ds;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s_d[1 + k][t] = (((ru1[k] - ru0[k]) * eig) * S(0.5)) * ds;
+        s_d[4][t] = ((eig * (e1 - e0)) * S(0.5)) * ds;
+        s_e[t] = eig * ds;
+    }
+    __syncthreads();
+
+    // one thread per (component, cell): accumulate the slots in order, update; the cell's
+    // residual components meet in LDS (row 0 of s_c, free after the reads) for V*|R|^2
+    const bool in_b = t < 5 * ncell;
+    const int cc_ = in_b ? t / ncell : 0, cl = in_b ? t - cc_ * ncell : 0;
+    S acc = 0;
+    if (in_b) {
+        const uint32_t i = c0 + cl;
+        const int sb = (int)s_start[cl], se = (int)s_start[cl + 1];
+        S dts = 0;
+        for (int s = sb; s < se; ++s) {
+            acc = (acc + s_c[cc_][s]) - s_d[cc_][s];
+            dts += s_e[s];
+        }
+        const S dt = fdiv((S)cfl, dts);
+        a.un[cc_][i] = a.u[cc_][i] + (-acc) * dt;
+    }
+    __syncthreads();
+    if (in_b) s_d[cc_][cl] = acc;
+    __syncthreads();
+    double e_loc = 0;
+    if (t < ncell) {
+        const double r = s_d[0][t], u0 = s_d[1][t], u1 = s_d[2][t], u2 = s_d[3][t], e = s_d[4][t];
+        e_loc = ((r * r + ((u0 * u0 + u1 * u1) + u2 * u2)) + e * e) * m.cv[c0 + t];
+    }
+    const double bs = block_sum(e_loc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+}  // namespace gr

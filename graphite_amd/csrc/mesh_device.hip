@@ -211,6 +211,29 @@ void gr_mesh::ensure_device() {
     on_device = true;
 }
 
+// Greedy cut of the interior cells into runs of at most max_slots face slots and max_cells
+// cells (one block of the slot-parallel kernels each).
+void gr_mesh::ensure_tiles(uint32_t max_slots, uint32_t max_cells) {
+    if (tiles.p) return;
+    ensure_device();
+    std::vector<uint32_t> t{0};
+    const auto &s = h.cell_faces_starts;
+    uint32_t first = 0;
+    for (uint32_t c = 0; c < h.n_cells; ++c) {
+        gr::require(s[c + 1] - s[c] <= max_slots, "cell with more face slots than a tile holds");
+        if (s[c + 1] - s[first] > max_slots || c + 1 - first > max_cells) {
+            t.push_back(c);
+            first = c;
+        }
+    }
+    if (t.back() != h.n_cells) t.push_back(h.n_cells);
+    ntiles = (uint32_t)t.size() - 1;
+    tiles.alloc(t.size());
+    tiles.upload(t.data(), t.size(), stream);
+    if (partials.n < ntiles + 64) partials.alloc(ntiles + 64);
+    GR_HIP(hipStreamSynchronize(stream));
+}
+
 gr::MeshView gr_mesh::view() const {
     gr::MeshView v;
     v.N = h.n_cells;
@@ -452,3 +475,55 @@ int gr_mesh_get_metrics(const gr_mesh *m, double *face_centers, double *face_nor
 }
 
 }  // extern "C"
+
+// ---- bandwidth probe -------------------------------------------------------------------------
+
+namespace gr {
+// 16-byte vector copy, grid-stride.
+__global__ __launch_bounds__(kBlock) void k_copy_probe(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                       size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace gr
+
+extern "C" int gr_stream_copy_probe(size_t bytes, int reps, double *gbps) {
+    return gr::guarded([&] {
+        gr::require(gbps != nullptr, "null argument");
+        gr::require(bytes >= 16 && bytes % 16 == 0 && bytes <= (size_t(64) << 30), "bytes must be a multiple of 16, <= 64 GiB");
+        gr::require(reps > 0 && reps <= 1000, "reps must be in 1..1000");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw gr::Error(GR_ERR_DEVICE, "no HIP device available");
+        gr::DevArray<double2> a, b;
+        const size_t n = bytes / 16;
+        a.alloc(n);
+        b.alloc(n);
+        hipStream_t s;
+        GR_HIP(hipStreamCreate(&s));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        struct Cleanup {
+            hipStream_t &s;
+            hipEvent_t &e0, &e1;
+            ~Cleanup() {
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+        } cleanup{s, e0, e1};
+        GR_HIP(hipMemsetAsync(a.p, 0, bytes, s));
+        GR_HIP(hipEventCreate(&e0));
+        GR_HIP(hipEventCreate(&e1));
+        const unsigned grid = (unsigned)std::min<size_t>((n + gr::kBlock - 1) / gr::kBlock, 256 * 64);
+        hipLaunchKernelGGL(gr::k_copy_probe, dim3(grid), dim3(gr::kBlock), 0, s, a.p, b.p, n);   // warm-up
+        GR_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(gr::k_copy_probe, dim3(grid), dim3(gr::kBlock), 0, s, a.p, b.p, n);
+        GR_HIP(hipEventRecord(e1, s));
+        GR_HIP(hipEventSynchronize(e1));
+        GR_HIP(hipGetLastError());
+        float ms = 0;
+        GR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    });
+}

@@ -266,6 +266,11 @@ int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]);
  * counted once), same order as gr_solver_profile. */
 int gr_solver_kernel_bytes(const gr_solver *s, double bytes[4]);
 
+/* Bandwidth probe (no reference counterpart): a plain copy kernel over two device buffers of
+ * `bytes` each, run `reps` times; gbps receives (read + written bytes) / mean kernel time.
+ * Calibrates hardware byte counters and measures the achievable HBM rate. */
+int gr_stream_copy_probe(size_t bytes, int reps, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -219,6 +219,28 @@ def test_solver_iterations_match_oracle(gr, orc, c1):
     assert normwise(r, s.rho) <= TOL and normwise(u, s.rhou) <= TOL and normwise(e, s.rhoe) <= TOL
 
 
+@pytest.mark.parametrize("mach", [(0.5, 0.1, 0.0), (1.6, 0.0, 0.2)])
+def test_solver_mixed_bcs_and_unset_region_match_oracle(gr, orc, c1, mach):
+    # supersonic and subsonic far field, slip walls, and one region without a boundary
+    # condition (its ghosts keep the values they were given, as in the reference when a
+    # patch has no "bcs" entry)
+    m, om, _ = c1
+    state = perturbed_state(om.cell_centers, mach=mach)
+    names = m.region_names()
+    kinds = ["farfield", "farfield", "slip", "slip", None, "farfield"]
+    bcs = [(nm, k, mach) for nm, k in zip(names, kinds) if k]
+    sv, res = _solver_run(gr, m, gr.GR_F64, bcs, state, 12)
+    s = _oracle_state(orc, om, *state)
+    obcs = [(om.region(nm), orc.BC_FARFIELD if k == "farfield" else orc.BC_SLIP, mc, P_INF, T_INF)
+            for nm, k, mc in bcs]
+    n, ohist = orc.iterate(om, s, obcs, 0.5, 12, threads=8)
+    assert normwise(sv.history(12), ohist) <= TOL
+    r, u, e = sv.get_state()
+    assert normwise(r, s.rho) <= TOL and normwise(u, s.rhou) <= TOL and normwise(e, s.rhoe) <= TOL
+    N = om.n_cells
+    assert np.array_equal(r[:N], s.rho[:N]) and np.array_equal(u[:N], s.rhou[:N])
+
+
 def test_solver_graph_equals_stream_and_is_deterministic(gr, orc, c1):
     m, om, _ = c1
     mach = (0.5, 0.0, 0.0)
