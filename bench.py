@@ -87,6 +87,64 @@ def cpu_baseline(gr, e, budget_s=15.0):
                        f"{threads} OpenMP threads, {t:.1f} s")
 
 
+CGRID_BCS = [("airfoil", "slip"), ("farfield", "farfield"), ("zmin", "slip"), ("zmax", "slip")]
+
+
+def cgrid_machs(n, seed=2024):
+    """Mach ~ U[0.5, 0.85], angle of attack ~ U[-2, 6] degrees (seeded)."""
+    rng = np.random.default_rng(seed)
+    M = rng.uniform(0.5, 0.85, n)
+    a = np.deg2rad(rng.uniform(-2.0, 6.0, n))
+    return np.stack([M * np.cos(a), M * np.sin(a), np.zeros(n)], axis=1)
+
+
+def make_batch(gr, m, dtype, cfl, machs):
+    b = gr.Batch(m, len(machs), dtype=dtype, cfl=cfl)
+    for name, kind in CGRID_BCS:
+        b.set_bc(name, gr.GR_BC_FARFIELD if kind == "farfield" else gr.GR_BC_SLIP)
+    b.set_freestream(machs, P_INF, T_INF)
+    return b
+
+
+def batched_leg(gr, args):
+    """64 instances on the NACA0012 C-grid (384x96x1 = 36,864 hexes) per launch."""
+    log("bench: batched C-grid leg")
+    m = gr.Mesh.build(gr.Elements.cgrid_naca())
+    m.renumber_rcm()
+    m.compute_metrics()
+    s = m.sizes()
+    machs = cgrid_machs(64)
+    res = {"workload": "NACA0012 C-grid 384x96x1 (36,864 hexes) x 64 instances, Mach U[0.5,0.85], "
+                       "AoA U[-2,6] deg", "instances": 64, "cells": s.n_cells}
+    steps = max(20, args.steps // 4)
+    for name, dtype in (("f64", gr.GR_F64), ("mixed", gr.GR_F32)):
+        b = make_batch(gr, m, dtype, args.cfl, machs)
+        b.iterate(args.warmup)
+        t0 = time.perf_counter()
+        b.iterate(steps)
+        dt = time.perf_counter() - t0
+        prof = b.profile(20)
+        kb = b.kernel_bytes()
+        res[name] = {"value": s.n_cells * 64 * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / steps * 1e3,
+                     "kernels": {k: {"ms": prof[k], "bytes": kb[k], "GBps": kb[k] / (prof[k] * 1e-3) / 1e9}
+                                 for k in prof}}
+    if not args.no_converge:
+        b = make_batch(gr, m, gr.GR_F64, args.cfl, machs)
+        t0 = time.perf_counter()
+        its, conv, ran = b.iterate(20000, tol=1e-3)
+        wall = time.perf_counter() - t0
+        per_iter = wall / max(ran, 1)
+        amort = its * per_iter / 64.0
+        res["time_to_1e-3_f64_cold"] = {"batch_seconds": wall, "batch_iterations": int(ran),
+                                        "converged": int(conv.sum()),
+                                        "iterations_p50": float(np.percentile(its, 50)),
+                                        "iterations_p99": float(np.percentile(its, 99)),
+                                        "seconds_per_instance_p50": float(np.percentile(amort, 50)),
+                                        "seconds_per_instance_p99": float(np.percentile(amort, 99)),
+                                        "normalisation": "residual of iteration 0 per instance"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +153,7 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -166,6 +225,9 @@ def main():
 
     # achievable HBM rate of a plain copy kernel (2 x 1 GiB per launch, far beyond the caches)
     out["stream_copy_GBps"] = gr.stream_copy_gbps(1 << 30, 20)
+
+    if not args.no_batch:
+        out["batched"] = batched_leg(gr, args)
 
     if not args.no_converge:
         log("bench: time to a 1e-3 residual drop (binary64, cold start)")

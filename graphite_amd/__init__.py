@@ -435,3 +435,172 @@ class Solver:
         b = np.zeros(4)
         _check(load().gr_solver_kernel_bytes(self._h, _p(b, C.c_double)))
         return dict(zip(self.KERNELS, b.tolist()))
+
+
+class Batch:
+    """Up to 64 Euler instances (own far-field Mach vectors) on one mesh, advanced together."""
+
+    KERNELS = ("grad_lim", "flux_update", "residual_reduce")
+
+    def __init__(self, mesh: Mesh, n_inst: int, dtype: int = GR_F64, cfl: float = 0.5,
+                 limiter: int = GR_LIM_MICHALAK, corrected: bool = False, use_graph: bool = True):
+        cfg = SolverConfig(dtype, cfl, limiter, int(corrected), int(use_graph))
+        h = C.c_void_p()
+        _check(load().gr_batch_create(mesh._h, n_inst, C.byref(cfg), C.byref(h)))
+        self._h, self.mesh, self.n_inst, self.dtype = h, mesh, n_inst, dtype
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().gr_batch_free(h)
+            self._h = None
+
+    def set_bc(self, region: str, kind: int) -> None:
+        _check(load().gr_batch_set_bc(self._h, region.encode(), kind))
+
+    def set_freestream(self, mach, p: float, T: float) -> None:
+        m = _f64(mach).reshape(-1)
+        assert m.size == 3 * self.n_inst
+        _check(load().gr_batch_set_freestream(self._h, _p(m, C.c_double), p, T))
+
+    def set_state(self, inst: int, rho, rhou, rhoe) -> None:
+        r, u, e = _f64(rho), _f64(rhou).reshape(-1), _f64(rhoe)
+        _check(load().gr_batch_set_state(self._h, inst, _p(r, C.c_double), _p(u, C.c_double), _p(e, C.c_double)))
+
+    def get_state(self, inst: int):
+        s = self.mesh.sizes()
+        n = s.n_cells + s.n_ghosts
+        r, u, e = np.zeros(n), np.zeros(3 * n), np.zeros(n)
+        _check(load().gr_batch_get_state(self._h, inst, _p(r, C.c_double), _p(u, C.c_double), _p(e, C.c_double)))
+        return r, u.reshape(-1, 3), e
+
+    def iterate(self, max_iter: int, tol: float = 0.0, reference=None):
+        """Returns (iterations per instance, converged per instance, batch iterations run)."""
+        its = (C.c_int * self.n_inst)()
+        conv = (C.c_int * self.n_inst)()
+        launched = C.c_int()
+        ref = None if reference is None else _f64(reference)
+        _check(load().gr_batch_iterate(self._h, max_iter, tol, _p(ref, C.c_double) if ref is not None else None,
+                                       its, conv, C.byref(launched)))
+        return np.array(its[:], dtype=int), np.array(conv[:], dtype=bool), launched.value
+
+    def history(self, inst: int, n: int) -> np.ndarray:
+        h = np.zeros(max(n, 1))
+        _check(load().gr_batch_history(self._h, inst, _p(h, C.c_double), n))
+        return h[:n]
+
+    def residual_norms(self) -> np.ndarray:
+        out = np.zeros(self.n_inst)
+        _check(load().gr_batch_residual_norms(self._h, _p(out, C.c_double)))
+        return out
+
+    def profile(self, n_iter: int) -> dict:
+        ms = np.zeros(3)
+        _check(load().gr_batch_profile(self._h, n_iter, _p(ms, C.c_double)))
+        return dict(zip(self.KERNELS, ms.tolist()))
+
+    def kernel_bytes(self) -> dict:
+        b = np.zeros(3)
+        _check(load().gr_batch_kernel_bytes(self._h, _p(b, C.c_double)))
+        return dict(zip(self.KERNELS, b.tolist()))
+
+    def set_graph_iters(self, iters: int) -> None:
+        _check(load().gr_batch_set_graph_iters(self._h, iters))
+
+
+def _solver_force(self, region: str) -> np.ndarray:
+    F = np.zeros(3)
+    _check(load().gr_solver_force(self._h, region.encode(), _p(F, C.c_double)))
+    return F
+
+
+def _batch_force(self, region: str) -> np.ndarray:
+    F = np.zeros(3 * self.n_inst)
+    _check(load().gr_batch_force(self._h, region.encode(), _p(F, C.c_double)))
+    return F.reshape(-1, 3)
+
+
+def _wall_distance(self, regions) -> np.ndarray:
+    names = (C.c_char_p * len(regions))(*[r.encode() for r in regions])
+    out = np.zeros(self.n_cells)
+    _check(load().gr_mesh_wall_distance(self._h, names, len(regions), _p(out, C.c_double)))
+    return out
+
+
+Solver.force = _solver_force
+Batch.force = _batch_force
+Mesh.wall_distance = _wall_distance
+
+
+def lift_drag(force, mach, rho_inf: float, p_inf: float, s_ref: float):
+    """(C_L, C_D) of a force for a far field with Mach vector `mach` in the x-y plane:
+    drag along the free-stream velocity, lift perpendicular to it in the x-y plane."""
+    mach = np.asarray(mach, float)
+    c_inf = np.sqrt(1.4 * p_inf / rho_inf)
+    u = mach * c_inf
+    q = 0.5 * rho_inf * float(u @ u) * s_ref
+    ed = u / np.linalg.norm(u)
+    el = np.array([-ed[1], ed[0], 0.0])
+    return float(force @ el) / q, float(force @ ed) / q
+
+
+# ---- warm-start network -----------------------------------------------------------------------
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """4-layer MLP n_in -> 256 -> 256 -> 256 -> n_out (tanh hidden layers) on bfloat16 matrix
+    cores with binary32 accumulation; binary32 master parameters in the padded layout of
+    include/graphite_hip.h (per layer W [in_pad][out_pad] then b [out_pad])."""
+
+    SHAPES = ((64, 256), (256, 256), (256, 256), (256, 32))
+
+    def __init__(self, n_in: int, n_out: int, seed: int = 0):
+        h = C.c_void_p()
+        _check(load().gr_mlp_create(n_in, n_out, seed, C.byref(h)))
+        self._h, self.n_in, self.n_out = h, n_in, n_out
+        n = C.c_size_t()
+        _check(load().gr_mlp_n_params(h, C.byref(n)))
+        self.n_params = n.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().gr_mlp_free(h)
+            self._h = None
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _check(load().gr_mlp_get_params(self._h, _p(p, C.c_float)))
+        return p
+
+    def set_params(self, p) -> None:
+        p = _f32(p)
+        assert p.size == self.n_params
+        _check(load().gr_mlp_set_params(self._h, _p(p, C.c_float)))
+
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x)
+        assert x.ndim == 2 and x.shape[1] == self.n_in
+        y = np.zeros((x.shape[0], self.n_out), np.float32)
+        _check(load().gr_mlp_forward(self._h, _p(x, C.c_float), x.shape[0], _p(y, C.c_float)))
+        return y
+
+    def loss_grad(self, x, t):
+        x, t = _f32(x), _f32(t)
+        assert x.shape[1] == self.n_in and t.shape == (x.shape[0], self.n_out)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        _check(load().gr_mlp_loss_grad(self._h, _p(x, C.c_float), _p(t, C.c_float), x.shape[0],
+                                       C.byref(loss), _p(g, C.c_float)))
+        return loss.value, g
+
+    def train_step(self, x, t, lr: float) -> float:
+        x, t = _f32(x), _f32(t)
+        assert x.shape[1] == self.n_in and t.shape == (x.shape[0], self.n_out)
+        loss = C.c_double()
+        _check(load().gr_mlp_train_step(self._h, _p(x, C.c_float), _p(t, C.c_float), x.shape[0], lr,
+                                        C.byref(loss)))
+        return loss.value

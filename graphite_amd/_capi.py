@@ -18,6 +18,7 @@ _vp = C.c_void_p
 _u32 = C.c_uint32
 _u32p = C.POINTER(C.c_uint32)
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _i = C.c_int
 _d = C.c_double
 _s = C.c_char_p
@@ -101,6 +102,29 @@ SIGNATURES = {
     "gr_solver_profile": (_i, [_vp, _i, _dp]),
     "gr_solver_kernel_bytes": (_i, [_vp, _dp]),
     "gr_stream_copy_probe": (_i, [C.c_size_t, _i, _dp]),
+    "gr_batch_create": (_i, [_vp, _u32, C.POINTER(SolverConfig), C.POINTER(_vp)]),
+    "gr_batch_free": (None, [_vp]),
+    "gr_batch_set_bc": (_i, [_vp, _s, _i]),
+    "gr_batch_set_freestream": (_i, [_vp, _dp, _d, _d]),
+    "gr_batch_set_state": (_i, [_vp, _u32, _dp, _dp, _dp]),
+    "gr_batch_get_state": (_i, [_vp, _u32, _dp, _dp, _dp]),
+    "gr_batch_iterate": (_i, [_vp, _i, _d, _dp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "gr_batch_history": (_i, [_vp, _u32, _dp, _i]),
+    "gr_batch_residual_norms": (_i, [_vp, _dp]),
+    "gr_batch_profile": (_i, [_vp, _i, _dp]),
+    "gr_batch_kernel_bytes": (_i, [_vp, _dp]),
+    "gr_batch_set_graph_iters": (_i, [_vp, _i]),
+    "gr_solver_force": (_i, [_vp, _s, _dp]),
+    "gr_batch_force": (_i, [_vp, _s, _dp]),
+    "gr_mesh_wall_distance": (_i, [_vp, C.POINTER(C.c_char_p), _u32, _dp]),
+    "gr_mlp_create": (_i, [_u32, _u32, C.c_uint64, C.POINTER(_vp)]),
+    "gr_mlp_free": (None, [_vp]),
+    "gr_mlp_n_params": (_i, [_vp, C.POINTER(C.c_size_t)]),
+    "gr_mlp_get_params": (_i, [_vp, _fp]),
+    "gr_mlp_set_params": (_i, [_vp, _fp]),
+    "gr_mlp_forward": (_i, [_vp, _fp, _i, _fp]),
+    "gr_mlp_loss_grad": (_i, [_vp, _fp, _fp, _i, _dp, _fp]),
+    "gr_mlp_train_step": (_i, [_vp, _fp, _fp, _i, C.c_float, _dp]),
 }
 
 _lib = None

@@ -1,0 +1,745 @@
+// Batched Euler solver: up to 64 flow instances (different far-field Mach vectors) on one
+// shared mesh, advanced together.
+//
+// Layout is instance-inner: component c of instance k in cell i lives at c*stride + i*64 + k.
+// One wavefront handles one cell, its 64 lanes the instances, so every state access is one
+// fully coalesced 512-byte request and every geometry/connectivity access is uniform across
+// the wave (scalar loads, amortised over the instances).  Each lane evaluates the same
+// expressions in the same order as the single-instance kernels, so instance k of a batch
+// equals a single solve with instance k's far field bit for bit in binary64.
+// Convergence is per instance: an instance that reaches the tolerance is frozen while the
+// others continue, and the launch sequence stops when all have stopped.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fvm_tiles.hpp"
+
+namespace gr {
+
+constexpr int kBatchLanes = 64;
+constexpr int kCellsPerBlock = kBlock / kBatchLanes;   // 4
+constexpr int kFinalParts = 16;                       // partial-sum groups of the batch reduction
+constexpr uint32_t kReduceBlocks = 256;                // first-stage blocks of the batch reduction
+
+struct BatchCtl {
+    double tol;
+    int max_iter;
+    int n_inst;
+    int stop;        // all instances stopped
+    int hist_cap;    // history entries per instance
+    double ref[kBatchLanes];
+    double last[kBatchLanes];
+    double ratio[kBatchLanes];
+    int iter[kBatchLanes];
+    int done[kBatchLanes];
+    int converged[kBatchLanes];
+    int diverged[kBatchLanes];
+};
+
+template <typename S>
+struct BatchArgs {
+    const S *u;   // 5 component blocks of stride elements
+    S *un;
+    S *g;         // 15 blocks
+    S *l;         // 5 blocks
+    size_t stride;
+};
+
+// Cell of this wave (wave-uniform, in a scalar register) and this lane's instance.
+__device__ inline uint32_t batch_cell() {
+    return __builtin_amdgcn_readfirstlane(blockIdx.x * kCellsPerBlock + (threadIdx.x >> 6));
+}
+
+// Far-field region uses the instance's Mach vector: copy the region with it substituted.
+__device__ inline BcRegion inst_region(const BcRegion &R, const double *inst_mach, uint32_t k) {
+    BcRegion r = R;
+    if (R.kind == GR_BC_FARFIELD) {
+        r.mach[0] = inst_mach[3 * k];
+        r.mach[1] = inst_mach[3 * k + 1];
+        r.mach[2] = inst_mach[3 * k + 2];
+    }
+    return r;
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<S> a, BcTable bc,
+                                                          const double *__restrict__ inst_mach, int kind,
+                                                          const BatchCtl *ctl) {
+    if (ctl->stop) return;
+    const uint32_t i = batch_cell();
+    if (i >= m.N) return;
+    const uint32_t k = threadIdx.x & 63;
+    const size_t st = a.stride, me = (size_t)i * kBatchLanes + k;
+    const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+    S ui[5], umin[5], umax[5], g[15];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ui[c] = umin[c] = umax[c] = a.u[c * st + me];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) g[q] = S(0);
+    const uint32_t s = m.cfs[i], e = m.cfs[i + 1];
+    for (uint32_t q = s; q < e; ++q) {
+        const uint32_t j = m.cc[q];
+        S uj[5];
+        const int r = j >= m.N ? bc_region(bc, j - m.N) : -1;
+        if (r < 0) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) uj[c] = a.u[c * st + (size_t)j * kBatchLanes + k];
+        } else {
+            S n[3];
+            outward_normal<S>(m, m.cf[q], xi, yi, zi, n);
+            ghost_state<S>(inst_region(bc.r[r], inst_mach, k), n, ui, uj);
+        }
+        const S d[3] = {(S)(m.cx[j] - xi), (S)(m.cy[j] - yi), (S)(m.cz[j] - zi)};
+        const S w2 = fdiv(S(1), (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const S du = uj[c] - ui[c];
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) g[3 * c + kk] += (d[kk] * du) * w2;
+            umin[c] = fmin(umin[c], uj[c]);
+            umax[c] = fmax(umax[c], uj[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        S out[3];
+        lsq_apply<S>(m, i, &g[3 * c], out);
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            g[3 * c + kk] = out[kk];
+            a.g[(3 * c + kk) * st + me] = out[kk];
+        }
+    }
+    S lim[5] = {S(2), S(2), S(2), S(2), S(2)};
+    for (uint32_t q = s; q < e; ++q) {
+        const uint32_t f = m.cf[q];
+        const S dx = (S)(m.fcx[f] - xi), dy = (S)(m.fcy[f] - yi), dz = (S)(m.fcz[f] - zi);
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+            lim[c] = fmin(lim[c], limiter_face<S>(ui[c], g[3 * c], g[3 * c + 1], g[3 * c + 2], umin[c], umax[c], dx,
+                                                  dy, dz, kind));
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) a.l[c * st + me] = lim[c];
+}
+
+template <typename S, bool CORR>
+__global__ __launch_bounds__(kBlock) void k_batch_flux(MeshView m, BatchArgs<S> a, BcTable bc,
+                                                       const double *__restrict__ inst_mach, double cfl,
+                                                       double *__restrict__ partials, const BatchCtl *ctl) {
+    if (ctl->stop) return;
+    __shared__ double s_err[kCellsPerBlock][kBatchLanes];
+    const uint32_t i = batch_cell();
+    const uint32_t k = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double e_loc = 0;
+    if (i < m.N) {
+        const size_t st = a.stride, me = (size_t)i * kBatchLanes + k;
+        S ui[5], gi[15], li[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            ui[c] = a.u[c * st + me];
+            li[c] = a.l[c * st + me];
+        }
+        if (ctl->done[k]) {   // frozen instance: carry its state over
+#pragma unroll
+            for (int c = 0; c < 5; ++c) a.un[c * st + me] = ui[c];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 15; ++q) gi[q] = a.g[q * st + me];
+            const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+            const double V = m.cv[i];
+            S acc[5] = {0, 0, 0, 0, 0}, dts = 0;
+            const uint32_t s = m.cfs[i], e = m.cfs[i + 1];
+            for (uint32_t q = s; q < e; ++q) {
+                const uint32_t f = m.cf[q], j = m.cc[q];
+                const double fcx = m.fcx[f], fcy = m.fcy[f], fcz = m.fcz[f];
+                S n[3];
+                outward_normal<S>(m, f, xi, yi, zi, n);
+                const S ds = geo_ratio<S>(m.fa[f], V);
+                const S d0[3] = {(S)(fcx - xi), (S)(fcy - yi), (S)(fcz - zi)};
+                const S d1[3] = {(S)(fcx - m.cx[j]), (S)(fcy - m.cy[j]), (S)(fcz - m.cz[j])};
+                S r0, ru0[3], e0, r1, ru1[3], e1;
+                reconstruct<S, CORR>(ui, gi, li, d0, r0, ru0, e0);
+                if (j < m.N) {
+                    const size_t nj = (size_t)j * kBatchLanes + k;
+                    S uj[5], gj[15], lj[5];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        uj[c] = a.u[c * st + nj];
+                        lj[c] = a.l[c * st + nj];
+                    }
+#pragma unroll
+                    for (int q2 = 0; q2 < 15; ++q2) gj[q2] = a.g[q2 * st + nj];
+                    reconstruct<S, CORR>(uj, gj, lj, d1, r1, ru1, e1);
+                } else {
+                    S ug[5];
+                    const int r = bc_region(bc, j - m.N);
+                    if (r < 0) {
+#pragma unroll
+                        for (int c = 0; c < 5; ++c) ug[c] = a.u[c * st + (size_t)j * kBatchLanes + k];
+                    } else {
+                        ghost_state<S>(inst_region(bc.r[r], inst_mach, k), n, ui, ug);
+                    }
+                    reconstruct<S, CORR>(ug, gi, li, d1, r1, ru1, e1);
+                }
+                const S p0 = S(0.4) * (e0 - fdiv(S(0.5), r0) * ((ru0[0] * ru0[0] + ru0[1] * ru0[1]) + ru0[2] * ru0[2]));
+                const S p1 = S(0.4) * (e1 - fdiv(S(0.5), r1) * ((ru1[0] * ru1[0] + ru1[1] * ru1[1]) + ru1[2] * ru1[2]));
+                const S V0 = fdiv((ru0[0] * n[0] + ru0[1] * n[1]) + ru0[2] * n[2], r0);
+                const S V1 = fdiv((ru1[0] * n[0] + ru1[1] * n[1]) + ru1[2] * n[2], r1);
+                acc[0] += (((V0 * r0) + (V1 * r1)) * ds) * S(0.5);
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    const S fu0 = ru0[kk] * V0 + n[kk] * p0;
+                    const S fu1 = ru1[kk] * V1 + n[kk] * p1;
+                    acc[1 + kk] += ((fu0 + fu1) * ds) * S(0.5);
+                }
+                acc[4] += (((V0 * (e0 + p0)) + (V1 * (e1 + p1))) * ds) * S(0.5);
+                const S v0[3] = {fdiv(ru0[0], r0), fdiv(ru0[1], r0), fdiv(ru0[2], r0)};
+                const S v1[3] = {fdiv(ru1[0], r1), fdiv(ru1[1], r1), fdiv(ru1[2], r1)};
+                const S cs0 = dsqrt(fmax(S(1e-12), fmin(S(1e30), fdiv(S(1.4) * p0, r0))));
+                const S cs1 = dsqrt(fmax(S(1e-12), fmin(S(1e30), fdiv(S(1.4) * p1, r1))));
+                const S eig = fmax(dsqrt((v0[0] * v0[0] + v0[1] * v0[1]) + v0[2] * v0[2]) + cs0,
+                                   dsqrt((v1[0] * v1[0] + v1[1] * v1[1]) + v1[2] * v1[2]) + cs1);
+                acc[0] -= ((eig * (r1 - r0)) * S(0.5)) * ds;
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) acc[1 + kk] -= (((ru1[kk] - ru0[kk]) * eig) * S(0.5)) * ds;
+                acc[4] -= ((eig * (e1 - e0)) * S(0.5)) * ds;
+                dts += eig * ds;
+            }
+            const S dt = fdiv((S)cfl, dts);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) a.un[c * st + me] = ui[c] + (-acc[c]) * dt;
+            const double r = acc[0], u0 = acc[1], u1 = acc[2], u2 = acc[3], en = acc[4];
+            e_loc = ((r * r + ((u0 * u0 + u1 * u1) + u2 * u2)) + en * en) * V;
+        }
+    }
+    s_err[w][k] = e_loc;
+    __syncthreads();
+    if (threadIdx.x < kBatchLanes) {
+        double t = 0;
+#pragma unroll
+        for (int ww = 0; ww < kCellsPerBlock; ++ww) t += s_err[ww][threadIdx.x];
+        partials[(size_t)blockIdx.x * kBatchLanes + threadIdx.x] = t;
+    }
+}
+
+// First stage of the per-instance residual sums: block r adds the flux blocks' partial rows
+// r*per .. r*per+per-1 (wave w takes every fourth row, then the 4 wave sums in order), so the
+// 64-lane rows are read by many CUs at once.  Fixed order: deterministic.
+__global__ __launch_bounds__(kBlock) void k_batch_reduce(const double *__restrict__ partials, uint32_t nblocks,
+                                                         const BatchCtl *ctl, double *__restrict__ partials2) {
+    if (ctl && ctl->stop) return;
+    __shared__ double s[kBlock / kBatchLanes][kBatchLanes];
+    const uint32_t k = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t per = (nblocks + kReduceBlocks - 1) / kReduceBlocks;
+    const uint32_t b0 = blockIdx.x * per, b1 = min(nblocks, b0 + per);
+    double acc = 0;
+    for (uint32_t b = b0 + w; b < b1; b += kBlock / kBatchLanes) acc += partials[(size_t)b * kBatchLanes + k];
+    s[w][k] = acc;
+    __syncthreads();
+    if (w == 0) partials2[(size_t)blockIdx.x * kBatchLanes + k] = ((s[0][k] + s[1][k]) + s[2][k]) + s[3][k];
+}
+
+// Per-instance residual sums (fixed order) and the per-instance stopping rule.
+__global__ __launch_bounds__(kFinalParts *kBatchLanes) void k_batch_finalize(const double *__restrict__ partials,
+                                                                           uint32_t nblocks, BatchCtl *ctl,
+                                                                           double *hist, double *out) {
+    if (ctl && ctl->stop) return;
+    __shared__ double s_part[kFinalParts][kBatchLanes];
+    __shared__ int s_alive;
+    const uint32_t k = threadIdx.x & 63, p = threadIdx.x >> 6;
+    double acc = 0;
+    for (uint32_t b = p; b < nblocks; b += kFinalParts) acc += partials[(size_t)b * kBatchLanes + k];
+    s_part[p][k] = acc;
+    if (threadIdx.x == 0) s_alive = 0;
+    __syncthreads();
+    if (threadIdx.x < kBatchLanes) {
+        double tot = 0;
+        for (int q = 0; q < kFinalParts; ++q) tot += s_part[q][k];
+        if (!ctl) {
+            out[k] = tot;
+        } else if ((int)k < ctl->n_inst && !ctl->done[k]) {
+            const int it = ctl->iter[k];
+            if (it < ctl->hist_cap) hist[(size_t)k * ctl->hist_cap + it] = tot;
+            if (it == 0 && ctl->ref[k] <= 0) ctl->ref[k] = tot;
+            ctl->last[k] = tot;
+            ctl->ratio[k] = tot / ctl->ref[k];
+            ctl->iter[k] = it + 1;
+            if (!isfinite(tot)) {
+                ctl->diverged[k] = 1;
+                ctl->done[k] = 1;
+            } else if (ctl->tol > 0 && ctl->ratio[k] < ctl->tol) {
+                ctl->converged[k] = 1;
+                ctl->done[k] = 1;
+            }
+            if (ctl->iter[k] >= ctl->max_iter) ctl->done[k] = 1;
+            if (!ctl->done[k]) s_alive = 1;
+        }
+    }
+    __syncthreads();
+    if (ctl && threadIdx.x == 0 && !s_alive) ctl->stop = 1;
+}
+
+// Boundary-condition ghost states of all instances into the state array (for reading back).
+template <typename S>
+__global__ void k_batch_ghosts(MeshView m, S *u, size_t stride, BcTable bc, const double *__restrict__ inst_mach) {
+    const uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * kCellsPerBlock + (threadIdx.x >> 6));
+    if (g >= m.G) return;
+    const uint32_t k = threadIdx.x & 63;
+    const int r = bc_region(bc, g);
+    if (r < 0) return;
+    const uint32_t c0 = m.N + g, q = m.cfs[c0];
+    const uint32_t i = m.cc[q], f = m.cf[q];
+    S ui[5], ug[5], n[3];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ui[c] = u[c * stride + (size_t)i * kBatchLanes + k];
+    outward_normal<S>(m, f, m.cx[i], m.cy[i], m.cz[i], n);
+    ghost_state<S>(inst_region(bc.r[r], inst_mach, k), n, ui, ug);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) u[c * stride + (size_t)c0 * kBatchLanes + k] = ug[c];
+}
+
+}  // namespace gr
+
+using gr::guarded;
+using gr::require;
+
+struct gr_batch {
+    gr_mesh *mesh = nullptr;
+    gr_solver_config cfg{};
+    uint32_t n_inst = 0;
+    size_t stride = 0;   // elements per component block: pad64(N+G) * 64
+    gr::DevArray<char> ubuf[2], grad, lim;
+    gr::DevArray<double> inst_mach;   // 3 per lane
+    std::vector<double> mach_h;       // 3 per instance
+    double p_inf = 1.0, T_inf = 1.0 / 1.4;
+    std::vector<int> region_kind;     // -1: no BC
+    int cur = 0, ghost_src = 0;
+    gr::DevArray<gr::BatchCtl> ctl;
+    gr::DevArray<double> partials, partials2, hist, out;
+    int hist_cap = 0;
+    gr::BatchCtl *ctl_host = nullptr;   // pinned: [0] upload, [1],[2] polling
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int graph_iters = 10;
+    int last_iters = 0;
+
+    ~gr_batch() {
+        if (mesh && mesh->stream) (void)hipStreamSynchronize(mesh->stream);
+        drop_graphs();
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (ctl_host) (void)hipHostFree(ctl_host);
+    }
+    void drop_graphs() {
+        for (auto &g : exec) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+    uint32_t nblocks() const { return (mesh->h.n_cells + gr::kCellsPerBlock - 1) / gr::kCellsPerBlock; }
+    template <typename S> gr::BatchArgs<S> args(int b) const {
+        return gr::BatchArgs<S>{(const S *)ubuf[b].p, (S *)ubuf[1 - b].p, (S *)grad.p, (S *)lim.p, stride};
+    }
+    gr::BcTable table() const {
+        gr::BcTable t{};
+        for (uint32_t r = 0; r < mesh->h.n_regions; ++r) {
+            if (region_kind[r] < 0) continue;
+            auto &R = t.r[t.n++];
+            R.begin = mesh->h.ghost_cell_starts[r];
+            R.end = mesh->h.ghost_cell_starts[r + 1];
+            R.kind = region_kind[r];
+            R.p = p_inf;
+            R.T = T_inf;
+        }
+        return t;
+    }
+    template <typename S> void enqueue_iteration(int b, gr::BatchCtl *c, hipEvent_t *ev) {
+        gr_mesh *m = mesh;
+        const gr::BcTable t = table();
+        const auto a = args<S>(b);
+        const dim3 grid(nblocks()), blk(gr::kBlock);
+        if (ev) GR_HIP(hipEventRecord(ev[0], m->stream));
+        hipLaunchKernelGGL(gr::k_batch_gradlim<S>, grid, blk, 0, m->stream, m->view(), a, t, inst_mach.p,
+                           cfg.limiter, (const gr::BatchCtl *)c);
+        if (ev) GR_HIP(hipEventRecord(ev[1], m->stream));
+        if (cfg.corrected)
+            hipLaunchKernelGGL((gr::k_batch_flux<S, true>), grid, blk, 0, m->stream, m->view(), a, t, inst_mach.p,
+                               cfg.cfl, partials.p, (const gr::BatchCtl *)c);
+        else
+            hipLaunchKernelGGL((gr::k_batch_flux<S, false>), grid, blk, 0, m->stream, m->view(), a, t, inst_mach.p,
+                               cfg.cfl, partials.p, (const gr::BatchCtl *)c);
+        if (ev) GR_HIP(hipEventRecord(ev[2], m->stream));
+        hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream, partials.p,
+                           nblocks(), (const gr::BatchCtl *)c, partials2.p);
+        hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,
+                           partials2.p, gr::kReduceBlocks, c, hist.p, out.p);
+        if (ev) GR_HIP(hipEventRecord(ev[3], m->stream));
+    }
+    void enqueue(int b, gr::BatchCtl *c, hipEvent_t *ev = nullptr) {
+        if (cfg.dtype == GR_F64) enqueue_iteration<double>(b, c, ev);
+        else enqueue_iteration<float>(b, c, ev);
+    }
+    hipGraphExec_t graph(int b) {
+        if (exec[b]) return exec[b];
+        hipGraph_t g = nullptr;
+        GR_HIP(hipStreamBeginCapture(mesh->stream, hipStreamCaptureModeRelaxed));
+        try {
+            for (int k = 0; k < graph_iters; ++k) enqueue((b + k) % 2, ctl.p);
+        } catch (...) {
+            (void)hipStreamEndCapture(mesh->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        GR_HIP(hipStreamEndCapture(mesh->stream, &g));
+        hipError_t e = hipGraphInstantiate(&exec[b], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        GR_HIP(e);
+        return exec[b];
+    }
+    // Algorithmic bytes per launch.  Geometry and connectivity are read once per cell for all
+    // instances (wave-uniform loads); state traffic is per instance.
+    void kernel_bytes(double out3[3]) const {
+        const double N = mesh->h.n_cells, G = mesh->h.n_ghosts, F = mesh->h.n_faces;
+        const double slots = mesh->h.cell_faces_starts[mesh->h.n_cells];
+        const double su = (double)cfg.dtype, I = n_inst;
+        out3[0] = I * su * (5 * N + 20 * N) + 8.0 * (3 * (N + G) + 6 * N + 3 * F + 3 * G) + 4.0 * (N + 1 + 2 * slots);
+        out3[1] = I * su * (25 * N + 5 * N) + 8.0 * (3 * (N + G) + N + 7 * F) + 4.0 * (N + 1 + 2 * slots);
+        out3[2] = 8.0 * 64 * (nblocks() + 2.0 * gr::kReduceBlocks);
+    }
+    template <typename S> void fill_freestream() {
+        // every lane (padding lanes copy instance 0) of every cell and ghost, both buffers
+        const size_t NG = (size_t)mesh->h.n_cells + mesh->h.n_ghosts;
+        std::vector<S> soa(5 * stride, S(0));
+        for (uint32_t k = 0; k < (uint32_t)gr::kBatchLanes; ++k) {
+            const double *mk = &mach_h[3 * (k < n_inst ? k : 0)];
+            const double rho_v = p_inf / T_inf;
+            const double c = std::sqrt((1.4 * p_inf) / rho_v);
+            const double vel[3] = {mk[0] * c, mk[1] * c, mk[2] * c};
+            const double e = p_inf / 0.4 + (0.5 * rho_v) * ((vel[0] * vel[0] + vel[1] * vel[1]) + vel[2] * vel[2]);
+            const double val[5] = {rho_v, vel[0] * rho_v, vel[1] * rho_v, vel[2] * rho_v, e};
+            for (size_t i = 0; i < NG; ++i)
+                for (int q = 0; q < 5; ++q) soa[q * stride + i * gr::kBatchLanes + k] = (S)val[q];
+        }
+        for (int b = 0; b < 2; ++b)
+            GR_HIP(hipMemcpyAsync(ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice, mesh->stream));
+        GR_HIP(hipStreamSynchronize(mesh->stream));
+    }
+    template <typename S> void set_instance(uint32_t k, const double *rho, const double *rhou, const double *rhoe) {
+        const size_t NG = (size_t)mesh->h.n_cells + mesh->h.n_ghosts;
+        std::vector<S> soa(5 * stride);
+        // read-modify-write of the whole array keeps the other instances
+        GR_HIP(hipMemcpyAsync(soa.data(), ubuf[cur].p, soa.size() * sizeof(S), hipMemcpyDeviceToHost, mesh->stream));
+        GR_HIP(hipStreamSynchronize(mesh->stream));
+        for (size_t i = 0; i < NG; ++i) {
+            const size_t o = i * gr::kBatchLanes + k;
+            soa[o] = (S)rho[i];
+            for (int d = 0; d < 3; ++d) soa[(1 + d) * stride + o] = (S)rhou[3 * i + d];
+            soa[4 * stride + o] = (S)rhoe[i];
+        }
+        for (int b = 0; b < 2; ++b)
+            GR_HIP(hipMemcpyAsync(ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice, mesh->stream));
+        GR_HIP(hipStreamSynchronize(mesh->stream));
+        ghost_src = cur;
+    }
+    template <typename S> void get_instance(uint32_t k, double *rho, double *rhou, double *rhoe) {
+        gr_mesh *m = mesh;
+        const size_t N = m->h.n_cells, NG = N + m->h.n_ghosts;
+        if (ghost_src != cur && m->h.n_ghosts) {
+            hipLaunchKernelGGL(gr::k_batch_ghosts<S>, dim3((m->h.n_ghosts + gr::kCellsPerBlock - 1) / gr::kCellsPerBlock),
+                               dim3(gr::kBlock), 0, m->stream, m->view(), (S *)ubuf[ghost_src].p, stride, table(),
+                               (const double *)inst_mach.p);
+            GR_HIP(hipGetLastError());
+        }
+        std::vector<S> a(5 * stride), b(5 * stride);
+        GR_HIP(hipMemcpyAsync(a.data(), ubuf[cur].p, a.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
+        GR_HIP(hipMemcpyAsync(b.data(), ubuf[ghost_src].p, b.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
+        GR_HIP(hipStreamSynchronize(m->stream));
+        for (size_t i = 0; i < NG; ++i) {
+            const std::vector<S> &src = i < N ? a : b;
+            const size_t o = i * gr::kBatchLanes + k;
+            if (rho) rho[i] = (double)src[o];
+            if (rhou)
+                for (int d = 0; d < 3; ++d) rhou[3 * i + d] = (double)src[(1 + d) * stride + o];
+            if (rhoe) rhoe[i] = (double)src[4 * stride + o];
+        }
+    }
+};
+
+extern "C" {
+
+int gr_batch_create(gr_mesh *m, uint32_t n_inst, const gr_solver_config *cfg, gr_batch **out) {
+    return guarded([&] {
+        require(m && cfg && out, "null argument");
+        require(m->metrics_ready, "mesh metrics not computed (gr_mesh_compute_metrics)");
+        require(n_inst >= 1 && n_inst <= (uint32_t)gr::kBatchLanes, "a batch holds 1..64 instances");
+        require(cfg->dtype == GR_F64 || cfg->dtype == GR_F32, "batch dtype must be GR_F64 or GR_F32");
+        require(cfg->cfl > 0 && cfg->limiter >= 0 && cfg->limiter <= 2, "bad cfl or limiter");
+        require(m->h.n_regions <= (uint32_t)gr::kMaxRegions, "too many boundary regions");
+        require(!m->h.one_dimensional, "the Euler solver needs a 3-D mesh");
+        auto *b = new gr_batch();
+        try {
+            b->mesh = m;
+            b->cfg = *cfg;
+            b->n_inst = n_inst;
+            b->stride = gr::pad64((size_t)m->h.n_cells + m->h.n_ghosts) * gr::kBatchLanes;
+            const size_t blk = b->stride * (size_t)cfg->dtype;
+            for (auto &u : b->ubuf) {
+                u.alloc(5 * blk);
+                GR_HIP(hipMemsetAsync(u.p, 0, u.n, m->stream));
+            }
+            b->grad.alloc(15 * blk);
+            b->lim.alloc(5 * blk);
+            GR_HIP(hipMemsetAsync(b->grad.p, 0, b->grad.n, m->stream));
+            GR_HIP(hipMemsetAsync(b->lim.p, 0, b->lim.n, m->stream));
+            b->inst_mach.alloc(3 * gr::kBatchLanes);
+            b->mach_h.assign(3 * n_inst, 0.0);
+            b->region_kind.assign(m->h.n_regions, -1);
+            b->ctl.alloc(1);
+            b->partials.alloc((size_t)b->nblocks() * gr::kBatchLanes);
+            b->partials2.alloc((size_t)gr::kReduceBlocks * gr::kBatchLanes);
+            b->out.alloc(gr::kBatchLanes);
+            b->hist_cap = 1 << 14;
+            b->hist.alloc((size_t)b->hist_cap * gr::kBatchLanes);
+            GR_HIP(hipHostMalloc((void **)&b->ctl_host, 3 * sizeof(gr::BatchCtl)));
+            for (auto &e : b->ev) GR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            GR_HIP(hipStreamSynchronize(m->stream));
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+void gr_batch_free(gr_batch *b) { delete b; }
+
+int gr_batch_set_bc(gr_batch *b, const char *region, int kind) {
+    return guarded([&] {
+        require(b && region, "null argument");
+        require(kind == GR_BC_SLIP || kind == GR_BC_FARFIELD, "unknown boundary kind");
+        uint32_t r = 0;
+        for (; r < b->mesh->h.n_regions; ++r)
+            if (b->mesh->h.region_names[r] == region) break;
+        if (r == b->mesh->h.n_regions) throw gr::Error(GR_ERR_NOT_FOUND, std::string("region '") + region + "' not found in mesh");
+        b->region_kind[r] = kind;
+        b->drop_graphs();
+    });
+}
+
+int gr_batch_set_freestream(gr_batch *b, const double *mach, double pressure, double temperature) {
+    return guarded([&] {
+        require(b && mach, "null argument");
+        require(pressure > 0 && temperature > 0, "pressure and temperature must be positive");
+        b->mach_h.assign(mach, mach + 3 * b->n_inst);
+        b->p_inf = pressure;
+        b->T_inf = temperature;
+        std::vector<double> lanes(3 * gr::kBatchLanes);
+        for (uint32_t k = 0; k < (uint32_t)gr::kBatchLanes; ++k)
+            for (int d = 0; d < 3; ++d) lanes[3 * k + d] = b->mach_h[3 * (k < b->n_inst ? k : 0) + d];
+        b->inst_mach.upload(lanes.data(), lanes.size(), b->mesh->stream);
+        if (b->cfg.dtype == GR_F64) b->fill_freestream<double>();
+        else b->fill_freestream<float>();
+        b->ghost_src = b->cur;
+        b->drop_graphs();
+    });
+}
+
+int gr_batch_set_state(gr_batch *b, uint32_t inst, const double *rho, const double *rhou, const double *rhoe) {
+    return guarded([&] {
+        require(b && rho && rhou && rhoe, "null argument");
+        require(inst < b->n_inst, "instance index out of range");
+        if (b->cfg.dtype == GR_F64) b->set_instance<double>(inst, rho, rhou, rhoe);
+        else b->set_instance<float>(inst, rho, rhou, rhoe);
+    });
+}
+
+int gr_batch_get_state(gr_batch *b, uint32_t inst, double *rho, double *rhou, double *rhoe) {
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        require(inst < b->n_inst, "instance index out of range");
+        if (b->cfg.dtype == GR_F64) b->get_instance<double>(inst, rho, rhou, rhoe);
+        else b->get_instance<float>(inst, rho, rhou, rhoe);
+    });
+}
+
+int gr_batch_iterate(gr_batch *b, int max_iter, double tol, const double *reference, int *iterations,
+                     int *converged, int *launched) {
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        require(max_iter > 0, "max_iter must be positive");
+        gr_mesh *m = b->mesh;
+        gr::BatchCtl &c0 = b->ctl_host[0];
+        std::memset(&c0, 0, sizeof c0);
+        c0.tol = tol;
+        c0.max_iter = max_iter;
+        c0.n_inst = (int)b->n_inst;
+        c0.hist_cap = b->hist_cap;
+        for (int k = 0; k < gr::kBatchLanes; ++k) {
+            c0.done[k] = k < (int)b->n_inst ? 0 : 1;
+            c0.ref[k] = (reference && k < (int)b->n_inst) ? reference[k] : 0.0;
+        }
+        GR_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
+        const int K = b->graph_iters;
+        const int launches = (max_iter + K - 1) / K;
+        int L = 0;
+        for (; L < launches; ++L) {
+            if (b->cfg.use_graph) {
+                GR_HIP(hipGraphLaunch(b->graph(b->cur), m->stream));
+            } else {
+                for (int k = 0; k < K; ++k) b->enqueue((b->cur + k) % 2, b->ctl.p);
+                GR_HIP(hipGetLastError());
+            }
+            gr::BatchCtl *poll = &b->ctl_host[1 + (L & 1)];
+            GR_HIP(hipMemcpyAsync(poll, b->ctl.p, sizeof *poll, hipMemcpyDeviceToHost, m->stream));
+            GR_HIP(hipEventRecord(b->ev[L & 1], m->stream));
+            if (L >= 1) {
+                GR_HIP(hipEventSynchronize(b->ev[(L - 1) & 1]));
+                if (b->ctl_host[1 + ((L - 1) & 1)].stop) {
+                    ++L;
+                    break;
+                }
+            }
+        }
+        GR_HIP(hipStreamSynchronize(m->stream));
+        gr::BatchCtl fin;
+        GR_HIP(hipMemcpy(&fin, b->ctl.p, sizeof fin, hipMemcpyDeviceToHost));
+        // iterations launched for the batch: frozen instances still occupy their lanes
+        const int total = std::min(L * K, max_iter);
+        int ran = 0;
+        for (uint32_t k = 0; k < b->n_inst; ++k) ran = std::max(ran, fin.iter[k]);
+        if (ran > 0) {
+            b->ghost_src = (b->cur + ran - 1) % 2;
+            b->cur = (b->cur + ran) % 2;
+        }
+        b->last_iters = std::min(ran, b->hist_cap);
+        for (uint32_t k = 0; k < b->n_inst; ++k) {
+            if (iterations) iterations[k] = fin.iter[k];
+            if (converged) converged[k] = fin.converged[k];
+        }
+        if (launched) *launched = std::max(ran, 0);
+        (void)total;
+        for (uint32_t k = 0; k < b->n_inst; ++k)
+            if (fin.diverged[k]) throw gr::Error(GR_ERR_DIVERGED, "instance " + std::to_string(k) + " diverged");
+    });
+}
+
+int gr_batch_history(const gr_batch *b, uint32_t inst, double *hist, int n) {
+    return guarded([&] {
+        require(b && hist, "null argument");
+        require(inst < b->n_inst, "instance index out of range");
+        require(n >= 0 && n <= b->last_iters, "history request longer than the last run");
+        if (n) GR_HIP(hipMemcpy(hist, b->hist.p + (size_t)inst * b->hist_cap, n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int gr_batch_residual_norms(gr_batch *b, double *out) {
+    return guarded([&] {
+        require(b && out, "null argument");
+        gr_mesh *m = b->mesh;
+        // a control block with nothing done and no stop, not recording (max_iter 0 marks it)
+        gr::BatchCtl &c0 = b->ctl_host[0];
+        std::memset(&c0, 0, sizeof c0);
+        for (int k = 0; k < gr::kBatchLanes; ++k) c0.done[k] = k < (int)b->n_inst ? 0 : 1;
+        GR_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
+        const gr::BcTable t = b->table();
+        const dim3 grid(b->nblocks()), blk(gr::kBlock);
+        if (b->cfg.dtype == GR_F64) {
+            hipLaunchKernelGGL(gr::k_batch_gradlim<double>, grid, blk, 0, m->stream, m->view(), b->args<double>(b->cur), t,
+                               (const double *)b->inst_mach.p, b->cfg.limiter, (const gr::BatchCtl *)b->ctl.p);
+            hipLaunchKernelGGL((gr::k_batch_flux<double, false>), grid, blk, 0, m->stream, m->view(),
+                               b->args<double>(b->cur), t, (const double *)b->inst_mach.p, b->cfg.cfl, b->partials.p,
+                               (const gr::BatchCtl *)b->ctl.p);
+        } else {
+            hipLaunchKernelGGL(gr::k_batch_gradlim<float>, grid, blk, 0, m->stream, m->view(), b->args<float>(b->cur), t,
+                               (const double *)b->inst_mach.p, b->cfg.limiter, (const gr::BatchCtl *)b->ctl.p);
+            hipLaunchKernelGGL((gr::k_batch_flux<float, false>), grid, blk, 0, m->stream, m->view(),
+                               b->args<float>(b->cur), t, (const double *)b->inst_mach.p, b->cfg.cfl, b->partials.p,
+                               (const gr::BatchCtl *)b->ctl.p);
+        }
+        hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream,
+                           b->partials.p, b->nblocks(), (const gr::BatchCtl *)nullptr, b->partials2.p);
+        hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,
+                           b->partials2.p, gr::kReduceBlocks, (gr::BatchCtl *)nullptr, (double *)nullptr, b->out.p);
+        GR_HIP(hipGetLastError());
+        std::vector<double> o(gr::kBatchLanes);
+        b->out.download(o.data(), o.size(), m->stream);
+        GR_HIP(hipStreamSynchronize(m->stream));
+        std::copy(o.begin(), o.begin() + b->n_inst, out);
+        b->ghost_src = b->cur;
+    });
+}
+
+int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]) {
+    return guarded([&] {
+        require(b && ms, "null argument");
+        require(n_iter > 0 && n_iter <= 10000, "n_iter must be in 1..10000");
+        gr_mesh *m = b->mesh;
+        std::vector<hipEvent_t> ev(4 * (size_t)n_iter, nullptr);
+        struct Guard {
+            std::vector<hipEvent_t> &e;
+            ~Guard() {
+                for (auto x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } guard{ev};
+        for (auto &e : ev) GR_HIP(hipEventCreate(&e));
+        gr::BatchCtl &c0 = b->ctl_host[0];
+        std::memset(&c0, 0, sizeof c0);
+        c0.max_iter = n_iter;
+        c0.n_inst = (int)b->n_inst;
+        c0.hist_cap = b->hist_cap;
+        for (int k = 0; k < gr::kBatchLanes; ++k) c0.done[k] = k < (int)b->n_inst ? 0 : 1;
+        GR_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
+        for (int k = 0; k < n_iter; ++k) b->enqueue((b->cur + k) % 2, b->ctl.p, &ev[4 * (size_t)k]);
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+        double acc[3] = {0, 0, 0};
+        for (int k = 0; k < n_iter; ++k)
+            for (int j = 0; j < 3; ++j) {
+                float t = 0;
+                GR_HIP(hipEventElapsedTime(&t, ev[4 * (size_t)k + j], ev[4 * (size_t)k + j + 1]));
+                acc[j] += t;
+            }
+        for (int j = 0; j < 3; ++j) ms[j] = acc[j] / n_iter;
+        b->ghost_src = (b->cur + n_iter - 1) % 2;
+        b->cur = (b->cur + n_iter) % 2;
+        b->last_iters = 0;
+    });
+}
+
+int gr_batch_kernel_bytes(const gr_batch *b, double bytes[3]) {
+    return guarded([&] {
+        require(b && bytes, "null argument");
+        b->kernel_bytes(bytes);
+    });
+}
+
+int gr_batch_force(gr_batch *b, const char *region, double *F) {
+    return guarded([&] {
+        require(b && region && F, "null argument");
+        uint32_t r = 0;
+        for (; r < b->mesh->h.n_regions; ++r)
+            if (b->mesh->h.region_names[r] == region) break;
+        if (r == b->mesh->h.n_regions) throw gr::Error(GR_ERR_NOT_FOUND, std::string("region '") + region + "' not found in mesh");
+        for (uint32_t k = 0; k < b->n_inst; ++k) {
+            int rc = gr_internal_wall_force(b->mesh, b->ubuf[b->cur].p, b->cfg.dtype, b->stride, gr::kBatchLanes, k, r,
+                                            F + 3 * k);
+            if (rc != GR_OK) throw gr::Error(rc, gr_last_error());
+        }
+    });
+}
+
+int gr_batch_set_graph_iters(gr_batch *b, int iters) {
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        require(iters >= 2 && iters <= 200 && iters % 2 == 0, "iterations per graph must be even, 2..200");
+        b->graph_iters = iters;
+        b->drop_graphs();
+    });
+}
+
+}  // extern "C"

@@ -96,6 +96,11 @@ struct gr_mesh {
     size_t mstride() const { return gr::pad64(h.n_cells); }
 };
 
+// Pressure force on wall region `region` from a state array whose element (c, cell, lane) is
+// at c*stride + cell*lanes + lane (post.hip).  Returns a gr_status.
+int gr_internal_wall_force(gr_mesh *m, const void *u, int dtype, size_t stride, uint32_t lanes, uint32_t lane,
+                           uint32_t region, double F[3]);
+
 struct gr_field {
     gr_mesh *mesh = nullptr;
     uint32_t ncomp = 1;

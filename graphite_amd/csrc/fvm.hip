@@ -928,6 +928,15 @@ int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]) {
     });
 }
 
+int gr_solver_force(gr_solver *s, const char *region, double F[3]) {
+    return guarded([&] {
+        require(s && region && F, "null argument");
+        const uint32_t r = region_of(s->mesh, region);
+        int rc = gr_internal_wall_force(s->mesh, s->ubuf[s->cur].p, s->cfg.dtype, s->stride, 1, 0, r, F);
+        if (rc != GR_OK) throw gr::Error(rc, gr_last_error());
+    });
+}
+
 int gr_solver_kernel_bytes(const gr_solver *s, double bytes[4]) {
     return guarded([&] {
         require(s && bytes, "null argument");

@@ -1,0 +1,500 @@
+// MLP of the learned warm start: 4 dense layers (n_in -> W -> W -> W -> n_out, tanh between),
+// trained and evaluated with hand-written BF16 MFMA kernels (v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation) on gfx950.  Master weights, Adam state, biases, losses and gradient
+// reductions are FP32; activations and GEMM operands BF16.
+//
+// Layout: activations row-major [rows][cols] BF16; each layer keeps its weights twice in
+// BF16, W^T [out][in] for the forward product and W [in][out] for the input-gradient product,
+// so every MFMA operand fragment is one contiguous 16-byte load.  The weight gradient
+// X^T * dZ reduces over rows; its tiles are transposed through LDS.  Every reduction runs in
+// a fixed order, so training is deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+#include "device.hpp"
+#include "fvm_kernels.hpp"
+
+namespace gr {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMlpIn = 64;     // padded input width (n_in <= 64)
+constexpr int kMlpOut = 32;    // padded output width (n_out <= 32)
+constexpr int kMlpW = 256;     // hidden width
+constexpr int kKC = 64;        // K chunk of the dense kernel staged in LDS
+constexpr int kRowsPerBlock = 128;
+constexpr int kWgradRows = 4096;   // rows reduced by one block of the weight-gradient kernel
+
+__device__ inline bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) z[j] = (bf16)0.0f;
+    return z;
+}
+
+// Y[M x N] = act(X[M x K] * B + bias), B given as BT = B^T [N x K] row-major.  ACT 1: tanh.
+// K % 32 == 0, N == 16*NT.  Block: 4 waves x 32 rows; each wave 2 row groups x NT column
+// tiles of 16x16 MFMA accumulators.  In the 16x16x32 MFMA, lane l holds row/column (l mod 16)
+// of its operand and eight consecutive k of group l/16; the accumulator element j of lane l
+// is row 4*(l/16)+j, column l mod 16.
+template <int ACT, typename OUT, int NT>
+__global__ __launch_bounds__(kBlock) void k_dense(const bf16 *__restrict__ X, int ldx, int M, int K,
+                                                  const bf16 *__restrict__ BT, const float *__restrict__ bias,
+                                                  OUT *__restrict__ Y, int ldy) {
+    constexpr int N = 16 * NT;
+    __shared__ bf16 sB[N][kKC + 8];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row0 = blockIdx.x * kRowsPerBlock + w * 32;
+    const int lr = lane & 15, lg = lane >> 4;
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < K; kc += kKC) {
+        const int kn = min(kKC, K - kc);
+        __syncthreads();
+        for (int v = threadIdx.x; v < N * (kn / 8); v += blockDim.x) {
+            const int n = v / (kn / 8), c8 = v - n * (kn / 8);
+            *(bf16x8 *)&sB[n][c8 * 8] = *(const bf16x8 *)&BT[(size_t)n * K + kc + c8 * 8];
+        }
+        __syncthreads();
+        for (int ks = 0; ks < kn; ks += 32) {
+            bf16x8 a[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int row = row0 + r * 16 + lr;
+                a[r] = row < M ? *(const bf16x8 *)&X[(size_t)row * ldx + kc + ks + 8 * lg] : zero8();
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const bf16x8 b = *(const bf16x8 *)&sB[t * 16 + lr][ks + 8 * lg];
+                acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b, acc[0][t], 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b, acc[1][t], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int col = t * 16 + lr;
+            const float bb = bias ? bias[col] : 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = row0 + r * 16 + 4 * lg + j;
+                if (row < M) {
+                    float v = acc[r][t][j] + bb;
+                    if (ACT == 1) v = tanhf(v);
+                    Y[(size_t)row * ldy + col] = (OUT)v;
+                }
+            }
+        }
+}
+
+// Partial weight gradient P[chunk][K][N] = X[rows of chunk]^T * D[rows of chunk] with
+// X [M x K], D [M x N] (N == 16*NT) BF16; block (kt, chunk) computes rows kt*64..kt*64+63 of
+// the K dimension.  Tiles of 64 rows are written transposed to LDS so that the MFMA
+// fragments (eight consecutive rows) are contiguous.
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_wgrad(const bf16 *__restrict__ X, int ldx, const bf16 *__restrict__ D,
+                                                  int ldd, int M, int K, float *__restrict__ P) {
+    constexpr int N = 16 * NT;
+    __shared__ bf16 sX[64][64 + 8];   // [k_in local][row local]
+    __shared__ bf16 sD[N][64 + 8];    // [n][row local]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int kt = blockIdx.x, chunk = blockIdx.y;
+    const int r_begin = chunk * kWgradRows, r_end = min(M, r_begin + kWgradRows);
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int r0 = r_begin; r0 < r_end; r0 += 64) {
+        __syncthreads();
+        // X rows r0..r0+63, columns kt*64..+63 -> sX transposed (zeros past M)
+        for (int v = threadIdx.x; v < 64 * 8; v += blockDim.x) {
+            const int rr = v >> 3, c8 = v & 7;
+            const int row = r0 + rr;
+            bf16x8 x = row < r_end ? *(const bf16x8 *)&X[(size_t)row * ldx + kt * 64 + c8 * 8] : zero8();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sX[c8 * 8 + j][rr] = x[j];
+        }
+        for (int v = threadIdx.x; v < 64 * (N / 8); v += blockDim.x) {
+            const int rr = v / (N / 8), c8 = v - rr * (N / 8);
+            const int row = r0 + rr;
+            bf16x8 d = row < r_end ? *(const bf16x8 *)&D[(size_t)row * ldd + c8 * 8] : zero8();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sD[c8 * 8 + j][rr] = d[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 64; ks += 32) {
+            const bf16x8 a = *(const bf16x8 *)&sX[w * 16 + lr][ks + 8 * lg];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const bf16x8 b = *(const bf16x8 *)&sD[t * 16 + lr][ks + 8 * lg];
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    float *Pc = P + (size_t)chunk * K * N;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = kt * 64 + w * 16 + 4 * lg + j;
+            Pc[(size_t)k * N + t * 16 + lr] = acc[t][j];
+        }
+}
+
+// out[e] = sum over chunks c (in order) of P[c][e]
+__global__ void k_sum_chunks(const float *__restrict__ P, int nchunks, int n, float *__restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; ++c) s += P[(size_t)c * n + e];
+    out[e] = s;
+}
+
+// Column sums of an FP32 [M x N] array over row chunks: P[chunk][N].
+__global__ void k_colsum(const float *__restrict__ Z, int ldz, int M, int N, float *__restrict__ P) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x, chunk = blockIdx.y;
+    if (n >= N) return;
+    const int r0 = chunk * kWgradRows, r1 = min(M, r0 + kWgradRows);
+    float s = 0.f;
+    for (int r = r0; r < r1; ++r) s += Z[(size_t)r * ldz + n];
+    P[(size_t)chunk * N + n] = s;
+}
+
+// FP32 host input [M x n_in] -> BF16 [M x kMlpIn], zero padded.
+__global__ void k_pack_input(const float *__restrict__ x, int M, int n_in, bf16 *__restrict__ X) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)M * kMlpIn) return;
+    const int r = (int)(e / kMlpIn), c = (int)(e % kMlpIn);
+    X[e] = (bf16)(c < n_in ? x[(size_t)r * n_in + c] : 0.f);
+}
+
+// Mean squared error: dOut = 2 (out - t) / (M n_out) (BF16 for the GEMMs, FP32 for the bias
+// gradient), padded columns 0; per-block partial sums of the squared error.
+__global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ out, const float *__restrict__ t, int M,
+                                                int n_out, bf16 *__restrict__ d16, float *__restrict__ d32,
+                                                double *__restrict__ partials) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double se = 0;
+    if (e < (size_t)M * kMlpOut) {
+        const int r = (int)(e / kMlpOut), c = (int)(e % kMlpOut);
+        float g = 0.f;
+        if (c < n_out) {
+            const float diff = out[e] - t[(size_t)r * n_out + c];
+            se = (double)diff * (double)diff;
+            g = 2.f * diff / ((float)M * (float)n_out);
+        }
+        d16[e] = (bf16)g;
+        d32[e] = g;
+    }
+    const double bs = block_sum(se);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// dZ = dH * (1 - H^2) for tanh layers (H the layer's BF16 output).
+__global__ void k_dtanh(const float *__restrict__ dH, const bf16 *__restrict__ H, size_t n, bf16 *__restrict__ d16,
+                        float *__restrict__ d32) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float h = (float)H[e];
+    const float g = dH[e] * (1.f - h * h);
+    d16[e] = (bf16)g;
+    d32[e] = g;
+}
+
+// Adam on the FP32 master parameters.
+__global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, size_t n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float gg = g[e];
+    const float mm = b1 * m[e] + (1.f - b1) * gg;
+    const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
+    m[e] = mm;
+    v[e] = vv;
+    p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// BF16 copies of layer weights W [in][out] (FP32, padded to Kp x Np): WT [Np][Kp] and W [Kp][Np].
+__global__ void k_pack_weights(const float *__restrict__ W, int Kp, int Np, bf16 *__restrict__ WT,
+                               bf16 *__restrict__ Wb) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Kp * Np) return;
+    const int k = e / Np, n = e - k * Np;
+    const bf16 v = (bf16)W[e];
+    Wb[e] = v;
+    WT[(size_t)n * Kp + k] = v;
+}
+
+}  // namespace gr
+
+using gr::guarded;
+using gr::require;
+
+struct gr_mlp {
+    uint32_t n_in = 0, n_out = 0;
+    // padded layer shapes: (in, out) = (64,256), (256,256), (256,256), (256,32)
+    int Kp[4] = {gr::kMlpIn, gr::kMlpW, gr::kMlpW, gr::kMlpW};
+    int Np[4] = {gr::kMlpW, gr::kMlpW, gr::kMlpW, gr::kMlpOut};
+    size_t woff[4], boff[4], n_params = 0;   // offsets into the padded FP32 parameter vector
+    hipStream_t stream = nullptr;
+    gr::DevArray<float> p, m, v, g;
+    gr::DevArray<gr::bf16> WT[4], Wb[4];
+    int cap = 0;   // rows of the activation buffers
+    gr::DevArray<float> xin, tgt, out, d32, dH, part;
+    gr::DevArray<gr::bf16> X0, H[3], d16;
+    gr::DevArray<double> lpart;
+    gr::DevArray<double> scal;
+    long step = 0;
+
+    ~gr_mlp() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+    void ensure(int M) {
+        if (M <= cap) return;
+        cap = ((M + 4095) / 4096) * 4096;
+        xin.alloc((size_t)cap * gr::kMlpIn);
+        tgt.alloc((size_t)cap * gr::kMlpOut);
+        out.alloc((size_t)cap * gr::kMlpOut);
+        X0.alloc((size_t)cap * gr::kMlpIn);
+        for (auto &h : H) h.alloc((size_t)cap * gr::kMlpW);
+        d16.alloc((size_t)cap * gr::kMlpW);
+        d32.alloc((size_t)cap * gr::kMlpW);
+        dH.alloc((size_t)cap * gr::kMlpW);
+        const int chunks = (cap + gr::kWgradRows - 1) / gr::kWgradRows;
+        part.alloc((size_t)chunks * gr::kMlpW * gr::kMlpW);
+        lpart.alloc((size_t)cap * gr::kMlpOut / gr::kBlock + 1);
+    }
+    void pack() {
+        for (int l = 0; l < 4; ++l)
+            hipLaunchKernelGGL(gr::k_pack_weights, dim3(gr::blocks_for((size_t)Kp[l] * Np[l])), dim3(gr::kBlock), 0,
+                               stream, p.p + woff[l], Kp[l], Np[l], WT[l].p, Wb[l].p);
+    }
+    void upload_x(const float *x, int M) {
+        GR_HIP(hipMemcpyAsync(xin.p, x, (size_t)M * n_in * sizeof(float), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(gr::k_pack_input, dim3(gr::blocks_for((size_t)M * gr::kMlpIn)), dim3(gr::kBlock), 0, stream,
+                           xin.p, M, (int)n_in, X0.p);
+    }
+    void forward(int M) {
+        const dim3 grid((M + gr::kRowsPerBlock - 1) / gr::kRowsPerBlock), blk(gr::kBlock);
+        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, stream, X0.p, gr::kMlpIn, M, Kp[0], WT[0].p,
+                           p.p + boff[0], H[0].p, gr::kMlpW);
+        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, stream, H[0].p, gr::kMlpW, M, Kp[1], WT[1].p,
+                           p.p + boff[1], H[1].p, gr::kMlpW);
+        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, stream, H[1].p, gr::kMlpW, M, Kp[2], WT[2].p,
+                           p.p + boff[2], H[2].p, gr::kMlpW);
+        hipLaunchKernelGGL((gr::k_dense<0, float, 2>), grid, blk, 0, stream, H[2].p, gr::kMlpW, M, Kp[3], WT[3].p,
+                           p.p + boff[3], out.p, gr::kMlpOut);
+    }
+    // weight and bias gradients of layer l from its input A (BF16, lda) and d16/d32 (ld = Np)
+    template <int NT>
+    void wgrad(int l, const gr::bf16 *A, int lda, int M) {
+        const int chunks = (M + gr::kWgradRows - 1) / gr::kWgradRows;
+        hipLaunchKernelGGL((gr::k_wgrad<NT>), dim3(Kp[l] / 64, chunks), dim3(gr::kBlock), 0, stream, A, lda, d16.p,
+                           Np[l], M, Kp[l], part.p);
+        const int n = Kp[l] * Np[l];
+        hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(n)), dim3(gr::kBlock), 0, stream, part.p, chunks, n,
+                           g.p + woff[l]);
+        hipLaunchKernelGGL(gr::k_colsum, dim3((Np[l] + gr::kBlock - 1) / gr::kBlock, chunks), dim3(gr::kBlock), 0,
+                           stream, d32.p, Np[l], M, Np[l], part.p);
+        hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(Np[l])), dim3(gr::kBlock), 0, stream, part.p, chunks,
+                           Np[l], g.p + boff[l]);
+    }
+    // loss and gradients for targets already in tgt
+    double backward(int M) {
+        const size_t no = (size_t)M * gr::kMlpOut;
+        const unsigned nb = gr::blocks_for(no);
+        hipLaunchKernelGGL(gr::k_mse, dim3(nb), dim3(gr::kBlock), 0, stream, out.p, tgt.p, M, (int)n_out, d16.p, d32.p,
+                           lpart.p);
+        std::vector<double> lp(nb);
+        GR_HIP(hipMemcpyAsync(lp.data(), lpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+        const dim3 grid((M + gr::kRowsPerBlock - 1) / gr::kRowsPerBlock), blk(gr::kBlock);
+        // layer 4 (linear): dW4 = H3^T dOut, dH3 = dOut W4^T
+        wgrad<2>(3, H[2].p, gr::kMlpW, M);
+        hipLaunchKernelGGL((gr::k_dense<0, float, 16>), grid, blk, 0, stream, d16.p, gr::kMlpOut, M, Np[3], Wb[3].p,
+                           (const float *)nullptr, dH.p, gr::kMlpW);
+        for (int l = 2; l >= 0; --l) {
+            const size_t nh = (size_t)M * gr::kMlpW;
+            hipLaunchKernelGGL(gr::k_dtanh, dim3(gr::blocks_for(nh)), dim3(gr::kBlock), 0, stream, dH.p, H[l].p, nh,
+                               d16.p, d32.p);
+            const gr::bf16 *A = l == 0 ? X0.p : H[l - 1].p;
+            const int lda = l == 0 ? gr::kMlpIn : gr::kMlpW;
+            wgrad<16>(l, A, lda, M);
+            if (l > 0)
+                hipLaunchKernelGGL((gr::k_dense<0, float, 16>), grid, blk, 0, stream, d16.p, gr::kMlpW, M, Np[l],
+                                   Wb[l].p, (const float *)nullptr, dH.p, gr::kMlpW);
+        }
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(stream));
+        double s = 0;
+        for (double x : lp) s += x;
+        return s / ((double)M * n_out);
+    }
+};
+
+namespace {
+void check_dims(gr_mlp *n, const float *x, int M) {
+    require(n && x, "null argument");
+    require(M > 0 && M <= (1 << 24), "row count must be in 1..2^24");
+}
+}  // namespace
+
+extern "C" {
+
+int gr_mlp_create(uint32_t n_in, uint32_t n_out, uint64_t seed, gr_mlp **out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        require(n_in >= 1 && n_in <= (uint32_t)gr::kMlpIn, "n_in must be 1..64");
+        require(n_out >= 1 && n_out <= (uint32_t)gr::kMlpOut, "n_out must be 1..32");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw gr::Error(GR_ERR_DEVICE, "no HIP device available");
+        auto *n = new gr_mlp();
+        try {
+            n->n_in = n_in;
+            n->n_out = n_out;
+            GR_HIP(hipStreamCreate(&n->stream));
+            size_t off = 0;
+            for (int l = 0; l < 4; ++l) {
+                n->woff[l] = off;
+                off += (size_t)n->Kp[l] * n->Np[l];
+                n->boff[l] = off;
+                off += n->Np[l];
+            }
+            n->n_params = off;
+            // Xavier-uniform weights on the real (unpadded) block, zero elsewhere
+            std::vector<float> h(off, 0.f);
+            std::mt19937_64 rng(seed);
+            const int real_in[4] = {(int)n_in, gr::kMlpW, gr::kMlpW, gr::kMlpW};
+            const int real_out[4] = {gr::kMlpW, gr::kMlpW, gr::kMlpW, (int)n_out};
+            for (int l = 0; l < 4; ++l) {
+                const double a = std::sqrt(6.0 / (real_in[l] + real_out[l]));
+                for (int k = 0; k < real_in[l]; ++k)
+                    for (int c = 0; c < real_out[l]; ++c) {
+                        const double u = (double)(rng() >> 11) * (1.0 / 9007199254740992.0);
+                        h[n->woff[l] + (size_t)k * n->Np[l] + c] = (float)((2 * u - 1) * a);
+                    }
+            }
+            for (auto *arr : {&n->p, &n->m, &n->v, &n->g}) arr->alloc(off);
+            n->p.upload(h.data(), off, n->stream);
+            GR_HIP(hipMemsetAsync(n->m.p, 0, off * sizeof(float), n->stream));
+            GR_HIP(hipMemsetAsync(n->v.p, 0, off * sizeof(float), n->stream));
+            GR_HIP(hipMemsetAsync(n->g.p, 0, off * sizeof(float), n->stream));
+            for (int l = 0; l < 4; ++l) {
+                n->WT[l].alloc((size_t)n->Kp[l] * n->Np[l]);
+                n->Wb[l].alloc((size_t)n->Kp[l] * n->Np[l]);
+            }
+            n->scal.alloc(1);
+            n->pack();
+            GR_HIP(hipStreamSynchronize(n->stream));
+        } catch (...) {
+            delete n;
+            throw;
+        }
+        *out = n;
+    });
+}
+
+void gr_mlp_free(gr_mlp *n) { delete n; }
+
+int gr_mlp_n_params(const gr_mlp *n, size_t *count) {
+    return guarded([&] {
+        require(n && count, "null argument");
+        *count = n->n_params;
+    });
+}
+
+// Parameters in the padded layout: per layer W [in_pad][out_pad] then b [out_pad], with
+// (in_pad, out_pad) = (64,256), (256,256), (256,256), (256,32).
+int gr_mlp_get_params(const gr_mlp *n, float *out) {
+    return guarded([&] {
+        require(n && out, "null argument");
+        n->p.download(out, n->n_params, n->stream);
+        GR_HIP(hipStreamSynchronize(n->stream));
+    });
+}
+
+int gr_mlp_set_params(gr_mlp *n, const float *in) {
+    return guarded([&] {
+        require(n && in, "null argument");
+        n->p.upload(in, n->n_params, n->stream);
+        n->pack();
+        GR_HIP(hipStreamSynchronize(n->stream));
+    });
+}
+
+int gr_mlp_forward(gr_mlp *n, const float *x, int M, float *y) {
+    return guarded([&] {
+        check_dims(n, x, M);
+        require(y != nullptr, "null output");
+        const int chunk = 1 << 18;
+        std::vector<float> o;
+        for (int r0 = 0; r0 < M; r0 += chunk) {
+            const int m = std::min(chunk, M - r0);
+            n->ensure(m);
+            n->upload_x(x + (size_t)r0 * n->n_in, m);
+            n->forward(m);
+            GR_HIP(hipGetLastError());
+            o.resize((size_t)m * gr::kMlpOut);
+            n->out.download(o.data(), o.size(), n->stream);
+            GR_HIP(hipStreamSynchronize(n->stream));
+            for (int r = 0; r < m; ++r)
+                for (uint32_t c = 0; c < n->n_out; ++c) y[(size_t)(r0 + r) * n->n_out + c] = o[(size_t)r * gr::kMlpOut + c];
+        }
+    });
+}
+
+int gr_mlp_loss_grad(gr_mlp *n, const float *x, const float *t, int M, double *loss, float *grad) {
+    return guarded([&] {
+        check_dims(n, x, M);
+        require(t && loss, "null argument");
+        require(M <= (1 << 20), "at most 2^20 rows per gradient step");
+        n->ensure(M);
+        n->upload_x(x, M);
+        GR_HIP(hipMemcpyAsync(n->tgt.p, t, (size_t)M * n->n_out * sizeof(float), hipMemcpyHostToDevice, n->stream));
+        n->forward(M);
+        *loss = n->backward(M);
+        if (grad) {
+            n->g.download(grad, n->n_params, n->stream);
+            GR_HIP(hipStreamSynchronize(n->stream));
+        }
+    });
+}
+
+int gr_mlp_train_step(gr_mlp *n, const float *x, const float *t, int M, float lr, double *loss) {
+    return guarded([&] {
+        check_dims(n, x, M);
+        require(t && loss, "null argument");
+        require(M <= (1 << 20), "at most 2^20 rows per training step");
+        require(lr > 0, "learning rate must be positive");
+        n->ensure(M);
+        n->upload_x(x, M);
+        GR_HIP(hipMemcpyAsync(n->tgt.p, t, (size_t)M * n->n_out * sizeof(float), hipMemcpyHostToDevice, n->stream));
+        n->forward(M);
+        *loss = n->backward(M);
+        n->step += 1;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)n->step), c2 = 1.f - std::pow(b2, (float)n->step);
+        hipLaunchKernelGGL(gr::k_adam, dim3(gr::blocks_for(n->n_params)), dim3(gr::kBlock), 0, n->stream, n->p.p, n->m.p,
+                           n->v.p, n->g.p, n->n_params, lr, b1, b2, 1e-8f, c1, c2);
+        n->pack();
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(n->stream));
+    });
+}
+
+}  // extern "C"

@@ -266,10 +266,75 @@ int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]);
  * counted once), same order as gr_solver_profile. */
 int gr_solver_kernel_bytes(const gr_solver *s, double bytes[4]);
 
+/* ---- batched solver: up to 64 flow instances on one mesh (no reference counterpart) ---------
+ * All instances share the mesh, the boundary kinds, p and T; each has its own far-field Mach
+ * vector.  State layout is instance-inner (one wavefront per cell, lanes = instances); in
+ * binary64 each instance evolves bit for bit as a gr_solver with the same far field would.
+ * Convergence is per instance: an instance that reaches the tolerance stops changing while
+ * the others continue. */
+typedef struct gr_batch gr_batch;
+
+int gr_batch_create(gr_mesh *m, uint32_t n_inst, const gr_solver_config *cfg, gr_batch **out);
+void gr_batch_free(gr_batch *b);
+/* Boundary kind of a region; far-field regions use each instance's Mach vector. */
+int gr_batch_set_bc(gr_batch *b, const char *region, int kind);
+/* Far field of every instance (mach: 3 per instance) and freestream initial state. */
+int gr_batch_set_freestream(gr_batch *b, const double *mach, double pressure, double temperature);
+/* State of one instance, reference layouts (rho N+G, rhou 3(N+G), rhoe N+G). */
+int gr_batch_set_state(gr_batch *b, uint32_t inst, const double *rho, const double *rhou, const double *rhoe);
+int gr_batch_get_state(gr_batch *b, uint32_t inst, double *rho, double *rhou, double *rhoe);
+/* Up to max_iter iterations; instance k stops after the iteration whose residual / reference[k]
+ * falls below tol (reference NULL or entries <= 0: that instance's first residual).
+ * iterations/converged (n_inst entries, may be NULL) receive per-instance results; launched
+ * receives the number of batch iterations that did work. */
+int gr_batch_iterate(gr_batch *b, int max_iter, double tol, const double *reference, int *iterations,
+                     int *converged, int *launched);
+int gr_batch_history(const gr_batch *b, uint32_t inst, double *hist, int n);
+/* Residual sum of each instance's current state (n_inst entries), state unchanged. */
+int gr_batch_residual_norms(gr_batch *b, double *out);
+/* Mean kernel durations (ms) over n_iter iterations: [0] gradient+limiter, [1] flux+update,
+ * [2] per-instance residual reduction; and the algorithmic bytes of one launch of each. */
+int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]);
+int gr_batch_kernel_bytes(const gr_batch *b, double bytes[3]);
+/* Iterations captured per HIP graph (even, 2..200; default 10). */
+int gr_batch_set_graph_iters(gr_batch *b, int iters);
+
+/* ---- post-processing (no reference counterpart) ---------------------------------------------
+ * Pressure force on a wall region: F = sum over its faces of p_cell * n * A, n pointing out
+ * of the fluid cell (into the body); p from the current state (first-order wall pressure).
+ * Lift and drag are its projections.  gr_batch_force writes 3 entries per instance. */
+int gr_solver_force(gr_solver *s, const char *region, double F[3]);
+int gr_batch_force(gr_batch *b, const char *region, double *F);
+/* Distance from each interior cell centre to the nearest face centre of the given wall
+ * regions (out: n_cells entries). */
+int gr_mesh_wall_distance(gr_mesh *m, const char *const *regions, uint32_t n_regions, double *out);
+
 /* Bandwidth probe (no reference counterpart): a plain copy kernel over two device buffers of
  * `bytes` each, run `reps` times; gbps receives (read + written bytes) / mean kernel time.
  * Calibrates hardware byte counters and measures the achievable HBM rate. */
 int gr_stream_copy_probe(size_t bytes, int reps, double *gbps);
+
+/* ---- warm-start network (no reference counterpart) -------------------------------------------
+ * A 4-layer MLP n_in -> 256 -> 256 -> 256 -> n_out (tanh on the hidden layers) trained and run
+ * with bfloat16 matrix-core products accumulated in binary32.  Parameters are binary32 in a
+ * padded layout: per layer W [in_pad][out_pad] (row-major, y = x W + b) then b [out_pad], with
+ * (in_pad, out_pad) = (64,256), (256,256), (256,256), (256,32).  Rows are samples, row-major. */
+typedef struct gr_mlp gr_mlp;
+
+/* n_in 1..64, n_out 1..32; Xavier-uniform weights from seed, zero biases. */
+int gr_mlp_create(uint32_t n_in, uint32_t n_out, uint64_t seed, gr_mlp **out);
+void gr_mlp_free(gr_mlp *n);
+int gr_mlp_n_params(const gr_mlp *n, size_t *count);
+int gr_mlp_get_params(const gr_mlp *n, float *out);
+int gr_mlp_set_params(gr_mlp *n, const float *in);
+/* y [M][n_out] = MLP(x [M][n_in]), M up to 2^24 rows. */
+int gr_mlp_forward(gr_mlp *n, const float *x, int M, float *y);
+/* Mean squared error against t [M][n_out] and its gradient (padded layout; grad may be NULL),
+ * parameters unchanged.  M up to 2^20. */
+int gr_mlp_loss_grad(gr_mlp *n, const float *x, const float *t, int M, double *loss, float *grad);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the mean squared error of one mini-batch;
+ * loss receives the loss before the step.  Deterministic: the same calls give the same bits. */
+int gr_mlp_train_step(gr_mlp *n, const float *x, const float *t, int M, float lr, double *loss);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,15 @@ def lib() -> C.CDLL:
         L.orc_euler_iterate.argtypes = [mp, C.POINTER(_OrcState), C.POINTER(_OrcBC), C.c_int,
                                         C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.c_double, _f64p, C.c_int]
+        L.orc_mlp_n_params.restype = C.c_size_t
+        L.orc_mlp_n_params.argtypes = []
+        L.orc_bf16_round.restype = C.c_double
+        L.orc_bf16_round.argtypes = [C.c_double]
+        L.orc_mlp_forward.restype = C.c_int
+        L.orc_mlp_forward.argtypes = [_f64p, C.c_int, C.c_int, _f64p, C.c_int, C.c_int, _f64p]
+        L.orc_mlp_loss_grad.restype = C.c_int
+        L.orc_mlp_loss_grad.argtypes = [_f64p, C.c_int, C.c_int, _f64p, _f64p, C.c_int, C.c_int,
+                                        C.POINTER(C.c_double), _f64p]
         _lib = L
     return _lib
 
@@ -285,3 +294,41 @@ def iterate(m: OracleMesh, s: EulerState, bcs, cfl: float, n_iter: int, tol: flo
     n = lib().orc_euler_iterate(m._p, C.byref(st), arr, len(bcs), cfl, limiter, int(corrected),
                                 n_iter, tol, ref_norm, hist, threads)
     return n, hist[:n]
+
+
+# ---- warm-start MLP (oracle_mlp.c) ----
+
+MLP_SHAPES = ((64, 256), (256, 256), (256, 256), (256, 32))  # padded (in, out) per layer
+
+
+def mlp_n_params() -> int:
+    return int(lib().orc_mlp_n_params())
+
+
+def bf16_round(v: float) -> float:
+    return float(lib().orc_bf16_round(float(v)))
+
+
+def mlp_forward(params, x, n_out: int, emulate_bf16: bool = False) -> np.ndarray:
+    """y = MLP(x) in binary64 (optionally with the device's bfloat16 operand rounding)."""
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    M, n_in = x.shape
+    y = np.zeros((M, n_out))
+    if lib().orc_mlp_forward(p, n_in, n_out, x.reshape(-1), M, int(emulate_bf16), y.reshape(-1)):
+        raise ValueError("bad MLP dimensions")
+    return y
+
+
+def mlp_loss_grad(params, x, t, emulate_bf16: bool = False):
+    """(mean squared error, gradient in the padded parameter layout)."""
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    M, n_in = x.shape
+    g = np.zeros(mlp_n_params())
+    loss = C.c_double()
+    if lib().orc_mlp_loss_grad(p, n_in, t.shape[1], x.reshape(-1), t.reshape(-1), M, int(emulate_bf16),
+                               C.byref(loss), g):
+        raise ValueError("bad MLP dimensions")
+    return loss.value, g

@@ -162,6 +162,14 @@ int orc_euler_iterate(const orc_mesh *m, orc_euler_state *s, const orc_bc *bcs, 
                       double cfl, int limiter_kind, int corrected, int n_iter, double tol,
                       double ref_norm, double *history, int n_threads);
 
+/* Warm-start MLP (oracle_mlp.c): padded parameter layout of the device library. */
+size_t orc_mlp_n_params(void);
+double orc_bf16_round(double v);
+int orc_mlp_forward(const double *params, int n_in, int n_out, const double *x, int M, int emulate_bf16,
+                    double *y);
+int orc_mlp_loss_grad(const double *params, int n_in, int n_out, const double *x, const double *t, int M,
+                      int emulate_bf16, double *loss, double *grad);
+
 #ifdef __cplusplus
 }
 #endif
