@@ -16,7 +16,7 @@ import numpy as np
 
 from ._capi import (GR_BC_FARFIELD, GR_BC_SLIP, GR_F32, GR_F64, GR_HEX, GR_LIM_BARTH_JESPERSEN,
                     GR_LIM_MICHALAK, GR_LIM_VENKATAKRISHNAN, GR_OK, GR_PYRAMID, GR_QUAD, GR_TET,
-                    GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, load)
+                    GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, WarmNorm, load)
 
 __all__ = ["GraphiteError", "Elements", "Mesh", "Field", "Solver", "device_count", "load",
            "euler_init", "euler_bc_slip", "euler_bc_farfield", "euler_step_prepare", "euler_step",
@@ -507,6 +507,37 @@ class Batch:
     def set_graph_iters(self, iters: int) -> None:
         _check(load().gr_batch_set_graph_iters(self._h, iters))
 
+    def set_states(self, rho, rhou, rhoe) -> None:
+        """All instances: rho/rhoe (n_inst, N+G), rhou (n_inst, N+G, 3)."""
+        r, u, e = _f64(rho).reshape(-1), _f64(rhou).reshape(-1), _f64(rhoe).reshape(-1)
+        _check(load().gr_batch_set_states(self._h, _p(r, C.c_double), _p(u, C.c_double), _p(e, C.c_double)))
+
+    def get_states(self):
+        s = self.mesh.sizes()
+        n = s.n_cells + s.n_ghosts
+        r, u, e = np.zeros((self.n_inst, n)), np.zeros((self.n_inst, n, 3)), np.zeros((self.n_inst, n))
+        _check(load().gr_batch_get_states(self._h, _p(r, C.c_double), _p(u, C.c_double), _p(e, C.c_double)))
+        return r, u, e
+
+    def warm_rows(self, wall_distance):
+        """Raw warm-start inputs (n_inst*N, 6) and targets (n_inst*N, 5) of the current states,
+        row = cell * n_inst + instance."""
+        wd = _f64(wall_distance)
+        rows = self.mesh.n_cells * self.n_inst
+        x, t = np.zeros((rows, 6), np.float32), np.zeros((rows, 5), np.float32)
+        _check(load().gr_batch_warm_rows(self._h, _p(wd, C.c_double), _p(x, C.c_float), _p(t, C.c_float)))
+        return x, t
+
+    def warm_start(self, net, wall_distance, norm, max_ratio: float = 1.0):
+        """Network prediction of every instance, judged by its residual (see gr_batch_warm_start).
+        Returns (freestream residuals, kept flags)."""
+        wd = _f64(wall_distance)
+        fs = np.zeros(self.n_inst)
+        used = (C.c_int * self.n_inst)()
+        _check(load().gr_batch_warm_start(self._h, net._h, _p(wd, C.c_double), C.byref(norm), max_ratio,
+                                          _p(fs, C.c_double), used))
+        return fs, np.array(used[:], dtype=bool)
+
 
 def _solver_force(self, region: str) -> np.ndarray:
     F = np.zeros(3)
@@ -597,6 +628,19 @@ class Mlp:
                                        C.byref(loss), _p(g, C.c_float)))
         return loss.value, g
 
+    def set_data(self, x, t) -> None:
+        """Keep a training set resident on the device."""
+        x, t = _f32(x), _f32(t)
+        assert x.shape[1] == self.n_in and t.shape == (x.shape[0], self.n_out)
+        _check(load().gr_mlp_set_data(self._h, _p(x, C.c_float), _p(t, C.c_float), x.shape[0]))
+
+    def train(self, batch: int, steps: int, lr: float, lr_final: float, seed: int = 0) -> np.ndarray:
+        """Adam steps over the resident set (permuted passes, cosine learning-rate decay);
+        returns the loss of every step."""
+        losses = np.zeros(steps)
+        _check(load().gr_mlp_train(self._h, batch, steps, lr, lr_final, seed, _p(losses, C.c_double)))
+        return losses
+
     def train_step(self, x, t, lr: float) -> float:
         x, t = _f32(x), _f32(t)
         assert x.shape[1] == self.n_in and t.shape == (x.shape[0], self.n_out)
@@ -604,3 +648,23 @@ class Mlp:
         _check(load().gr_mlp_train_step(self._h, _p(x, C.c_float), _p(t, C.c_float), x.shape[0], lr,
                                         C.byref(loss)))
         return loss.value
+
+
+def warm_norm(x, t) -> WarmNorm:
+    """Input/output standardisation of a warm-start training set (mean, std per column)."""
+    x, t = np.asarray(x, np.float64), np.asarray(t, np.float64)
+    n = WarmNorm()
+    xm, xs = x.mean(0), x.std(0)
+    tm, ts = t.mean(0), t.std(0)
+    for c in range(6):
+        n.in_mean[c], n.in_std[c] = xm[c], xs[c] if xs[c] > 1e-12 else 1.0
+    for c in range(5):
+        n.out_mean[c], n.out_std[c] = tm[c], ts[c] if ts[c] > 1e-12 else 1.0
+    return n
+
+
+def normalise(x, t, norm: WarmNorm):
+    """(x - in_mean)/in_std and (t - out_mean)/out_std as binary32."""
+    xm, xs = np.array(norm.in_mean[:]), np.array(norm.in_std[:])
+    tm, ts = np.array(norm.out_mean[:]), np.array(norm.out_std[:])
+    return (((x - xm) / xs).astype(np.float32), ((t - tm) / ts).astype(np.float32))

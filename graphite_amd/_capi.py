@@ -41,6 +41,11 @@ class SolveResult(C.Structure):
     _fields_ = [("iterations", _i), ("converged", _i), ("residual", _d), ("ratio", _d), ("reference", _d)]
 
 
+class WarmNorm(C.Structure):
+    _fields_ = [("in_mean", C.c_float * 6), ("in_std", C.c_float * 6),
+                ("out_mean", C.c_float * 5), ("out_std", C.c_float * 5)]
+
+
 SIGNATURES = {
     "gr_last_error": (C.c_char_p, []),
     "gr_abi_version": (_i, []),
@@ -125,6 +130,12 @@ SIGNATURES = {
     "gr_mlp_forward": (_i, [_vp, _fp, _i, _fp]),
     "gr_mlp_loss_grad": (_i, [_vp, _fp, _fp, _i, _dp, _fp]),
     "gr_mlp_train_step": (_i, [_vp, _fp, _fp, _i, C.c_float, _dp]),
+    "gr_mlp_set_data": (_i, [_vp, _fp, _fp, C.c_uint64]),
+    "gr_mlp_train": (_i, [_vp, _i, _i, C.c_float, C.c_float, C.c_uint64, _dp]),
+    "gr_batch_set_states": (_i, [_vp, _dp, _dp, _dp]),
+    "gr_batch_get_states": (_i, [_vp, _dp, _dp, _dp]),
+    "gr_batch_warm_rows": (_i, [_vp, _dp, _fp, _fp]),
+    "gr_batch_warm_start": (_i, [_vp, _vp, _dp, C.POINTER(WarmNorm), _d, _dp, C.POINTER(_i)]),
 }
 
 _lib = None

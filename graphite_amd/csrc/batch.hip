@@ -302,6 +302,121 @@ __global__ void k_batch_ghosts(MeshView m, S *u, size_t stride, BcTable bc, cons
     for (int c = 0; c < 5; ++c) u[c * stride + (size_t)c0 * kBatchLanes + k] = ug[c];
 }
 
+// ---- learned warm start: network rows are (cell i, instance k), row = i * n_inst + k ----
+
+struct FarField {
+    double rho, c, E;   // density, speed of sound, total energy per volume without the kinetic part
+};
+
+__device__ inline FarField far_field(double p_inf, double T_inf) {
+    FarField f;
+    f.rho = p_inf / T_inf;
+    f.c = dsqrt((1.4 * p_inf) / f.rho);
+    f.E = p_inf / 0.4;
+    return f;
+}
+
+// Raw network inputs of rows row0..row0+rows-1; with nrm set, normalised and written as
+// bfloat16 into X [rows][64] (zero padded); else binary32 into X [rows][6].
+template <bool NORM>
+__global__ void k_warm_features(MeshView m, const double *__restrict__ wd, const double *__restrict__ inst_mach,
+                                uint32_t n_inst, size_t row0, int rows, gr_warm_norm nrm, void *X) {
+    constexpr int W = NORM ? 64 : 6;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)rows * W) return;
+    const int c = (int)(e % W);
+    const size_t r = row0 + e / W;
+    const uint32_t i = (uint32_t)(r / n_inst), k = (uint32_t)(r % n_inst);
+    double f = 0;
+    if (c < 6) {
+        const double mx = inst_mach[3 * k], my = inst_mach[3 * k + 1], mz = inst_mach[3 * k + 2];
+        switch (c) {
+        case 0: f = m.cx[i]; break;
+        case 1: f = m.cy[i]; break;
+        case 2: f = m.cz[i]; break;
+        case 3: f = log(wd[i] + 1e-3); break;
+        case 4: f = dsqrt((mx * mx + my * my) + mz * mz); break;
+        default: f = atan2(my, mx); break;
+        }
+    }
+    if (NORM) {
+        const float v = c < 6 ? (float)((f - nrm.in_mean[c]) / nrm.in_std[c]) : 0.f;
+        ((__bf16 *)X)[e] = (__bf16)v;
+    } else {
+        ((float *)X)[e] = (float)f;
+    }
+}
+
+// Raw targets t [rows][5] of every interior cell and instance from the state array.
+template <typename S>
+__global__ void k_warm_targets(const S *__restrict__ u, size_t stride, uint32_t N, uint32_t n_inst,
+                               const double *__restrict__ inst_mach, double p_inf, double T_inf, float *__restrict__ t) {
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (size_t)N * n_inst) return;
+    const uint32_t i = (uint32_t)(r / n_inst), k = (uint32_t)(r % n_inst);
+    const FarField ff = far_field(p_inf, T_inf);
+    const size_t o = (size_t)i * kBatchLanes + k;
+    double vel[3], ke = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        vel[d] = inst_mach[3 * k + d] * ff.c;
+        ke += vel[d] * vel[d];
+    }
+    const double Einf = ff.E + 0.5 * ff.rho * ke;
+    t[5 * r + 0] = (float)((double)u[o] / ff.rho - 1.0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        t[5 * r + 1 + d] = (float)(((double)u[(1 + d) * stride + o] - ff.rho * vel[d]) / (ff.rho * ff.c));
+    t[5 * r + 4] = (float)(((double)u[4 * stride + o] - Einf) / (ff.rho * ff.c * ff.c));
+}
+
+// Network outputs Y [rows][32] of rows row0.. -> conservative state in both state buffers.
+template <typename S>
+__global__ void k_warm_apply(const float *__restrict__ Y, size_t row0, int rows, uint32_t n_inst, gr_warm_norm nrm,
+                             const double *__restrict__ inst_mach, double p_inf, double T_inf, S *__restrict__ u0,
+                             S *__restrict__ u1, size_t stride) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= rows) return;
+    const size_t r = row0 + q;
+    const uint32_t i = (uint32_t)(r / n_inst), k = (uint32_t)(r % n_inst);
+    double t[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) t[c] = (double)Y[(size_t)q * 32 + c] * nrm.out_std[c] + nrm.out_mean[c];
+    const FarField ff = far_field(p_inf, T_inf);
+    double vel[3], ke = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        vel[d] = inst_mach[3 * k + d] * ff.c;
+        ke += vel[d] * vel[d];
+    }
+    S v[5];
+    v[0] = (S)(ff.rho * (1.0 + t[0]));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[1 + d] = (S)(ff.rho * vel[d] + ff.rho * ff.c * t[1 + d]);
+    v[4] = (S)(ff.E + 0.5 * ff.rho * ke + ff.rho * ff.c * ff.c * t[4]);
+    const size_t o = (size_t)i * kBatchLanes + k;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        u0[c * stride + o] = v[c];
+        u1[c * stride + o] = v[c];
+    }
+}
+
+// Lanes with reset[lane] set get vals[5*lane..] in every cell and ghost of both buffers.
+template <typename S>
+__global__ void k_lane_reset(S *__restrict__ u0, S *__restrict__ u1, size_t stride, size_t NG,
+                             const int *__restrict__ reset, const double *__restrict__ vals) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NG * kBatchLanes) return;
+    const uint32_t k = (uint32_t)(e & (kBatchLanes - 1));
+    if (!reset[k]) return;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        u0[c * stride + e] = (S)vals[5 * k + c];
+        u1[c * stride + e] = (S)vals[5 * k + c];
+    }
+}
+
 }  // namespace gr
 
 using gr::guarded;
@@ -320,6 +435,8 @@ struct gr_batch {
     int cur = 0, ghost_src = 0;
     gr::DevArray<gr::BatchCtl> ctl;
     gr::DevArray<double> partials, partials2, hist, out;
+    gr::DevArray<int> lane_flags;      // 64
+    gr::DevArray<double> lane_vals;    // 5 per lane
     int hist_cap = 0;
     gr::BatchCtl *ctl_host = nullptr;   // pinned: [0] upload, [1],[2] polling
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -410,23 +527,33 @@ struct gr_batch {
         out3[1] = I * su * (25 * N + 5 * N) + 8.0 * (3 * (N + G) + N + 7 * F) + 4.0 * (N + 1 + 2 * slots);
         out3[2] = 8.0 * 64 * (nblocks() + 2.0 * gr::kReduceBlocks);
     }
-    template <typename S> void fill_freestream() {
-        // every lane (padding lanes copy instance 0) of every cell and ghost, both buffers
+    // Conservative freestream state of lane k (padding lanes: instance 0).
+    void freestream_values(uint32_t k, double val[5]) const {
+        const double *mk = &mach_h[3 * (k < n_inst ? k : 0)];
+        const double rho_v = p_inf / T_inf;
+        const double c = std::sqrt((1.4 * p_inf) / rho_v);
+        const double vel[3] = {mk[0] * c, mk[1] * c, mk[2] * c};
+        const double e = p_inf / 0.4 + (0.5 * rho_v) * ((vel[0] * vel[0] + vel[1] * vel[1]) + vel[2] * vel[2]);
+        val[0] = rho_v;
+        for (int d = 0; d < 3; ++d) val[1 + d] = vel[d] * rho_v;
+        val[4] = e;
+    }
+    // Lanes with flags[k] set (64 entries) back to their freestream, every cell and ghost of
+    // both state buffers (padding lanes copy instance 0).
+    template <typename S> void reset_lanes(const int *flags) {
         const size_t NG = (size_t)mesh->h.n_cells + mesh->h.n_ghosts;
-        std::vector<S> soa(5 * stride, S(0));
-        for (uint32_t k = 0; k < (uint32_t)gr::kBatchLanes; ++k) {
-            const double *mk = &mach_h[3 * (k < n_inst ? k : 0)];
-            const double rho_v = p_inf / T_inf;
-            const double c = std::sqrt((1.4 * p_inf) / rho_v);
-            const double vel[3] = {mk[0] * c, mk[1] * c, mk[2] * c};
-            const double e = p_inf / 0.4 + (0.5 * rho_v) * ((vel[0] * vel[0] + vel[1] * vel[1]) + vel[2] * vel[2]);
-            const double val[5] = {rho_v, vel[0] * rho_v, vel[1] * rho_v, vel[2] * rho_v, e};
-            for (size_t i = 0; i < NG; ++i)
-                for (int q = 0; q < 5; ++q) soa[q * stride + i * gr::kBatchLanes + k] = (S)val[q];
-        }
-        for (int b = 0; b < 2; ++b)
-            GR_HIP(hipMemcpyAsync(ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice, mesh->stream));
+        std::vector<double> vals(5 * gr::kBatchLanes);
+        for (uint32_t k = 0; k < (uint32_t)gr::kBatchLanes; ++k) freestream_values(k, &vals[5 * k]);
+        lane_flags.upload(flags, gr::kBatchLanes, mesh->stream);
+        lane_vals.upload(vals.data(), vals.size(), mesh->stream);
+        hipLaunchKernelGGL(gr::k_lane_reset<S>, dim3(gr::blocks_for(NG * gr::kBatchLanes)), dim3(gr::kBlock), 0,
+                           mesh->stream, (S *)ubuf[0].p, (S *)ubuf[1].p, stride, NG, lane_flags.p, lane_vals.p);
+        GR_HIP(hipGetLastError());
         GR_HIP(hipStreamSynchronize(mesh->stream));
+    }
+    template <typename S> void fill_freestream() {
+        std::vector<int> all(gr::kBatchLanes, 1);
+        reset_lanes<S>(all.data());
     }
     template <typename S> void set_instance(uint32_t k, const double *rho, const double *rhou, const double *rhoe) {
         const size_t NG = (size_t)mesh->h.n_cells + mesh->h.n_ghosts;
@@ -444,6 +571,80 @@ struct gr_batch {
             GR_HIP(hipMemcpyAsync(ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice, mesh->stream));
         GR_HIP(hipStreamSynchronize(mesh->stream));
         ghost_src = cur;
+    }
+    // Residual sum of each instance's current state (n_inst entries), state unchanged.
+    void residual_norms(double *res) {
+        gr_mesh *m = mesh;
+        // a control block with nothing done and no stop, not recording (max_iter 0 marks it)
+        gr::BatchCtl &c0 = ctl_host[0];
+        std::memset(&c0, 0, sizeof c0);
+        for (int k = 0; k < gr::kBatchLanes; ++k) c0.done[k] = k < (int)n_inst ? 0 : 1;
+        GR_HIP(hipMemcpyAsync(ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
+        const gr::BcTable t = table();
+        const dim3 grid(nblocks()), blk(gr::kBlock);
+        if (cfg.dtype == GR_F64) {
+            hipLaunchKernelGGL(gr::k_batch_gradlim<double>, grid, blk, 0, m->stream, m->view(), args<double>(cur), t,
+                               (const double *)inst_mach.p, cfg.limiter, (const gr::BatchCtl *)ctl.p);
+            hipLaunchKernelGGL((gr::k_batch_flux<double, false>), grid, blk, 0, m->stream, m->view(), args<double>(cur),
+                               t, (const double *)inst_mach.p, cfg.cfl, partials.p, (const gr::BatchCtl *)ctl.p);
+        } else {
+            hipLaunchKernelGGL(gr::k_batch_gradlim<float>, grid, blk, 0, m->stream, m->view(), args<float>(cur), t,
+                               (const double *)inst_mach.p, cfg.limiter, (const gr::BatchCtl *)ctl.p);
+            hipLaunchKernelGGL((gr::k_batch_flux<float, false>), grid, blk, 0, m->stream, m->view(), args<float>(cur),
+                               t, (const double *)inst_mach.p, cfg.cfl, partials.p, (const gr::BatchCtl *)ctl.p);
+        }
+        hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream, partials.p,
+                           nblocks(), (const gr::BatchCtl *)nullptr, partials2.p);
+        hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,
+                           partials2.p, gr::kReduceBlocks, (gr::BatchCtl *)nullptr, (double *)nullptr, out.p);
+        GR_HIP(hipGetLastError());
+        std::vector<double> o(gr::kBatchLanes);
+        out.download(o.data(), o.size(), m->stream);
+        GR_HIP(hipStreamSynchronize(m->stream));
+        std::copy(o.begin(), o.begin() + n_inst, res);
+        ghost_src = cur;
+    }
+    // All instances at once (one transfer each way): rho/rhoe [n_inst][N+G], rhou [n_inst][N+G][3].
+    template <typename S> void set_all(const double *rho, const double *rhou, const double *rhoe) {
+        const size_t NG = (size_t)mesh->h.n_cells + mesh->h.n_ghosts;
+        std::vector<S> soa(5 * stride, S(0));
+        for (uint32_t k = 0; k < (uint32_t)gr::kBatchLanes; ++k) {
+            const size_t src = (size_t)(k < n_inst ? k : 0) * NG;   // padding lanes copy instance 0
+            for (size_t i = 0; i < NG; ++i) {
+                const size_t o = i * gr::kBatchLanes + k;
+                soa[o] = (S)rho[src + i];
+                for (int d = 0; d < 3; ++d) soa[(1 + d) * stride + o] = (S)rhou[3 * (src + i) + d];
+                soa[4 * stride + o] = (S)rhoe[src + i];
+            }
+        }
+        for (int b = 0; b < 2; ++b)
+            GR_HIP(hipMemcpyAsync(ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice, mesh->stream));
+        GR_HIP(hipStreamSynchronize(mesh->stream));
+        ghost_src = cur;
+    }
+    template <typename S> void get_all(double *rho, double *rhou, double *rhoe) {
+        gr_mesh *m = mesh;
+        const size_t N = m->h.n_cells, NG = N + m->h.n_ghosts;
+        if (ghost_src != cur && m->h.n_ghosts) {
+            hipLaunchKernelGGL(gr::k_batch_ghosts<S>, dim3((m->h.n_ghosts + gr::kCellsPerBlock - 1) / gr::kCellsPerBlock),
+                               dim3(gr::kBlock), 0, m->stream, m->view(), (S *)ubuf[ghost_src].p, stride, table(),
+                               (const double *)inst_mach.p);
+            GR_HIP(hipGetLastError());
+        }
+        std::vector<S> a(5 * stride), b(ghost_src != cur ? 5 * stride : 0);
+        GR_HIP(hipMemcpyAsync(a.data(), ubuf[cur].p, a.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
+        if (!b.empty())
+            GR_HIP(hipMemcpyAsync(b.data(), ubuf[ghost_src].p, b.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
+        GR_HIP(hipStreamSynchronize(m->stream));
+        for (uint32_t k = 0; k < n_inst; ++k)
+            for (size_t i = 0; i < NG; ++i) {
+                const std::vector<S> &src = (i < N || b.empty()) ? a : b;
+                const size_t o = i * gr::kBatchLanes + k, d0 = (size_t)k * NG + i;
+                if (rho) rho[d0] = (double)src[o];
+                if (rhou)
+                    for (int d = 0; d < 3; ++d) rhou[3 * d0 + d] = (double)src[(1 + d) * stride + o];
+                if (rhoe) rhoe[d0] = (double)src[4 * stride + o];
+            }
     }
     template <typename S> void get_instance(uint32_t k, double *rho, double *rhou, double *rhoe) {
         gr_mesh *m = mesh;
@@ -502,6 +703,8 @@ int gr_batch_create(gr_mesh *m, uint32_t n_inst, const gr_solver_config *cfg, gr
             b->partials.alloc((size_t)b->nblocks() * gr::kBatchLanes);
             b->partials2.alloc((size_t)gr::kReduceBlocks * gr::kBatchLanes);
             b->out.alloc(gr::kBatchLanes);
+            b->lane_flags.alloc(gr::kBatchLanes);
+            b->lane_vals.alloc(5 * gr::kBatchLanes);
             b->hist_cap = 1 << 14;
             b->hist.alloc((size_t)b->hist_cap * gr::kBatchLanes);
             GR_HIP(hipHostMalloc((void **)&b->ctl_host, 3 * sizeof(gr::BatchCtl)));
@@ -563,6 +766,22 @@ int gr_batch_get_state(gr_batch *b, uint32_t inst, double *rho, double *rhou, do
         require(inst < b->n_inst, "instance index out of range");
         if (b->cfg.dtype == GR_F64) b->get_instance<double>(inst, rho, rhou, rhoe);
         else b->get_instance<float>(inst, rho, rhou, rhoe);
+    });
+}
+
+int gr_batch_set_states(gr_batch *b, const double *rho, const double *rhou, const double *rhoe) {
+    return guarded([&] {
+        require(b && rho && rhou && rhoe, "null argument");
+        if (b->cfg.dtype == GR_F64) b->set_all<double>(rho, rhou, rhoe);
+        else b->set_all<float>(rho, rhou, rhoe);
+    });
+}
+
+int gr_batch_get_states(gr_batch *b, double *rho, double *rhou, double *rhoe) {
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        if (b->cfg.dtype == GR_F64) b->get_all<double>(rho, rhou, rhoe);
+        else b->get_all<float>(rho, rhou, rhoe);
     });
 }
 
@@ -639,37 +858,7 @@ int gr_batch_history(const gr_batch *b, uint32_t inst, double *hist, int n) {
 int gr_batch_residual_norms(gr_batch *b, double *out) {
     return guarded([&] {
         require(b && out, "null argument");
-        gr_mesh *m = b->mesh;
-        // a control block with nothing done and no stop, not recording (max_iter 0 marks it)
-        gr::BatchCtl &c0 = b->ctl_host[0];
-        std::memset(&c0, 0, sizeof c0);
-        for (int k = 0; k < gr::kBatchLanes; ++k) c0.done[k] = k < (int)b->n_inst ? 0 : 1;
-        GR_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
-        const gr::BcTable t = b->table();
-        const dim3 grid(b->nblocks()), blk(gr::kBlock);
-        if (b->cfg.dtype == GR_F64) {
-            hipLaunchKernelGGL(gr::k_batch_gradlim<double>, grid, blk, 0, m->stream, m->view(), b->args<double>(b->cur), t,
-                               (const double *)b->inst_mach.p, b->cfg.limiter, (const gr::BatchCtl *)b->ctl.p);
-            hipLaunchKernelGGL((gr::k_batch_flux<double, false>), grid, blk, 0, m->stream, m->view(),
-                               b->args<double>(b->cur), t, (const double *)b->inst_mach.p, b->cfg.cfl, b->partials.p,
-                               (const gr::BatchCtl *)b->ctl.p);
-        } else {
-            hipLaunchKernelGGL(gr::k_batch_gradlim<float>, grid, blk, 0, m->stream, m->view(), b->args<float>(b->cur), t,
-                               (const double *)b->inst_mach.p, b->cfg.limiter, (const gr::BatchCtl *)b->ctl.p);
-            hipLaunchKernelGGL((gr::k_batch_flux<float, false>), grid, blk, 0, m->stream, m->view(),
-                               b->args<float>(b->cur), t, (const double *)b->inst_mach.p, b->cfg.cfl, b->partials.p,
-                               (const gr::BatchCtl *)b->ctl.p);
-        }
-        hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream,
-                           b->partials.p, b->nblocks(), (const gr::BatchCtl *)nullptr, b->partials2.p);
-        hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,
-                           b->partials2.p, gr::kReduceBlocks, (gr::BatchCtl *)nullptr, (double *)nullptr, b->out.p);
-        GR_HIP(hipGetLastError());
-        std::vector<double> o(gr::kBatchLanes);
-        b->out.download(o.data(), o.size(), m->stream);
-        GR_HIP(hipStreamSynchronize(m->stream));
-        std::copy(o.begin(), o.begin() + b->n_inst, out);
-        b->ghost_src = b->cur;
+        b->residual_norms(out);
     });
 }
 
@@ -739,6 +928,93 @@ int gr_batch_set_graph_iters(gr_batch *b, int iters) {
         require(iters >= 2 && iters <= 200 && iters % 2 == 0, "iterations per graph must be even, 2..200");
         b->graph_iters = iters;
         b->drop_graphs();
+    });
+}
+
+int gr_batch_warm_rows(gr_batch *b, const double *wall_distance, float *x, float *t) {
+    return guarded([&] {
+        require(b && wall_distance && x && t, "null argument");
+        gr_mesh *m = b->mesh;
+        const uint32_t N = m->h.n_cells;
+        const size_t rows = (size_t)N * b->n_inst;
+        gr::DevArray<double> wd;
+        gr::DevArray<float> X, T;
+        wd.alloc(N);
+        X.alloc(rows * 6);
+        T.alloc(rows * 5);
+        wd.upload(wall_distance, N, m->stream);
+        const gr_warm_norm none{};
+        hipLaunchKernelGGL(gr::k_warm_features<false>, dim3(gr::blocks_for(rows * 6)), dim3(gr::kBlock), 0, m->stream,
+                           m->view(), wd.p, (const double *)b->inst_mach.p, b->n_inst, (size_t)0, (int)rows, none,
+                           (void *)X.p);
+        if (b->cfg.dtype == GR_F64)
+            hipLaunchKernelGGL(gr::k_warm_targets<double>, dim3(gr::blocks_for(rows)), dim3(gr::kBlock), 0, m->stream,
+                               (const double *)b->ubuf[b->cur].p, b->stride, N, b->n_inst, (const double *)b->inst_mach.p,
+                               b->p_inf, b->T_inf, T.p);
+        else
+            hipLaunchKernelGGL(gr::k_warm_targets<float>, dim3(gr::blocks_for(rows)), dim3(gr::kBlock), 0, m->stream,
+                               (const float *)b->ubuf[b->cur].p, b->stride, N, b->n_inst, (const double *)b->inst_mach.p,
+                               b->p_inf, b->T_inf, T.p);
+        GR_HIP(hipGetLastError());
+        X.download(x, rows * 6, m->stream);
+        T.download(t, rows * 5, m->stream);
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+int gr_batch_warm_start(gr_batch *b, gr_mlp *n, const double *wall_distance, const gr_warm_norm *norm,
+                        double max_ratio, double *fs_residual, int *used) {
+    return guarded([&] {
+        require(b && n && wall_distance && norm && fs_residual, "null argument");
+        require(gr::mlp_n_in(n) == 6 && gr::mlp_n_out(n) == 5, "the warm-start network must have n_in 6 and n_out 5");
+        for (int c = 0; c < 6; ++c) require(norm->in_std[c] > 0, "input scales must be positive");
+        for (int c = 0; c < 5; ++c) require(norm->out_std[c] > 0, "output scales must be positive");
+        require(max_ratio > 0, "max_ratio must be positive");
+        require(!b->mach_h.empty(), "far field not set (gr_batch_set_freestream)");
+        gr_mesh *m = b->mesh;
+        const uint32_t N = m->h.n_cells, I = b->n_inst;
+        // cold state and its residuals: the normalisation and the fallback
+        if (b->cfg.dtype == GR_F64) b->fill_freestream<double>();
+        else b->fill_freestream<float>();
+        b->ghost_src = b->cur;
+        b->residual_norms(fs_residual);
+        // prediction of every (cell, instance) row
+        gr::DevArray<double> wd;
+        wd.alloc(N);
+        wd.upload(wall_distance, N, m->stream);
+        const gr_warm_norm nrm = *norm;
+        const double *mach = (const double *)b->inst_mach.p;
+        const auto fill = [&](void *X, size_t row0, int rows) {
+            hipLaunchKernelGGL(gr::k_warm_features<true>, dim3(gr::blocks_for((size_t)rows * gr::kMlpInPad)),
+                               dim3(gr::kBlock), 0, m->stream, m->view(), wd.p, mach, I, row0, rows, nrm, X);
+        };
+        const auto consume = [&](const float *Y, size_t row0, int rows) {
+            if (b->cfg.dtype == GR_F64)
+                hipLaunchKernelGGL(gr::k_warm_apply<double>, dim3(gr::blocks_for(rows)), dim3(gr::kBlock), 0, m->stream,
+                                   Y, row0, rows, I, nrm, mach, b->p_inf, b->T_inf, (double *)b->ubuf[0].p,
+                                   (double *)b->ubuf[1].p, b->stride);
+            else
+                hipLaunchKernelGGL(gr::k_warm_apply<float>, dim3(gr::blocks_for(rows)), dim3(gr::kBlock), 0, m->stream,
+                                   Y, row0, rows, I, nrm, mach, b->p_inf, b->T_inf, (float *)b->ubuf[0].p,
+                                   (float *)b->ubuf[1].p, b->stride);
+        };
+        gr::mlp_run_device(n, (size_t)N * I, m->stream, fill, consume);
+        // the plain solver judges the prediction
+        std::vector<double> pred(I);
+        b->residual_norms(pred.data());
+        std::vector<int> reset(gr::kBatchLanes, 0);
+        bool any = false;
+        for (uint32_t k = 0; k < I; ++k) {
+            const bool keep = std::isfinite(pred[k]) && pred[k] < max_ratio * fs_residual[k];
+            if (used) used[k] = keep ? 1 : 0;
+            reset[k] = keep ? 0 : 1;
+            any = any || !keep;
+        }
+        if (any) {
+            if (b->cfg.dtype == GR_F64) b->reset_lanes<double>(reset.data());
+            else b->reset_lanes<float>(reset.data());
+        }
+        b->ghost_src = b->cur;
     });
 }
 

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 
 #include "gr_internal.hpp"
@@ -100,6 +101,19 @@ struct gr_mesh {
 // at c*stride + cell*lanes + lane (post.hip).  Returns a gr_status.
 int gr_internal_wall_force(gr_mesh *m, const void *u, int dtype, size_t stride, uint32_t lanes, uint32_t lane,
                            uint32_t region, double F[3]);
+
+// Device-side evaluation of the warm-start network (mlp.hip): rows are processed in chunks on
+// stream s; fill(X, row0, m) writes the bfloat16 inputs of rows row0..row0+m-1 into X
+// ([m][kMlpInPad], zero padded), consume(Y, row0, m) reads the binary32 outputs ([m][kMlpOutPad]).
+struct gr_mlp;
+namespace gr {
+constexpr int kMlpInPad = 64;
+constexpr int kMlpOutPad = 32;
+uint32_t mlp_n_in(const gr_mlp *n);
+uint32_t mlp_n_out(const gr_mlp *n);
+void mlp_run_device(gr_mlp *n, size_t M, hipStream_t s, const std::function<void(void *, size_t, int)> &fill,
+                    const std::function<void(const float *, size_t, int)> &consume);
+}  // namespace gr
 
 struct gr_field {
     gr_mesh *mesh = nullptr;

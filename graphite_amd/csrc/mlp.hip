@@ -180,6 +180,19 @@ __global__ void k_pack_input(const float *__restrict__ x, int M, int n_in, bf16 
     X[e] = (bf16)(c < n_in ? x[(size_t)r * n_in + c] : 0.f);
 }
 
+// Mini-batch of a resident training set: row r of the batch is row (a*(pos0+r) + b) mod Mall of
+// the set (a coprime to Mall, so one pass of pos over 0..Mall-1 visits every row once).
+__global__ void k_gather_rows(const float *__restrict__ xall, const float *__restrict__ tall, uint64_t Mall, int n_in,
+                              int n_out, uint64_t a, uint64_t b, uint64_t pos0, int M, bf16 *__restrict__ X,
+                              float *__restrict__ T) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)M * kMlpIn) return;
+    const int r = (int)(e / kMlpIn), c = (int)(e % kMlpIn);
+    const uint64_t src = (a * ((pos0 + r) % Mall) + b) % Mall;
+    X[e] = (bf16)(c < n_in ? xall[src * n_in + c] : 0.f);
+    if (c < n_out) T[(size_t)r * n_out + c] = tall[src * n_out + c];
+}
+
 // Mean squared error: dOut = 2 (out - t) / (M n_out) (BF16 for the GEMMs, FP32 for the bias
 // gradient), padded columns 0; per-block partial sums of the squared error.
 __global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ out, const float *__restrict__ t, int M,
@@ -200,6 +213,15 @@ __global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ out, c
     }
     const double bs = block_sum(se);
     if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// dst = (sum of the n partials, in a fixed order) / denom
+__global__ __launch_bounds__(kBlock) void k_loss_final(const double *__restrict__ partials, int n, double denom,
+                                                       double *__restrict__ dst) {
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) *dst = t / denom;
 }
 
 // dZ = dH * (1 - H^2) for tanh layers (H the layer's BF16 output).
@@ -238,6 +260,8 @@ __global__ void k_pack_weights(const float *__restrict__ W, int Kp, int Np, bf16
     WT[(size_t)n * Kp + k] = v;
 }
 
+constexpr int kMlpChunk = 1 << 18;   // rows per forward chunk of device-side evaluation
+
 }  // namespace gr
 
 using gr::guarded;
@@ -255,8 +279,9 @@ struct gr_mlp {
     int cap = 0;   // rows of the activation buffers
     gr::DevArray<float> xin, tgt, out, d32, dH, part;
     gr::DevArray<gr::bf16> X0, H[3], d16;
-    gr::DevArray<double> lpart;
-    gr::DevArray<double> scal;
+    gr::DevArray<double> lpart, loss;
+    gr::DevArray<float> xall, tall;   // resident training set
+    uint64_t n_rows = 0;
     long step = 0;
 
     ~gr_mlp() {
@@ -280,25 +305,25 @@ struct gr_mlp {
         part.alloc((size_t)chunks * gr::kMlpW * gr::kMlpW);
         lpart.alloc((size_t)cap * gr::kMlpOut / gr::kBlock + 1);
     }
-    void pack() {
+    void pack(hipStream_t s) {
         for (int l = 0; l < 4; ++l)
-            hipLaunchKernelGGL(gr::k_pack_weights, dim3(gr::blocks_for((size_t)Kp[l] * Np[l])), dim3(gr::kBlock), 0,
-                               stream, p.p + woff[l], Kp[l], Np[l], WT[l].p, Wb[l].p);
+            hipLaunchKernelGGL(gr::k_pack_weights, dim3(gr::blocks_for((size_t)Kp[l] * Np[l])), dim3(gr::kBlock), 0, s,
+                               p.p + woff[l], Kp[l], Np[l], WT[l].p, Wb[l].p);
     }
     void upload_x(const float *x, int M) {
         GR_HIP(hipMemcpyAsync(xin.p, x, (size_t)M * n_in * sizeof(float), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(gr::k_pack_input, dim3(gr::blocks_for((size_t)M * gr::kMlpIn)), dim3(gr::kBlock), 0, stream,
                            xin.p, M, (int)n_in, X0.p);
     }
-    void forward(int M) {
+    void forward(int M, hipStream_t s) {
         const dim3 grid((M + gr::kRowsPerBlock - 1) / gr::kRowsPerBlock), blk(gr::kBlock);
-        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, stream, X0.p, gr::kMlpIn, M, Kp[0], WT[0].p,
+        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, s, X0.p, gr::kMlpIn, M, Kp[0], WT[0].p,
                            p.p + boff[0], H[0].p, gr::kMlpW);
-        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, stream, H[0].p, gr::kMlpW, M, Kp[1], WT[1].p,
+        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, s, H[0].p, gr::kMlpW, M, Kp[1], WT[1].p,
                            p.p + boff[1], H[1].p, gr::kMlpW);
-        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, stream, H[1].p, gr::kMlpW, M, Kp[2], WT[2].p,
+        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, s, H[1].p, gr::kMlpW, M, Kp[2], WT[2].p,
                            p.p + boff[2], H[2].p, gr::kMlpW);
-        hipLaunchKernelGGL((gr::k_dense<0, float, 2>), grid, blk, 0, stream, H[2].p, gr::kMlpW, M, Kp[3], WT[3].p,
+        hipLaunchKernelGGL((gr::k_dense<0, float, 2>), grid, blk, 0, s, H[2].p, gr::kMlpW, M, Kp[3], WT[3].p,
                            p.p + boff[3], out.p, gr::kMlpOut);
     }
     // weight and bias gradients of layer l from its input A (BF16, lda) and d16/d32 (ld = Np)
@@ -315,14 +340,14 @@ struct gr_mlp {
         hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(Np[l])), dim3(gr::kBlock), 0, stream, part.p, chunks,
                            Np[l], g.p + boff[l]);
     }
-    // loss and gradients for targets already in tgt
-    double backward(int M) {
+    // loss (into *loss_dst on the device) and gradients for the targets in tgt; no host sync
+    void backward(int M, double *loss_dst) {
         const size_t no = (size_t)M * gr::kMlpOut;
         const unsigned nb = gr::blocks_for(no);
         hipLaunchKernelGGL(gr::k_mse, dim3(nb), dim3(gr::kBlock), 0, stream, out.p, tgt.p, M, (int)n_out, d16.p, d32.p,
                            lpart.p);
-        std::vector<double> lp(nb);
-        GR_HIP(hipMemcpyAsync(lp.data(), lpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+        hipLaunchKernelGGL(gr::k_loss_final, dim3(1), dim3(gr::kBlock), 0, stream, lpart.p, (int)nb,
+                           (double)M * n_out, loss_dst);
         const dim3 grid((M + gr::kRowsPerBlock - 1) / gr::kRowsPerBlock), blk(gr::kBlock);
         // layer 4 (linear): dW4 = H3^T dOut, dH3 = dOut W4^T
         wgrad<2>(3, H[2].p, gr::kMlpW, M);
@@ -340,17 +365,54 @@ struct gr_mlp {
                                    Wb[l].p, (const float *)nullptr, dH.p, gr::kMlpW);
         }
         GR_HIP(hipGetLastError());
+    }
+    void adam(float lr) {
+        step += 1;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+        hipLaunchKernelGGL(gr::k_adam, dim3(gr::blocks_for(n_params)), dim3(gr::kBlock), 0, stream, p.p, m.p, v.p, g.p,
+                           n_params, lr, b1, b2, 1e-8f, c1, c2);
+        pack(stream);
+    }
+    double host_loss() {
+        double l = 0;
+        GR_HIP(hipMemcpyAsync(&l, loss.p, sizeof l, hipMemcpyDeviceToHost, stream));
         GR_HIP(hipStreamSynchronize(stream));
-        double s = 0;
-        for (double x : lp) s += x;
-        return s / ((double)M * n_out);
+        return l;
     }
 };
+
+namespace gr {
+uint32_t mlp_n_in(const gr_mlp *n) { return n->n_in; }
+uint32_t mlp_n_out(const gr_mlp *n) { return n->n_out; }
+
+void mlp_run_device(gr_mlp *n, size_t M, hipStream_t s, const std::function<void(void *, size_t, int)> &fill,
+                    const std::function<void(const float *, size_t, int)> &consume) {
+    GR_HIP(hipStreamSynchronize(n->stream));
+    n->ensure((int)std::min<size_t>(M, kMlpChunk));
+    for (size_t r0 = 0; r0 < M; r0 += kMlpChunk) {
+        const int m = (int)std::min<size_t>(kMlpChunk, M - r0);
+        fill(n->X0.p, r0, m);
+        n->forward(m, s);
+        consume(n->out.p, r0, m);
+        GR_HIP(hipGetLastError());
+    }
+}
+}  // namespace gr
 
 namespace {
 void check_dims(gr_mlp *n, const float *x, int M) {
     require(n && x, "null argument");
     require(M > 0 && M <= (1 << 24), "row count must be in 1..2^24");
+}
+
+uint64_t gcd64(uint64_t a, uint64_t b) {
+    while (b) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
 }
 }  // namespace
 
@@ -398,8 +460,8 @@ int gr_mlp_create(uint32_t n_in, uint32_t n_out, uint64_t seed, gr_mlp **out) {
                 n->WT[l].alloc((size_t)n->Kp[l] * n->Np[l]);
                 n->Wb[l].alloc((size_t)n->Kp[l] * n->Np[l]);
             }
-            n->scal.alloc(1);
-            n->pack();
+            n->loss.alloc(1);
+            n->pack(n->stream);
             GR_HIP(hipStreamSynchronize(n->stream));
         } catch (...) {
             delete n;
@@ -432,7 +494,7 @@ int gr_mlp_set_params(gr_mlp *n, const float *in) {
     return guarded([&] {
         require(n && in, "null argument");
         n->p.upload(in, n->n_params, n->stream);
-        n->pack();
+        n->pack(n->stream);
         GR_HIP(hipStreamSynchronize(n->stream));
     });
 }
@@ -441,13 +503,12 @@ int gr_mlp_forward(gr_mlp *n, const float *x, int M, float *y) {
     return guarded([&] {
         check_dims(n, x, M);
         require(y != nullptr, "null output");
-        const int chunk = 1 << 18;
         std::vector<float> o;
-        for (int r0 = 0; r0 < M; r0 += chunk) {
-            const int m = std::min(chunk, M - r0);
+        for (int r0 = 0; r0 < M; r0 += gr::kMlpChunk) {
+            const int m = std::min(gr::kMlpChunk, M - r0);
             n->ensure(m);
             n->upload_x(x + (size_t)r0 * n->n_in, m);
-            n->forward(m);
+            n->forward(m, n->stream);
             GR_HIP(hipGetLastError());
             o.resize((size_t)m * gr::kMlpOut);
             n->out.download(o.data(), o.size(), n->stream);
@@ -466,8 +527,9 @@ int gr_mlp_loss_grad(gr_mlp *n, const float *x, const float *t, int M, double *l
         n->ensure(M);
         n->upload_x(x, M);
         GR_HIP(hipMemcpyAsync(n->tgt.p, t, (size_t)M * n->n_out * sizeof(float), hipMemcpyHostToDevice, n->stream));
-        n->forward(M);
-        *loss = n->backward(M);
+        n->forward(M, n->stream);
+        n->backward(M, n->loss.p);
+        *loss = n->host_loss();
         if (grad) {
             n->g.download(grad, n->n_params, n->stream);
             GR_HIP(hipStreamSynchronize(n->stream));
@@ -484,15 +546,58 @@ int gr_mlp_train_step(gr_mlp *n, const float *x, const float *t, int M, float lr
         n->ensure(M);
         n->upload_x(x, M);
         GR_HIP(hipMemcpyAsync(n->tgt.p, t, (size_t)M * n->n_out * sizeof(float), hipMemcpyHostToDevice, n->stream));
-        n->forward(M);
-        *loss = n->backward(M);
-        n->step += 1;
-        const float b1 = 0.9f, b2 = 0.999f;
-        const float c1 = 1.f - std::pow(b1, (float)n->step), c2 = 1.f - std::pow(b2, (float)n->step);
-        hipLaunchKernelGGL(gr::k_adam, dim3(gr::blocks_for(n->n_params)), dim3(gr::kBlock), 0, n->stream, n->p.p, n->m.p,
-                           n->v.p, n->g.p, n->n_params, lr, b1, b2, 1e-8f, c1, c2);
-        n->pack();
+        n->forward(M, n->stream);
+        n->backward(M, n->loss.p);
+        n->adam(lr);
         GR_HIP(hipGetLastError());
+        *loss = n->host_loss();
+    });
+}
+
+int gr_mlp_set_data(gr_mlp *n, const float *x, const float *t, uint64_t rows) {
+    return guarded([&] {
+        require(n && x && t, "null argument");
+        require(rows >= 1 && rows <= (1ull << 31), "row count must be in 1..2^31");
+        n->xall.alloc(rows * n->n_in);
+        n->tall.alloc(rows * n->n_out);
+        n->xall.upload(x, rows * n->n_in, n->stream);
+        n->tall.upload(t, rows * n->n_out, n->stream);
+        GR_HIP(hipStreamSynchronize(n->stream));
+        n->n_rows = rows;
+    });
+}
+
+int gr_mlp_train(gr_mlp *n, int batch, int steps, float lr, float lr_final, uint64_t seed, double *losses) {
+    return guarded([&] {
+        require(n != nullptr, "null argument");
+        require(n->n_rows > 0, "no training set (gr_mlp_set_data)");
+        require(batch >= 1 && batch <= (1 << 20) && (uint64_t)batch <= n->n_rows, "batch must be in 1..min(2^20, rows)");
+        require(steps >= 1, "steps must be positive");
+        require(lr > 0 && lr_final > 0, "learning rates must be positive");
+        n->ensure(batch);
+        gr::DevArray<double> ldev;
+        ldev.alloc((size_t)steps);
+        const uint64_t Mall = n->n_rows, per_epoch = Mall / (uint64_t)batch;
+        uint64_t a = 1, b = 0;
+        for (int s = 0; s < steps; ++s) {
+            const uint64_t epoch = (uint64_t)s / per_epoch, within = (uint64_t)s % per_epoch;
+            if (within == 0) {
+                std::mt19937_64 rng(seed * 0x9E3779B97F4A7C15ull + epoch);
+                do {
+                    a = Mall > 1 ? 1 + rng() % (Mall - 1) : 1;
+                } while (gcd64(a, Mall) != 1);
+                b = rng() % Mall;
+            }
+            hipLaunchKernelGGL(gr::k_gather_rows, dim3(gr::blocks_for((size_t)batch * gr::kMlpIn)), dim3(gr::kBlock), 0,
+                               n->stream, n->xall.p, n->tall.p, Mall, (int)n->n_in, (int)n->n_out, a, b,
+                               within * (uint64_t)batch, batch, n->X0.p, n->tgt.p);
+            n->forward(batch, n->stream);
+            n->backward(batch, ldev.p + s);
+            const double cosf_ = 0.5 * (1.0 + std::cos(M_PI * (double)s / (double)steps));
+            n->adam((float)(lr_final + (lr - lr_final) * cosf_));
+        }
+        GR_HIP(hipGetLastError());
+        if (losses) ldev.download(losses, (size_t)steps, n->stream);
         GR_HIP(hipStreamSynchronize(n->stream));
     });
 }

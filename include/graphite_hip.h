@@ -335,6 +335,38 @@ int gr_mlp_loss_grad(gr_mlp *n, const float *x, const float *t, int M, double *l
 /* One Adam step (beta 0.9/0.999, eps 1e-8) on the mean squared error of one mini-batch;
  * loss receives the loss before the step.  Deterministic: the same calls give the same bits. */
 int gr_mlp_train_step(gr_mlp *n, const float *x, const float *t, int M, float lr, double *loss);
+/* Keeps a training set resident on the device (x [rows][n_in], t [rows][n_out]). */
+int gr_mlp_set_data(gr_mlp *n, const float *x, const float *t, uint64_t rows);
+/* `steps` Adam steps on mini-batches of `batch` rows of the resident set: each pass over the set
+ * visits its rows in a seeded permuted order; the learning rate decays from lr to lr_final on a
+ * cosine.  losses (steps entries, may be NULL) receives each step's mini-batch loss. */
+int gr_mlp_train(gr_mlp *n, int batch, int steps, float lr, float lr_final, uint64_t seed, double *losses);
+
+/* ---- learned warm start of the batched solver (no reference counterpart) --------------------
+ * Network rows are (cell, instance) pairs, row = cell * n_inst + instance, over interior cells.
+ * Raw inputs (n_in = 6): cell centre x, y, z, log(d + 1e-3) with d the wall distance, the
+ * instance's Mach number |M| and angle of attack atan2(M_y, M_x) in radians.  Raw targets
+ * (n_out = 5): rho/rho_inf - 1, (rho u - rho_inf u_inf)/(rho_inf c_inf) (3 entries) and
+ * (rho E - rho_inf E_inf)/(rho_inf c_inf^2), with the far-field state of the instance. */
+typedef struct gr_warm_norm {
+    float in_mean[6], in_std[6];     /* network input = (raw - mean) / std */
+    float out_mean[5], out_std[5];   /* raw target = network output * std + mean */
+} gr_warm_norm;
+
+/* All instances' states at once: rho/rhoe [n_inst][N+G], rhou [n_inst][N+G][3]. */
+int gr_batch_set_states(gr_batch *b, const double *rho, const double *rhou, const double *rhoe);
+int gr_batch_get_states(gr_batch *b, double *rho, double *rhou, double *rhoe);
+/* Raw inputs x [n_inst*N][6] and targets t [n_inst*N][5] of the batch's current states
+ * (a training set); wall_distance has n_cells entries. */
+int gr_batch_warm_rows(gr_batch *b, const double *wall_distance, float *x, float *t);
+/* Resets every instance to its freestream (gr_batch_set_freestream must have run), records
+ * the freestream residuals (fs_residual, n_inst entries: the normalisation of the solve that
+ * follows), replaces the states by the network's prediction and then judges each prediction
+ * by its residual: an instance whose predicted residual is not finite or not below
+ * max_ratio * its freestream residual goes back to freestream.  used[k] (may be NULL) = 1 where
+ * the prediction was kept.  The network must have n_in = 6 and n_out = 5. */
+int gr_batch_warm_start(gr_batch *b, gr_mlp *n, const double *wall_distance, const gr_warm_norm *norm,
+                        double max_ratio, double *fs_residual, int *used);
 
 #ifdef __cplusplus
 }

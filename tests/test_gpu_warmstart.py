@@ -1,0 +1,139 @@
+"""Learned warm start of the batched solver: feature/target rows, the device-side prediction
+path, the residual judgement with its freestream fallback, and the gain of a trained network."""
+import numpy as np
+import pytest
+
+from test_mlp_oracle import offsets
+
+pytestmark = pytest.mark.gpu
+
+RHO_INF, C_INF = 1.4, 1.0          # p = 1, T = 1/1.4
+
+
+@pytest.fixture(scope="module")
+def small(gr):
+    m = gr.Mesh.build(gr.Elements.cgrid_naca(n_i=128, n_j=32, n_wake=24, radius=20.0, first_height=3e-3))
+    m.renumber_rcm()
+    m.compute_metrics()
+    return m
+
+
+def machs(n, seed):
+    from graphite_amd import warmstart as ws
+    return ws.sample_instances(n, seed)
+
+
+def constant_net(gr, c):
+    """Network whose output is the constant vector c (zero weights, last-layer bias c)."""
+    net = gr.Mlp(6, 5, seed=0)
+    p = np.zeros(net.n_params, np.float32)
+    _, bo, _ = offsets()
+    p[bo[3]:bo[3] + 5] = c
+    net.set_params(p)
+    return net
+
+
+def test_rows_at_freestream(gr, small):
+    from graphite_amd import warmstart as ws
+    m = small
+    M = machs(3, seed=5)
+    b = ws.make_batch(m, M)
+    wd = ws.wall_distance(m)
+    x, t = b.warm_rows(wd)
+    N = m.n_cells
+    assert x.shape == (3 * N, 6) and t.shape == (3 * N, 5)
+    cc = m.metrics()["cell_centers"][:N]
+    for k in range(3):
+        xs = x[k::3]                                     # row = cell * n_inst + instance
+        assert np.array_equal(xs[:, :3], cc.astype(np.float32))
+        assert np.array_equal(xs[:, 3], np.log(wd + 1e-3).astype(np.float32))
+        assert np.allclose(xs[:, 4], np.linalg.norm(M[k]), rtol=1e-7)
+        assert np.allclose(xs[:, 5], np.arctan2(M[k, 1], M[k, 0]), rtol=1e-6, atol=1e-7)
+    assert not t.any()                                   # freestream: every deviation is zero
+
+
+def test_prediction_is_applied(gr, small):
+    """A constant network output c gives t = c*std + mean, hence a known conservative state."""
+    from graphite_amd import warmstart as ws
+    m = small
+    M = machs(2, seed=6)
+    c = np.array([0.5, -0.25, 0.125, 0.0, 1.0], np.float32)
+    norm = gr.WarmNorm()
+    for j in range(6):
+        norm.in_mean[j], norm.in_std[j] = 0.0, 1.0
+    tm, ts = [0.01, 0.02, -0.03, 0.0, 0.05], [0.1, 0.2, 0.3, 1.0, 0.4]
+    for j in range(5):
+        norm.out_mean[j], norm.out_std[j] = tm[j], ts[j]
+    b = ws.make_batch(m, M)
+    fs, used = b.warm_start(constant_net(gr, c), ws.wall_distance(m), norm, max_ratio=1e300)
+    assert used.all() and np.all(fs > 0)
+    r, u, e = b.get_states()
+    N = m.n_cells
+    tt = c.astype(np.float64) * np.array(ts, np.float32) + np.array(tm, np.float32)
+    for k in range(2):
+        vel = M[k] * C_INF
+        assert np.allclose(r[k, :N], RHO_INF * (1 + tt[0]), rtol=1e-14)
+        for d in range(3):
+            assert np.allclose(u[k, :N, d], RHO_INF * vel[d] + RHO_INF * C_INF * tt[1 + d], rtol=1e-14, atol=1e-15)
+        E = 1.0 / 0.4 + 0.5 * RHO_INF * vel @ vel
+        assert np.allclose(e[k, :N], E + RHO_INF * C_INF ** 2 * tt[4], rtol=1e-14)
+
+
+def test_bad_prediction_falls_back_to_freestream(gr, small):
+    """A network that outputs NaN is rejected by the residual test: every instance restarts from
+    freestream, bit for bit the cold state."""
+    from graphite_amd import warmstart as ws
+    m = small
+    M = machs(4, seed=7)
+    net = gr.Mlp(6, 5, seed=0)
+    p = net.params()
+    p[:] = np.nan
+    net.set_params(p)
+    norm = gr.WarmNorm()
+    for j in range(6):
+        norm.in_mean[j], norm.in_std[j] = 0.0, 1.0
+    for j in range(5):
+        norm.out_mean[j], norm.out_std[j] = 0.0, 1.0
+    warm = ws.make_batch(m, M)
+    fs, used = warm.warm_start(net, ws.wall_distance(m), norm)
+    assert not used.any()
+    cold = ws.make_batch(m, M)
+    assert np.array_equal(fs, cold.residual_norms())
+    for x, y in zip(warm.get_states(), cold.get_states()):
+        assert np.array_equal(x, y)
+    # and the fallen-back instances then solve exactly like cold ones
+    a = warm.iterate(20)
+    b = cold.iterate(20)
+    assert np.array_equal(a[0], b[0])
+    for x, y in zip(warm.get_states(), cold.get_states()):
+        assert np.array_equal(x, y)
+
+
+def test_nan_state_reports_divergence(gr, small):
+    from graphite_amd import warmstart as ws
+    m = small
+    b = ws.make_batch(m, machs(2, seed=8))
+    r, u, e = b.get_states()
+    r[1, 10] = np.nan
+    b.set_states(r, u, e)
+    with pytest.raises(gr.GraphiteError) as ei:
+        b.iterate(5)
+    assert ei.value.code == gr._capi.GR_ERR_DIVERGED
+
+
+def test_bulk_states_round_trip(gr, small):
+    from graphite_amd import warmstart as ws
+    m = small
+    b = ws.make_batch(m, machs(3, seed=9))
+    b.iterate(7)
+    bulk = b.get_states()
+    for k in range(3):
+        for x, y in zip(b.get_state(k), bulk):
+            assert np.array_equal(x, y[k])
+    rng = np.random.default_rng(0)
+    new = [a * (1 + 0.01 * rng.standard_normal(a.shape)) for a in bulk]
+    b.set_states(*new)
+    got = b.get_states()
+    N = m.n_cells
+    for x, y in zip(got, new):
+        assert np.array_equal(x[:, :N], y[:, :N])
