@@ -16,7 +16,7 @@ import numpy as np
 
 from ._capi import (GR_BC_FARFIELD, GR_BC_SLIP, GR_F32, GR_F64, GR_HEX, GR_LIM_BARTH_JESPERSEN,
                     GR_LIM_MICHALAK, GR_LIM_VENKATAKRISHNAN, GR_OK, GR_PYRAMID, GR_QUAD, GR_TET,
-                    GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, WarmNorm, load)
+                    GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, WarmNorm, WingParams, load)
 
 __all__ = ["GraphiteError", "Elements", "Mesh", "Field", "Solver", "device_count", "load",
            "euler_init", "euler_bc_slip", "euler_bc_farfield", "euler_step_prepare", "euler_step",
@@ -103,6 +103,17 @@ class Elements:
     def cgrid_naca(n_i=384, n_j=96, n_wake=64, thickness=0.12, radius=20.0, first_height=1e-3, dz=0.1):
         h = C.c_void_p()
         _check(load().gr_gen_cgrid_naca(n_i, n_j, n_wake, thickness, radius, first_height, dz, C.byref(h)))
+        return Elements(h)
+
+    @staticmethod
+    def wing(n_around=160, n_prism=20, n_tet=23, n_span=24, n_cap=14, thickness=0.12, sweep_deg=30.0,
+             taper=0.56, semispan=1.5, first_height=1e-3, growth=1.2, radius=25.0):
+        """Swept NACA wing, prism layers + tetrahedra, half-sphere far field (about 1.1 M cells
+        with the defaults); markers wall, farfield, symmetry."""
+        prm = WingParams(n_around, n_prism, n_tet, n_span, n_cap, thickness, sweep_deg, taper, semispan,
+                         first_height, growth, radius)
+        h = C.c_void_p()
+        _check(load().gr_gen_wing(C.byref(prm), C.byref(h)))
         return Elements(h)
 
     @staticmethod
@@ -483,6 +494,24 @@ class Batch:
         _check(load().gr_batch_iterate(self._h, max_iter, tol, _p(ref, C.c_double) if ref is not None else None,
                                        its, conv, C.byref(launched)))
         return np.array(its[:], dtype=int), np.array(conv[:], dtype=bool), launched.value
+
+    def iterate_masked(self, max_iter: int, tol: float = 0.0, reference=None, active=None):
+        """Iterate the active instances only; returns (iterations, converged, diverged, batch
+        iterations run) with divergence reported per instance instead of raised."""
+        n = self.n_inst
+        its, conv, div = (C.c_int * n)(), (C.c_int * n)(), (C.c_int * n)()
+        launched = C.c_int()
+        ref = None if reference is None else _f64(reference)
+        act = None if active is None else (C.c_int * n)(*[int(bool(a)) for a in active])
+        _check(load().gr_batch_iterate_masked(self._h, max_iter, tol,
+                                              _p(ref, C.c_double) if ref is not None else None, act, its, conv,
+                                              div, C.byref(launched)))
+        return (np.array(its[:], dtype=int), np.array(conv[:], dtype=bool), np.array(div[:], dtype=bool),
+                launched.value)
+
+    def reset_freestream(self, mask) -> None:
+        m = (C.c_int * self.n_inst)(*[int(bool(a)) for a in mask])
+        _check(load().gr_batch_reset_freestream(self._h, m))
 
     def history(self, inst: int, n: int) -> np.ndarray:
         h = np.zeros(max(n, 1))

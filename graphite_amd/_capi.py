@@ -41,6 +41,12 @@ class SolveResult(C.Structure):
     _fields_ = [("iterations", _i), ("converged", _i), ("residual", _d), ("ratio", _d), ("reference", _d)]
 
 
+class WingParams(C.Structure):
+    _fields_ = [("n_around", _u32), ("n_prism", _u32), ("n_tet", _u32), ("n_span", _u32), ("n_cap", _u32),
+                ("thickness", _d), ("sweep_deg", _d), ("taper", _d), ("semispan", _d),
+                ("first_height", _d), ("growth", _d), ("radius", _d)]
+
+
 class WarmNorm(C.Structure):
     _fields_ = [("in_mean", C.c_float * 6), ("in_std", C.c_float * 6),
                 ("out_mean", C.c_float * 5), ("out_std", C.c_float * 5)]
@@ -61,6 +67,7 @@ SIGNATURES = {
     "gr_gen_box": (_i, [_u32, _u32, _u32, _d, _d, _d, _i, _d, C.c_uint64, C.POINTER(_vp)]),
     "gr_gen_ogrid": (_i, [_u32, _u32, _u32, _d, _d, _d, _d, C.POINTER(_vp)]),
     "gr_gen_cgrid_naca": (_i, [_u32, _u32, _u32, _d, _d, _d, _d, C.POINTER(_vp)]),
+    "gr_gen_wing": (_i, [C.POINTER(WingParams), C.POINTER(_vp)]),
     "gr_mesh_build": (_i, [_vp, C.POINTER(_vp)]),
     "gr_mesh_from_su2": (_i, [_s, _s, _i]),
     "gr_mesh_read_fvm": (_i, [_s, _i, C.POINTER(_vp)]),
@@ -114,6 +121,9 @@ SIGNATURES = {
     "gr_batch_set_state": (_i, [_vp, _u32, _dp, _dp, _dp]),
     "gr_batch_get_state": (_i, [_vp, _u32, _dp, _dp, _dp]),
     "gr_batch_iterate": (_i, [_vp, _i, _d, _dp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "gr_batch_iterate_masked": (_i, [_vp, _i, _d, _dp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
+                                     C.POINTER(_i)]),
+    "gr_batch_reset_freestream": (_i, [_vp, C.POINTER(_i)]),
     "gr_batch_history": (_i, [_vp, _u32, _dp, _i]),
     "gr_batch_residual_norms": (_i, [_vp, _dp]),
     "gr_batch_profile": (_i, [_vp, _i, _dp]),

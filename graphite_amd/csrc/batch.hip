@@ -389,11 +389,24 @@ __global__ void k_warm_apply(const float *__restrict__ Y, size_t row0, int rows,
         vel[d] = inst_mach[3 * k + d] * ff.c;
         ke += vel[d] * vel[d];
     }
-    S v[5];
-    v[0] = (S)(ff.rho * (1.0 + t[0]));
+    double w[5];
+    w[0] = ff.rho * (1.0 + t[0]);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) v[1 + d] = (S)(ff.rho * vel[d] + ff.rho * ff.c * t[1 + d]);
-    v[4] = (S)(ff.E + 0.5 * ff.rho * ke + ff.rho * ff.c * ff.c * t[4]);
+    for (int d = 0; d < 3; ++d) w[1 + d] = ff.rho * vel[d] + ff.rho * ff.c * t[1 + d];
+    w[4] = ff.E + 0.5 * ff.rho * ke + ff.rho * ff.c * ff.c * t[4];
+    // a cell whose prediction is not a physical state (density or pressure below a quarter of
+    // the far field's, or not finite) keeps the freestream
+    const double pr = 0.4 * (w[4] - 0.5 * ((w[1] * w[1] + w[2] * w[2]) + w[3] * w[3]) / w[0]);
+    const double p_inf_ = 0.4 * ff.E;
+    if (!(w[0] > 0.25 * ff.rho) || !(pr > 0.25 * p_inf_) || !isfinite(w[1] + w[2] + w[3] + w[4])) {
+        w[0] = ff.rho;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) w[1 + d] = vel[d] * ff.rho;
+        w[4] = ff.E + (0.5 * ff.rho) * ke;
+    }
+    S v[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) v[c] = (S)w[c];
     const size_t o = (size_t)i * kBatchLanes + k;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
@@ -571,6 +584,73 @@ struct gr_batch {
             GR_HIP(hipMemcpyAsync(ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice, mesh->stream));
         GR_HIP(hipStreamSynchronize(mesh->stream));
         ghost_src = cur;
+    }
+    // Up to max_iter iterations of the active instances (all when active is null); inactive
+    // instances keep their state.  Per-instance results go to the optional output arrays.
+    void iterate(int max_iter, double tol, const double *reference, const int *active, int *iterations,
+                 int *converged, int *diverged, int *launched) {
+        gr_mesh *m = mesh;
+        gr::BatchCtl &c0 = ctl_host[0];
+        std::memset(&c0, 0, sizeof c0);
+        c0.tol = tol;
+        c0.max_iter = max_iter;
+        c0.n_inst = (int)n_inst;
+        c0.hist_cap = hist_cap;
+        bool any = false;
+        for (int k = 0; k < gr::kBatchLanes; ++k) {
+            const bool on = k < (int)n_inst && (!active || active[k]);
+            c0.done[k] = on ? 0 : 1;
+            c0.ref[k] = (reference && k < (int)n_inst) ? reference[k] : 0.0;
+            any = any || on;
+        }
+        if (!any) {
+            for (uint32_t k = 0; k < n_inst; ++k) {
+                if (iterations) iterations[k] = 0;
+                if (converged) converged[k] = 0;
+                if (diverged) diverged[k] = 0;
+            }
+            if (launched) *launched = 0;
+            return;
+        }
+        GR_HIP(hipMemcpyAsync(ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
+        const int K = graph_iters;
+        const int launches = (max_iter + K - 1) / K;
+        int L = 0;
+        for (; L < launches; ++L) {
+            if (cfg.use_graph) {
+                GR_HIP(hipGraphLaunch(graph(cur), m->stream));
+            } else {
+                for (int k = 0; k < K; ++k) enqueue((cur + k) % 2, ctl.p);
+                GR_HIP(hipGetLastError());
+            }
+            gr::BatchCtl *poll = &ctl_host[1 + (L & 1)];
+            GR_HIP(hipMemcpyAsync(poll, ctl.p, sizeof *poll, hipMemcpyDeviceToHost, m->stream));
+            GR_HIP(hipEventRecord(ev[L & 1], m->stream));
+            if (L >= 1) {
+                GR_HIP(hipEventSynchronize(ev[(L - 1) & 1]));
+                if (ctl_host[1 + ((L - 1) & 1)].stop) {
+                    ++L;
+                    break;
+                }
+            }
+        }
+        GR_HIP(hipStreamSynchronize(m->stream));
+        gr::BatchCtl fin;
+        GR_HIP(hipMemcpy(&fin, ctl.p, sizeof fin, hipMemcpyDeviceToHost));
+        // buffer parity follows the longest-running instance; frozen lanes were copied along
+        int ran = 0;
+        for (uint32_t k = 0; k < n_inst; ++k) ran = std::max(ran, fin.iter[k]);
+        if (ran > 0) {
+            ghost_src = (cur + ran - 1) % 2;
+            cur = (cur + ran) % 2;
+        }
+        last_iters = std::min(ran, hist_cap);
+        for (uint32_t k = 0; k < n_inst; ++k) {
+            if (iterations) iterations[k] = fin.iter[k];
+            if (converged) converged[k] = fin.converged[k];
+            if (diverged) diverged[k] = fin.diverged[k];
+        }
+        if (launched) *launched = std::max(ran, 0);
     }
     // Residual sum of each instance's current state (n_inst entries), state unchanged.
     void residual_norms(double *res) {
@@ -790,59 +870,19 @@ int gr_batch_iterate(gr_batch *b, int max_iter, double tol, const double *refere
     return guarded([&] {
         require(b != nullptr, "null argument");
         require(max_iter > 0, "max_iter must be positive");
-        gr_mesh *m = b->mesh;
-        gr::BatchCtl &c0 = b->ctl_host[0];
-        std::memset(&c0, 0, sizeof c0);
-        c0.tol = tol;
-        c0.max_iter = max_iter;
-        c0.n_inst = (int)b->n_inst;
-        c0.hist_cap = b->hist_cap;
-        for (int k = 0; k < gr::kBatchLanes; ++k) {
-            c0.done[k] = k < (int)b->n_inst ? 0 : 1;
-            c0.ref[k] = (reference && k < (int)b->n_inst) ? reference[k] : 0.0;
-        }
-        GR_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
-        const int K = b->graph_iters;
-        const int launches = (max_iter + K - 1) / K;
-        int L = 0;
-        for (; L < launches; ++L) {
-            if (b->cfg.use_graph) {
-                GR_HIP(hipGraphLaunch(b->graph(b->cur), m->stream));
-            } else {
-                for (int k = 0; k < K; ++k) b->enqueue((b->cur + k) % 2, b->ctl.p);
-                GR_HIP(hipGetLastError());
-            }
-            gr::BatchCtl *poll = &b->ctl_host[1 + (L & 1)];
-            GR_HIP(hipMemcpyAsync(poll, b->ctl.p, sizeof *poll, hipMemcpyDeviceToHost, m->stream));
-            GR_HIP(hipEventRecord(b->ev[L & 1], m->stream));
-            if (L >= 1) {
-                GR_HIP(hipEventSynchronize(b->ev[(L - 1) & 1]));
-                if (b->ctl_host[1 + ((L - 1) & 1)].stop) {
-                    ++L;
-                    break;
-                }
-            }
-        }
-        GR_HIP(hipStreamSynchronize(m->stream));
-        gr::BatchCtl fin;
-        GR_HIP(hipMemcpy(&fin, b->ctl.p, sizeof fin, hipMemcpyDeviceToHost));
-        // iterations launched for the batch: frozen instances still occupy their lanes
-        const int total = std::min(L * K, max_iter);
-        int ran = 0;
-        for (uint32_t k = 0; k < b->n_inst; ++k) ran = std::max(ran, fin.iter[k]);
-        if (ran > 0) {
-            b->ghost_src = (b->cur + ran - 1) % 2;
-            b->cur = (b->cur + ran) % 2;
-        }
-        b->last_iters = std::min(ran, b->hist_cap);
-        for (uint32_t k = 0; k < b->n_inst; ++k) {
-            if (iterations) iterations[k] = fin.iter[k];
-            if (converged) converged[k] = fin.converged[k];
-        }
-        if (launched) *launched = std::max(ran, 0);
-        (void)total;
+        std::vector<int> div(b->n_inst, 0);
+        b->iterate(max_iter, tol, reference, nullptr, iterations, converged, div.data(), launched);
         for (uint32_t k = 0; k < b->n_inst; ++k)
-            if (fin.diverged[k]) throw gr::Error(GR_ERR_DIVERGED, "instance " + std::to_string(k) + " diverged");
+            if (div[k]) throw gr::Error(GR_ERR_DIVERGED, "instance " + std::to_string(k) + " diverged");
+    });
+}
+
+int gr_batch_iterate_masked(gr_batch *b, int max_iter, double tol, const double *reference, const int *active,
+                            int *iterations, int *converged, int *diverged, int *launched) {
+    return guarded([&] {
+        require(b != nullptr, "null argument");
+        require(max_iter > 0, "max_iter must be positive");
+        b->iterate(max_iter, tol, reference, active, iterations, converged, diverged, launched);
     });
 }
 
@@ -1014,6 +1054,18 @@ int gr_batch_warm_start(gr_batch *b, gr_mlp *n, const double *wall_distance, con
             if (b->cfg.dtype == GR_F64) b->reset_lanes<double>(reset.data());
             else b->reset_lanes<float>(reset.data());
         }
+        b->ghost_src = b->cur;
+    });
+}
+
+int gr_batch_reset_freestream(gr_batch *b, const int *mask) {
+    return guarded([&] {
+        require(b && mask, "null argument");
+        require(!b->mach_h.empty(), "far field not set (gr_batch_set_freestream)");
+        std::vector<int> flags(gr::kBatchLanes, 0);
+        for (uint32_t k = 0; k < b->n_inst; ++k) flags[k] = mask[k] ? 1 : 0;
+        if (b->cfg.dtype == GR_F64) b->reset_lanes<double>(flags.data());
+        else b->reset_lanes<float>(flags.data());
         b->ghost_src = b->cur;
     });
 }

@@ -381,6 +381,303 @@ void gen_cgrid(uint32_t ni, uint32_t nj, uint32_t nwake, double thick, double ra
     });
 }
 
+// ---- swept wing in a half-sphere: prism layers + tetrahedra ----------------------------------
+//
+// Structured O-grid topology (i around the section, j outward, k along the span), z spanwise
+// from the symmetry plane z = 0.  Stations k <= n_span carry the tapered, swept NACA section;
+// above the tip the section's interior is filled with centroid-split cells up to the far
+// field.  Cells: hexes j < n_prism split into two prisms along the constant-j face diagonal
+// (i,k)-(i+1,k+1); hexes j >= n_prism split into six tetrahedra (Kuhn, main diagonal
+// (i,j,k)-(i+1,j+1,k+1), which cuts every constant-j face along the same diagonal); fill cells
+// above the tip split into one tetrahedron per face triangle and a centroid node.
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+
+V3 sub(const V3 &a, const V3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+double det3(const V3 &u, const V3 &v, const V3 &w) {
+    return u.x * (v.y * w.z - v.z * w.y) - u.y * (v.x * w.z - v.z * w.x) + u.z * (v.x * w.y - v.y * w.x);
+}
+
+}  // namespace
+
+void gen_wing(const gr_wing_params &p, gr_elements &e) {
+    const uint32_t ni = p.n_around, np = p.n_prism, nt = p.n_tet, ns = p.n_span, nc = p.n_cap;
+    require(ni >= 8 && ni % 2 == 0, "wing: n_around must be even and >= 8");
+    require(np >= 1 && nt >= 1 && ns >= 2 && nc >= 2, "wing: need n_prism, n_tet >= 1 and n_span, n_cap >= 2");
+    require(p.thickness > 0 && p.thickness < 0.5 && p.taper > 0 && p.taper <= 1 && p.semispan > 0,
+            "wing: bad thickness, taper or semispan");
+    require(p.sweep_deg > -60 && p.sweep_deg < 60, "wing: sweep must be within +-60 degrees");
+    require(p.first_height > 0 && p.growth >= 1 && p.radius > 4 * (p.semispan + 1), "wing: bad layers or radius");
+    const uint32_t nj = np + nt, K = ns + nc;
+    const uint64_t n_grid = (uint64_t)ni * (nj + 1) * (K + 1);
+    require(n_grid * 8 < 0xFFFFFFF0ull, "wing: mesh too large for 32-bit indices");
+    const double pi = 3.14159265358979323846;
+    const double b = p.semispan, R = p.radius, tsw = std::tan(p.sweep_deg * pi / 180.0);
+    const double cx0 = 0.5, cy0 = 0.0;   // axis of the half-sphere (root mid-chord)
+    // prism layer offsets
+    std::vector<double> d(np + 1, 0.0);
+    {
+        double h = p.first_height;
+        for (uint32_t j = 1; j <= np; ++j) {
+            d[j] = d[j - 1] + h;
+            h *= p.growth;
+        }
+    }
+    const double h_tet = p.first_height * std::pow(p.growth, (double)np);
+    // unit section, ring order TE -> lower -> LE -> upper
+    std::vector<double> ux(ni), uy(ni), th(ni);
+    for (uint32_t i = 0; i < ni; ++i) {
+        // cosine spacing relaxed at the leading and trailing edges (beta + sin(2 beta)/4)
+        const double beta0 = 2 * pi * i / ni, beta = beta0 + 0.25 * std::sin(2 * beta0);
+        const double X = 0.5 * (1 + std::cos(beta));
+        const double t = naca_half_thickness(std::min(1.0, std::max(0.0, X)), p.thickness);
+        ux[i] = X;
+        uy[i] = (i == 0 || 2 * i == ni) ? 0.0 : (2 * i < ni ? -t : t);
+        th[i] = -beta0;
+    }
+    auto nid = [&](uint32_t i, uint32_t j, uint32_t k) { return (i % ni) + ni * (j + (nj + 1) * k); };
+    std::vector<V3> X(n_grid);
+    // wing stations
+    const double phi_b = std::asin(b / R), phi_top = 0.25 * pi;
+    for (uint32_t k = 0; k <= ns; ++k) {
+        const double z = b * std::sin(0.5 * pi * k / ns);
+        const double c = 1.0 - (1.0 - p.taper) * z / b, xle = z * tsw;
+        const double rho = std::sqrt(R * R - z * z);
+        std::vector<V3> P(ni);
+        for (uint32_t i = 0; i < ni; ++i) P[i] = {xle + c * ux[i], c * uy[i], z};
+        // extrusion directions: section normals smoothed along the ring ([1 2 1] passes), so
+        // the layers fan out gradually around the sharp trailing edge
+        std::vector<double> nxs(ni), nys(ni);
+        for (uint32_t i = 0; i < ni; ++i) {
+            const V3 &a = P[(i + 1) % ni], &bb = P[(i + ni - 1) % ni];
+            const double tx = a.x - bb.x, ty = a.y - bb.y, tl = std::sqrt(tx * tx + ty * ty);
+            nxs[i] = -ty / tl;
+            nys[i] = tx / tl;
+        }
+        for (int pass = 0; pass < 8; ++pass) {
+            std::vector<double> ax(ni), ay(ni);
+            for (uint32_t i = 0; i < ni; ++i) {
+                const uint32_t l = (i + ni - 1) % ni, r = (i + 1) % ni;
+                ax[i] = nxs[l] + 2 * nxs[i] + nxs[r];
+                ay[i] = nys[l] + 2 * nys[i] + nys[r];
+                const double al = std::sqrt(ax[i] * ax[i] + ay[i] * ay[i]);
+                ax[i] /= al;
+                ay[i] /= al;
+            }
+            nxs.swap(ax);
+            nys.swap(ay);
+        }
+        for (uint32_t i = 0; i < ni; ++i) {
+            const double nx = nxs[i], ny = nys[i];
+            for (uint32_t j = 0; j <= np; ++j) X[nid(i, j, k)] = {P[i].x + d[j] * nx, P[i].y + d[j] * ny, z};
+            const V3 Pp = X[nid(i, np, k)];
+            const V3 Q = {cx0 + rho * std::cos(th[i]), cy0 + rho * std::sin(th[i]), z};
+            const double L = std::sqrt((Q.x - Pp.x) * (Q.x - Pp.x) + (Q.y - Pp.y) * (Q.y - Pp.y));
+            require(L > 2 * h_tet, "wing: far field too close to the prism layers");
+            const double r = stretch_ratio(h_tet / L, nt);
+            double acc[64 + 1];
+            require(nt <= 64, "wing: at most 64 tetrahedral layers");
+            {
+                double h = h_tet / L;
+                acc[0] = 0;
+                for (uint32_t j = 1; j <= nt; ++j) {
+                    acc[j] = acc[j - 1] + h;
+                    h *= r;
+                }
+            }
+            for (uint32_t j = 1; j <= nt; ++j) {
+                const double s = j == nt ? 1.0 : acc[j] / acc[nt];
+                X[nid(i, np + j, k)] = {Pp.x + s * (Q.x - Pp.x), Pp.y + s * (Q.y - Pp.y), z};
+            }
+        }
+    }
+    // Stations above the tip blend from the tip-station layout to a top layout on the dome.
+    // Top layout: the tip section scaled by kCapScale about its mid-chord (so the fill cells
+    // above the tip grow with height), rings spread from it to the circle of latitude phi_top
+    // with a first spacing kCapFirst, every node lifted onto the sphere.  Outer-ring nodes move
+    // along the sphere at every station.
+    {
+        const double kCapScale = 3.0, kCapFirst = 0.25;
+        const double c_tip = p.taper, xm = b * tsw + 0.5 * c_tip;
+        const double rho_top = R * std::cos(phi_top);
+        std::vector<V3> T((size_t)ni * (nj + 1));
+        for (uint32_t i = 0; i < ni; ++i) {
+            const V3 B0 = X[nid(i, 0, ns)];
+            const double cxp = xm + kCapScale * (B0.x - xm), cyp = kCapScale * B0.y;
+            const double ox = cx0 + rho_top * std::cos(th[i]), oy = cy0 + rho_top * std::sin(th[i]);
+            const double L = std::sqrt((ox - cxp) * (ox - cxp) + (oy - cyp) * (oy - cyp));
+            require(L > 4 * kCapFirst, "wing: far field too close to the tip");
+            const double r = stretch_ratio(kCapFirst / L, nj);
+            std::vector<double> acc(nj + 1, 0.0);
+            double h = kCapFirst / L;
+            for (uint32_t j = 1; j <= nj; ++j) {
+                acc[j] = acc[j - 1] + h;
+                h *= r;
+            }
+            for (uint32_t j = 0; j <= nj; ++j) {
+                const double s = j == nj ? 1.0 : acc[j] / acc[nj];
+                const double x = cxp + s * (ox - cxp), y = cyp + s * (oy - cyp);
+                const double rr = (x - cx0) * (x - cx0) + (y - cy0) * (y - cy0);
+                T[(size_t)j * ni + i] = {x, y, j == nj ? R * std::sin(phi_top) : std::sqrt(std::max(0.0, R * R - rr))};
+            }
+        }
+        const double dz_tip = b * (1.0 - std::cos(0.5 * pi / ns));
+        const double H = R - b;
+        const double r = stretch_ratio(dz_tip / H, nc);
+        std::vector<double> sq(nc + 1, 0.0);
+        {
+            double acc = 0, h = dz_tip / H;
+            for (uint32_t q = 1; q <= nc; ++q) {
+                acc += h;
+                h *= r;
+                sq[q] = acc;
+            }
+            for (uint32_t q = 1; q <= nc; ++q) sq[q] /= sq[nc];
+        }
+        for (uint32_t j = 0; j <= nj; ++j)
+            for (uint32_t i = 0; i < ni; ++i) {
+                const V3 B = X[nid(i, j, ns)], D = T[(size_t)j * ni + i];
+                for (uint32_t q = 1; q <= nc; ++q) {
+                    const double s = sq[q];
+                    V3 Y;
+                    if (j == nj) {
+                        const double phi = phi_b + s * (phi_top - phi_b);
+                        Y = {cx0 + R * std::cos(phi) * std::cos(th[i]), cy0 + R * std::cos(phi) * std::sin(th[i]),
+                             R * std::sin(phi)};
+                    } else {
+                        Y = {B.x + s * (D.x - B.x), B.y + s * (D.y - B.y), B.z + s * (D.z - B.z)};
+                    }
+                    X[nid(i, j, ns + q)] = Y;
+                }
+            }
+    }
+    e = gr_elements();
+    e.xyz.reserve(3 * (n_grid + (uint64_t)(ni / 2) * nc));
+    for (const V3 &v : X) {
+        e.xyz.push_back(v.x);
+        e.xyz.push_back(v.y);
+        e.xyz.push_back(v.z);
+    }
+    auto pos = [&](uint32_t n) { return V3{e.xyz[3 * n], e.xyz[3 * n + 1], e.xyz[3 * n + 2]}; };
+    e.starts.push_back(0);
+    auto add = [&](uint32_t kind, const uint32_t *v, int cnt) {
+        e.kinds.push_back(kind);
+        e.nodes.insert(e.nodes.end(), v, v + cnt);
+        e.starts.push_back((uint32_t)e.nodes.size());
+    };
+    // tetrahedron with positive orientation ((1-0) x (2-0) . (3-0) > 0)
+    auto add_tet = [&](uint32_t a, uint32_t bb, uint32_t c, uint32_t dd) {
+        const V3 A = pos(a);
+        if (det3(sub(pos(bb), A), sub(pos(c), A), sub(pos(dd), A)) < 0) std::swap(bb, c);
+        const uint32_t t[4] = {a, bb, c, dd};
+        add(GR_TET, t, 4);
+    };
+    // prism: triangle (a,b,c) at j, (d,e,f) above at j+1
+    auto add_prism = [&](uint32_t a, uint32_t bb, uint32_t c, uint32_t dd, uint32_t ee, uint32_t f) {
+        const V3 A = pos(a);
+        if (det3(sub(pos(bb), A), sub(pos(c), A), sub(pos(dd), A)) < 0) {
+            std::swap(bb, c);
+            std::swap(ee, f);
+        }
+        const uint32_t w[6] = {a, bb, c, dd, ee, f};
+        add(GR_WEDGE, w, 6);
+    };
+    for (uint32_t k = 0; k < K; ++k)
+        for (uint32_t j = 0; j < nj; ++j)
+            for (uint32_t i = 0; i < ni; ++i) {
+                // corner[a + 2b + 4c]: offsets along i, j, k
+                uint32_t cr[8];
+                for (int o = 0; o < 8; ++o) cr[o] = nid(i + (o & 1), j + ((o >> 1) & 1), k + ((o >> 2) & 1));
+                if (j < np) {
+                    // constant-j triangles (i,k),(i+1,k),(i+1,k+1) and (i,k),(i+1,k+1),(i,k+1)
+                    add_prism(cr[0], cr[1], cr[5], cr[2], cr[3], cr[7]);
+                    add_prism(cr[0], cr[5], cr[4], cr[2], cr[7], cr[6]);
+                } else {
+                    static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+                    for (const auto &pm : perms) {
+                        const int o1 = 1 << pm[0], o2 = o1 | (1 << pm[1]);
+                        add_tet(cr[0], cr[o1], cr[o2], cr[7]);
+                    }
+                }
+            }
+    // fill above the tip: cell m between lower ring m..m+1 and upper ring ni-m..ni-m-1
+    const uint32_t half = ni / 2;
+    for (uint32_t k = ns; k < K; ++k)
+        for (uint32_t m = 0; m < half; ++m) {
+            const uint32_t a0 = m, a1 = m + 1, b0 = (ni - m) % ni, b1 = ni - m - 1;
+            std::vector<uint32_t> poly;   // ring indices, cyclic
+            if (m == 0) poly = {a0, a1, b1};
+            else if (m == half - 1) poly = {a0, a1, b0};
+            else poly = {a0, a1, b1, b0};
+            // centroid node
+            V3 cen = {0, 0, 0};
+            for (uint32_t kk = k; kk <= k + 1; ++kk)
+                for (uint32_t r : poly) {
+                    const V3 v = pos(nid(r, 0, kk));
+                    cen.x += v.x;
+                    cen.y += v.y;
+                    cen.z += v.z;
+                }
+            const double inv = 1.0 / (2.0 * poly.size());
+            const uint32_t cn = (uint32_t)(e.xyz.size() / 3);
+            e.xyz.push_back(cen.x * inv);
+            e.xyz.push_back(cen.y * inv);
+            e.xyz.push_back(cen.z * inv);
+            auto quad = [&](uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, bool diag02) {
+                if (diag02) {
+                    add_tet(p0, p1, p2, cn);
+                    add_tet(p0, p2, p3, cn);
+                } else {
+                    add_tet(p0, p1, p3, cn);
+                    add_tet(p1, p2, p3, cn);
+                }
+            };
+            // bottom and top faces: triangles as they are, quads along a0 - b1
+            for (uint32_t kk = k; kk <= k + 1; ++kk) {
+                if (poly.size() == 3) add_tet(nid(poly[0], 0, kk), nid(poly[1], 0, kk), nid(poly[2], 0, kk), cn);
+                else quad(nid(a0, 0, kk), nid(a1, 0, kk), nid(b1, 0, kk), nid(b0, 0, kk), true);
+            }
+            // side faces, one per polygon edge (u, v): quad (u,k),(v,k),(v,k+1),(u,k+1), cut along
+            // (from,k)-(to,k+1).  Ring edges r -> r+1 use the prism layers' diagonal (r,k)-(r+1,k+1);
+            // cross edges (lower l, upper ni-l) use (l,k)-(ni-l,k+1).
+            for (size_t q = 0; q < poly.size(); ++q) {
+                const uint32_t u = poly[q], v = poly[(q + 1) % poly.size()];
+                uint32_t from, to;
+                if ((v + ni - u) % ni == 1) {
+                    from = u;
+                    to = v;
+                } else if ((u + ni - v) % ni == 1) {
+                    from = v;
+                    to = u;
+                } else {
+                    from = std::min(u, v);
+                    to = std::max(u, v);
+                }
+                const uint32_t f0 = nid(from, 0, k), t1 = nid(to, 0, k + 1);
+                add_tet(f0, nid(to, 0, k), t1, cn);
+                add_tet(f0, t1, nid(from, 0, k + 1), cn);
+            }
+        }
+    auto tag_j = [&](uint32_t n) -> int64_t { return n < n_grid ? (int64_t)((n / ni) % (nj + 1)) : -1; };
+    auto tag_k = [&](uint32_t n) -> int64_t { return n < n_grid ? (int64_t)(n / (ni * (nj + 1))) : -1; };
+    add_boundary_markers(e, {"wall", "farfield", "symmetry"}, [&](const uint32_t *fn, int c) {
+        auto all = [&](auto pred) {
+            for (int q = 0; q < c; ++q)
+                if (!pred(fn[q])) return false;
+            return true;
+        };
+        if (all([&](uint32_t n) { return tag_k(n) == 0; })) return 2;
+        if (all([&](uint32_t n) { return tag_j(n) == 0 && tag_k(n) >= 0 && tag_k(n) <= (int64_t)ns; })) return 0;
+        if (all([&](uint32_t n) { return tag_j(n) == (int64_t)nj || tag_k(n) == (int64_t)K; })) return 1;
+        return -1;
+    });
+}
+
 // ---- su2 text format (src/meshio.cu:771-897) -----------------------------------------------
 
 void read_su2(const char *path, gr_elements &e) {
@@ -606,6 +903,20 @@ int gr_gen_ogrid(uint32_t n_theta, uint32_t n_r, uint32_t n_z, double r0, double
         auto *e = new gr_elements();
         try {
             gr::gen_ogrid(n_theta, n_r, n_z, r0, r1, ratio, dz, *e);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+int gr_gen_wing(const gr_wing_params *p, gr_elements **out) {
+    return guarded([&] {
+        require(p && out, "null argument");
+        auto *e = new gr_elements();
+        try {
+            gr::gen_wing(*p, *e);
         } catch (...) {
             delete e;
             throw;

@@ -29,7 +29,7 @@ constexpr int kMlpOut = 32;    // padded output width (n_out <= 32)
 constexpr int kMlpW = 256;     // hidden width
 constexpr int kKC = 64;        // K chunk of the dense kernel staged in LDS
 constexpr int kRowsPerBlock = 128;
-constexpr int kWgradRows = 4096;   // rows reduced by one block of the weight-gradient kernel
+constexpr int kWgradChunks = 64;   // row chunks (blocks along rows) of the weight-gradient kernels
 
 __device__ inline bf16x8 zero8() {
     bf16x8 z;
@@ -104,14 +104,14 @@ __global__ __launch_bounds__(kBlock) void k_dense(const bf16 *__restrict__ X, in
 // fragments (eight consecutive rows) are contiguous.
 template <int NT>
 __global__ __launch_bounds__(kBlock) void k_wgrad(const bf16 *__restrict__ X, int ldx, const bf16 *__restrict__ D,
-                                                  int ldd, int M, int K, float *__restrict__ P) {
+                                                  int ldd, int M, int K, int per, float *__restrict__ P) {
     constexpr int N = 16 * NT;
     __shared__ bf16 sX[64][64 + 8];   // [k_in local][row local]
     __shared__ bf16 sD[N][64 + 8];    // [n][row local]
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
     const int kt = blockIdx.x, chunk = blockIdx.y;
-    const int r_begin = chunk * kWgradRows, r_end = min(M, r_begin + kWgradRows);
+    const int r_begin = chunk * per, r_end = min(M, r_begin + per);
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -163,10 +163,10 @@ __global__ void k_sum_chunks(const float *__restrict__ P, int nchunks, int n, fl
 }
 
 // Column sums of an FP32 [M x N] array over row chunks: P[chunk][N].
-__global__ void k_colsum(const float *__restrict__ Z, int ldz, int M, int N, float *__restrict__ P) {
+__global__ void k_colsum(const float *__restrict__ Z, int ldz, int M, int N, int per, float *__restrict__ P) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x, chunk = blockIdx.y;
     if (n >= N) return;
-    const int r0 = chunk * kWgradRows, r1 = min(M, r0 + kWgradRows);
+    const int r0 = chunk * per, r1 = min(M, r0 + per);
     float s = 0.f;
     for (int r = r0; r < r1; ++r) s += Z[(size_t)r * ldz + n];
     P[(size_t)chunk * N + n] = s;
@@ -301,8 +301,7 @@ struct gr_mlp {
         d16.alloc((size_t)cap * gr::kMlpW);
         d32.alloc((size_t)cap * gr::kMlpW);
         dH.alloc((size_t)cap * gr::kMlpW);
-        const int chunks = (cap + gr::kWgradRows - 1) / gr::kWgradRows;
-        part.alloc((size_t)chunks * gr::kMlpW * gr::kMlpW);
+        part.alloc((size_t)gr::kWgradChunks * gr::kMlpW * gr::kMlpW);
         lpart.alloc((size_t)cap * gr::kMlpOut / gr::kBlock + 1);
     }
     void pack(hipStream_t s) {
@@ -329,14 +328,17 @@ struct gr_mlp {
     // weight and bias gradients of layer l from its input A (BF16, lda) and d16/d32 (ld = Np)
     template <int NT>
     void wgrad(int l, const gr::bf16 *A, int lda, int M) {
-        const int chunks = (M + gr::kWgradRows - 1) / gr::kWgradRows;
+        // row chunks of a multiple of 64 rows, at most kWgradChunks of them
+        const int want = std::min(gr::kWgradChunks, (M + 63) / 64);
+        const int per = (((M + want - 1) / want) + 63) / 64 * 64;
+        const int chunks = (M + per - 1) / per;
         hipLaunchKernelGGL((gr::k_wgrad<NT>), dim3(Kp[l] / 64, chunks), dim3(gr::kBlock), 0, stream, A, lda, d16.p,
-                           Np[l], M, Kp[l], part.p);
+                           Np[l], M, Kp[l], per, part.p);
         const int n = Kp[l] * Np[l];
         hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(n)), dim3(gr::kBlock), 0, stream, part.p, chunks, n,
                            g.p + woff[l]);
         hipLaunchKernelGGL(gr::k_colsum, dim3((Np[l] + gr::kBlock - 1) / gr::kBlock, chunks), dim3(gr::kBlock), 0,
-                           stream, d32.p, Np[l], M, Np[l], part.p);
+                           stream, d32.p, Np[l], M, Np[l], per, part.p);
         hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(Np[l])), dim3(gr::kBlock), 0, stream, part.p, chunks,
                            Np[l], g.p + boff[l]);
     }

@@ -52,9 +52,17 @@ class SolveStats:
     seconds: float
 
 
-def solve(b: Batch, tol: float, reference, max_iter: int) -> SolveStats:
+def solve(b: Batch, tol: float, reference, max_iter: int, active=None) -> SolveStats:
+    """Iterate to `tol` of `reference`.  An instance that diverges restarts from freestream
+    (the plain solver is the fallback) and its iterations before the restart are counted."""
     t0 = time.perf_counter()
-    its, conv, ran = b.iterate(max_iter, tol=tol, reference=reference)
+    its, conv, div, ran = b.iterate_masked(max_iter, tol=tol, reference=reference, active=active)
+    if div.any():
+        b.reset_freestream(div)
+        its2, conv2, div2, ran2 = b.iterate_masked(max_iter, tol=tol, reference=reference, active=div)
+        its = np.where(div, its + its2, its)
+        conv = np.where(div, conv2, conv)
+        ran += ran2
     return SolveStats(np.asarray(its), np.asarray(conv, bool), int(ran), time.perf_counter() - t0)
 
 

@@ -94,6 +94,19 @@ int gr_gen_ogrid(uint32_t n_theta, uint32_t n_r, uint32_t n_z, double r0, double
  * wake cut is merged.  Markers airfoil, farfield, zmin, zmax. */
 int gr_gen_cgrid_naca(uint32_t n_i, uint32_t n_j, uint32_t n_wake, double thickness, double radius,
                       double first_height, double dz, gr_elements **out);
+/* Swept, tapered wing with a symmetric NACA 4-digit section (root chord 1, root leading edge
+ * at the origin, span along +z from the symmetry plane z = 0) in a half-sphere far field
+ * centred on the root mid-chord.  n_prism wall layers of prisms (first_height, geometric
+ * growth), then n_tet layers of tetrahedra out to the sphere; n_span stations along the wing
+ * (clustered at the tip) and n_cap stations above the tip, where the section's interior is
+ * filled with tetrahedra up to the sphere.  Markers wall (wing surface and tip), farfield,
+ * symmetry.  Defaults giving about 1.1 M cells: 160, 20, 23, 24, 14, 0.12, 30, 0.56, 1.5,
+ * 1e-3, 1.2, 25. */
+typedef struct gr_wing_params {
+    uint32_t n_around, n_prism, n_tet, n_span, n_cap;
+    double thickness, sweep_deg, taper, semispan, first_height, growth, radius;
+} gr_wing_params;
+int gr_gen_wing(const gr_wing_params *p, gr_elements **out);
 
 /* ---- mesh ------------------------------------------------------------------------------ */
 typedef struct gr_mesh gr_mesh;
@@ -289,6 +302,13 @@ int gr_batch_get_state(gr_batch *b, uint32_t inst, double *rho, double *rhou, do
  * receives the number of batch iterations that did work. */
 int gr_batch_iterate(gr_batch *b, int max_iter, double tol, const double *reference, int *iterations,
                      int *converged, int *launched);
+/* As gr_batch_iterate for the instances with active[k] != 0 (active NULL: all); the others keep
+ * their state.  A diverged instance is reported in diverged[k] (may be NULL) and frozen, and is
+ * not an error here. */
+int gr_batch_iterate_masked(gr_batch *b, int max_iter, double tol, const double *reference, const int *active,
+                            int *iterations, int *converged, int *diverged, int *launched);
+/* Instances with mask[k] != 0 back to their freestream state. */
+int gr_batch_reset_freestream(gr_batch *b, const int *mask);
 int gr_batch_history(const gr_batch *b, uint32_t inst, double *hist, int n);
 /* Residual sum of each instance's current state (n_inst entries), state unchanged. */
 int gr_batch_residual_norms(gr_batch *b, double *out);
