@@ -385,11 +385,11 @@ void gen_cgrid(uint32_t ni, uint32_t nj, uint32_t nwake, double thick, double ra
 //
 // Structured O-grid topology (i around the section, j outward, k along the span), z spanwise
 // from the symmetry plane z = 0.  Stations k <= n_span carry the tapered, swept NACA section;
-// above the tip the section's interior is filled with centroid-split cells up to the far
-// field.  Cells: hexes j < n_prism split into two prisms along the constant-j face diagonal
-// (i,k)-(i+1,k+1); hexes j >= n_prism split into six tetrahedra (Kuhn, main diagonal
-// (i,j,k)-(i+1,j+1,k+1), which cuts every constant-j face along the same diagonal); fill cells
-// above the tip split into one tetrahedron per face triangle and a centroid node.
+// above the tip the section closes to a zero-thickness plate whose upper and lower nodes are
+// merged, capping the tip with a low ridge.  Cells: hexes j < n_prism split into two prisms
+// along the constant-j face diagonal (i,k)-(i+1,k+1); hexes j >= n_prism split into six
+// tetrahedra (Kuhn, main diagonal (i,j,k)-(i+1,j+1,k+1), which cuts every constant-j face
+// along the same diagonal).
 
 namespace {
 
@@ -430,7 +430,7 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
     }
     const double h_tet = p.first_height * std::pow(p.growth, (double)np);
     // unit section, ring order TE -> lower -> LE -> upper
-    std::vector<double> ux(ni), uy(ni), th(ni);
+    std::vector<double> ux(ni), uy(ni), th(ni), beta_of(ni);
     for (uint32_t i = 0; i < ni; ++i) {
         // cosine spacing relaxed at the leading and trailing edges (beta + sin(2 beta)/4)
         const double beta0 = 2 * pi * i / ni, beta = beta0 + 0.25 * std::sin(2 * beta0);
@@ -439,15 +439,22 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
         ux[i] = X;
         uy[i] = (i == 0 || 2 * i == ni) ? 0.0 : (2 * i < ni ? -t : t);
         th[i] = -beta0;
+        beta_of[i] = beta;
     }
     auto nid = [&](uint32_t i, uint32_t j, uint32_t k) { return (i % ni) + ni * (j + (nj + 1) * k); };
     std::vector<V3> X(n_grid);
     // wing stations
-    const double phi_b = std::asin(b / R), phi_top = 0.25 * pi;
+    // Far-field latitudes: the wing stations' radial lines end on the sphere at latitudes
+    // spread evenly over [0, phi_b] (not at the station's own height), so cells near the far
+    // field are not flattened to the station spacing; the stations above the tip continue to
+    // phi_top.
+    const double phi_b = 20.0 * pi / 180.0, phi_top = 0.25 * pi;
+    require(R * std::sin(phi_b) > 2 * b, "wing: far field too close to the tip");
     for (uint32_t k = 0; k <= ns; ++k) {
         const double z = b * std::sin(0.5 * pi * k / ns);
         const double c = 1.0 - (1.0 - p.taper) * z / b, xle = z * tsw;
-        const double rho = std::sqrt(R * R - z * z);
+        const double phi_k = phi_b * k / ns;
+        const double rho = R * std::cos(phi_k), zq = R * std::sin(phi_k);
         std::vector<V3> P(ni);
         for (uint32_t i = 0; i < ni; ++i) P[i] = {xle + c * ux[i], c * uy[i], z};
         // extrusion directions: section normals smoothed along the ring ([1 2 1] passes), so
@@ -476,8 +483,8 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             const double nx = nxs[i], ny = nys[i];
             for (uint32_t j = 0; j <= np; ++j) X[nid(i, j, k)] = {P[i].x + d[j] * nx, P[i].y + d[j] * ny, z};
             const V3 Pp = X[nid(i, np, k)];
-            const V3 Q = {cx0 + rho * std::cos(th[i]), cy0 + rho * std::sin(th[i]), z};
-            const double L = std::sqrt((Q.x - Pp.x) * (Q.x - Pp.x) + (Q.y - Pp.y) * (Q.y - Pp.y));
+            const V3 Q = {cx0 + rho * std::cos(th[i]), cy0 + rho * std::sin(th[i]), zq};
+            const double L = std::sqrt((Q.x - Pp.x) * (Q.x - Pp.x) + (Q.y - Pp.y) * (Q.y - Pp.y) + (Q.z - z) * (Q.z - z));
             require(L > 2 * h_tet, "wing: far field too close to the prism layers");
             const double r = stretch_ratio(h_tet / L, nt);
             double acc[64 + 1];
@@ -492,23 +499,25 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             }
             for (uint32_t j = 1; j <= nt; ++j) {
                 const double s = j == nt ? 1.0 : acc[j] / acc[nt];
-                X[nid(i, np + j, k)] = {Pp.x + s * (Q.x - Pp.x), Pp.y + s * (Q.y - Pp.y), z};
+                X[nid(i, np + j, k)] = {Pp.x + s * (Q.x - Pp.x), Pp.y + s * (Q.y - Pp.y), z + s * (Q.z - z)};
             }
         }
     }
-    // Stations above the tip blend from the tip-station layout to a top layout on the dome.
-    // Top layout: the tip section scaled by kCapScale about its mid-chord (so the fill cells
-    // above the tip grow with height), rings spread from it to the circle of latitude phi_top
-    // with a first spacing kCapFirst, every node lifted onto the sphere.  Outer-ring nodes move
-    // along the sphere at every station.
+    // Stations above the tip: the section closes to a plate of zero thickness (its upper and
+    // lower ring-0 nodes are merged, like a C-grid's wake cut), so the tip is capped by a low
+    // ridge between the tip station and the first station above it, and no cells are needed
+    // inside the section.  The stations blend from the tip-station layout to a top layout on
+    // the dome: the plate widened about the tip's mid-chord to half the radius of the circle of
+    // latitude phi_top (cells above the tip grow with height instead of forming a thin column),
+    // rings spread from it to that circle, every node lifted onto the sphere.  Outer-ring nodes
+    // move along the sphere at every station.
     {
-        const double kCapScale = 3.0, kCapFirst = 0.25;
         const double c_tip = p.taper, xm = b * tsw + 0.5 * c_tip;
         const double rho_top = R * std::cos(phi_top);
+        const double ae = 0.25 * rho_top, kCapFirst = 0.01 * rho_top;
         std::vector<V3> T((size_t)ni * (nj + 1));
         for (uint32_t i = 0; i < ni; ++i) {
-            const V3 B0 = X[nid(i, 0, ns)];
-            const double cxp = xm + kCapScale * (B0.x - xm), cyp = kCapScale * B0.y;
+            const double cxp = xm + ae * std::cos(beta_of[i]), cyp = 0.0;
             const double ox = cx0 + rho_top * std::cos(th[i]), oy = cy0 + rho_top * std::sin(th[i]);
             const double L = std::sqrt((ox - cxp) * (ox - cxp) + (oy - cyp) * (oy - cyp));
             require(L > 4 * kCapFirst, "wing: far field too close to the tip");
@@ -549,6 +558,11 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
                         const double phi = phi_b + s * (phi_top - phi_b);
                         Y = {cx0 + R * std::cos(phi) * std::cos(th[i]), cy0 + R * std::cos(phi) * std::sin(th[i]),
                              R * std::sin(phi)};
+                    } else if (j == 0) {
+                        // on the plate: upper node i coincides with lower node ni - i
+                        const uint32_t il = 2 * i > ni ? ni - i : i;
+                        const V3 Bl = X[nid(il, 0, ns)], Dl = T[il];
+                        Y = {Bl.x + s * (Dl.x - Bl.x), 0.0, Bl.z + s * (Dl.z - Bl.z)};
                     } else {
                         Y = {B.x + s * (D.x - B.x), B.y + s * (D.y - B.y), B.z + s * (D.z - B.z)};
                     }
@@ -556,13 +570,25 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
                 }
             }
     }
+    // node ids: upper plate nodes (j = 0, k > n_span, i > ni/2) are the lower ones
+    std::vector<uint32_t> remap(n_grid), canon;
+    canon.reserve(n_grid);
     e = gr_elements();
-    e.xyz.reserve(3 * (n_grid + (uint64_t)(ni / 2) * nc));
-    for (const V3 &v : X) {
-        e.xyz.push_back(v.x);
-        e.xyz.push_back(v.y);
-        e.xyz.push_back(v.z);
-    }
+    e.xyz.reserve(3 * n_grid);
+    for (uint32_t k = 0; k <= K; ++k)
+        for (uint32_t j = 0; j <= nj; ++j)
+            for (uint32_t i = 0; i < ni; ++i) {
+                const uint32_t g = nid(i, j, k);
+                if (j == 0 && k > ns && 2 * i > ni) {
+                    remap[g] = remap[nid(ni - i, 0, k)];
+                    continue;
+                }
+                remap[g] = (uint32_t)(e.xyz.size() / 3);
+                canon.push_back(g);
+                e.xyz.push_back(X[g].x);
+                e.xyz.push_back(X[g].y);
+                e.xyz.push_back(X[g].z);
+            }
     auto pos = [&](uint32_t n) { return V3{e.xyz[3 * n], e.xyz[3 * n + 1], e.xyz[3 * n + 2]}; };
     e.starts.push_back(0);
     auto add = [&](uint32_t kind, const uint32_t *v, int cnt) {
@@ -592,7 +618,7 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             for (uint32_t i = 0; i < ni; ++i) {
                 // corner[a + 2b + 4c]: offsets along i, j, k
                 uint32_t cr[8];
-                for (int o = 0; o < 8; ++o) cr[o] = nid(i + (o & 1), j + ((o >> 1) & 1), k + ((o >> 2) & 1));
+                for (int o = 0; o < 8; ++o) cr[o] = remap[nid(i + (o & 1), j + ((o >> 1) & 1), k + ((o >> 2) & 1))];
                 if (j < np) {
                     // constant-j triangles (i,k),(i+1,k),(i+1,k+1) and (i,k),(i+1,k+1),(i,k+1)
                     add_prism(cr[0], cr[1], cr[5], cr[2], cr[3], cr[7]);
@@ -605,66 +631,8 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
                     }
                 }
             }
-    // fill above the tip: cell m between lower ring m..m+1 and upper ring ni-m..ni-m-1
-    const uint32_t half = ni / 2;
-    for (uint32_t k = ns; k < K; ++k)
-        for (uint32_t m = 0; m < half; ++m) {
-            const uint32_t a0 = m, a1 = m + 1, b0 = (ni - m) % ni, b1 = ni - m - 1;
-            std::vector<uint32_t> poly;   // ring indices, cyclic
-            if (m == 0) poly = {a0, a1, b1};
-            else if (m == half - 1) poly = {a0, a1, b0};
-            else poly = {a0, a1, b1, b0};
-            // centroid node
-            V3 cen = {0, 0, 0};
-            for (uint32_t kk = k; kk <= k + 1; ++kk)
-                for (uint32_t r : poly) {
-                    const V3 v = pos(nid(r, 0, kk));
-                    cen.x += v.x;
-                    cen.y += v.y;
-                    cen.z += v.z;
-                }
-            const double inv = 1.0 / (2.0 * poly.size());
-            const uint32_t cn = (uint32_t)(e.xyz.size() / 3);
-            e.xyz.push_back(cen.x * inv);
-            e.xyz.push_back(cen.y * inv);
-            e.xyz.push_back(cen.z * inv);
-            auto quad = [&](uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, bool diag02) {
-                if (diag02) {
-                    add_tet(p0, p1, p2, cn);
-                    add_tet(p0, p2, p3, cn);
-                } else {
-                    add_tet(p0, p1, p3, cn);
-                    add_tet(p1, p2, p3, cn);
-                }
-            };
-            // bottom and top faces: triangles as they are, quads along a0 - b1
-            for (uint32_t kk = k; kk <= k + 1; ++kk) {
-                if (poly.size() == 3) add_tet(nid(poly[0], 0, kk), nid(poly[1], 0, kk), nid(poly[2], 0, kk), cn);
-                else quad(nid(a0, 0, kk), nid(a1, 0, kk), nid(b1, 0, kk), nid(b0, 0, kk), true);
-            }
-            // side faces, one per polygon edge (u, v): quad (u,k),(v,k),(v,k+1),(u,k+1), cut along
-            // (from,k)-(to,k+1).  Ring edges r -> r+1 use the prism layers' diagonal (r,k)-(r+1,k+1);
-            // cross edges (lower l, upper ni-l) use (l,k)-(ni-l,k+1).
-            for (size_t q = 0; q < poly.size(); ++q) {
-                const uint32_t u = poly[q], v = poly[(q + 1) % poly.size()];
-                uint32_t from, to;
-                if ((v + ni - u) % ni == 1) {
-                    from = u;
-                    to = v;
-                } else if ((u + ni - v) % ni == 1) {
-                    from = v;
-                    to = u;
-                } else {
-                    from = std::min(u, v);
-                    to = std::max(u, v);
-                }
-                const uint32_t f0 = nid(from, 0, k), t1 = nid(to, 0, k + 1);
-                add_tet(f0, nid(to, 0, k), t1, cn);
-                add_tet(f0, t1, nid(from, 0, k + 1), cn);
-            }
-        }
-    auto tag_j = [&](uint32_t n) -> int64_t { return n < n_grid ? (int64_t)((n / ni) % (nj + 1)) : -1; };
-    auto tag_k = [&](uint32_t n) -> int64_t { return n < n_grid ? (int64_t)(n / (ni * (nj + 1))) : -1; };
+    auto tag_j = [&](uint32_t n) -> int64_t { return (int64_t)((canon[n] / ni) % (nj + 1)); };
+    auto tag_k = [&](uint32_t n) -> int64_t { return (int64_t)(canon[n] / (ni * (nj + 1))); };
     add_boundary_markers(e, {"wall", "farfield", "symmetry"}, [&](const uint32_t *fn, int c) {
         auto all = [&](auto pred) {
             for (int q = 0; q < c; ++q)
@@ -672,7 +640,7 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             return true;
         };
         if (all([&](uint32_t n) { return tag_k(n) == 0; })) return 2;
-        if (all([&](uint32_t n) { return tag_j(n) == 0 && tag_k(n) >= 0 && tag_k(n) <= (int64_t)ns; })) return 0;
+        if (all([&](uint32_t n) { return tag_j(n) == 0; })) return 0;
         if (all([&](uint32_t n) { return tag_j(n) == (int64_t)nj || tag_k(n) == (int64_t)K; })) return 1;
         return -1;
     });

@@ -8,8 +8,7 @@ WING_SMALL = dict(n_around=32, n_prism=4, n_tet=5, n_span=6, n_cap=4, radius=12.
 
 def wing_counts(ni, npr, nt, ns, nc):
     K = ns + nc
-    fill = (2 * 8 + (ni // 2 - 2) * 12) * nc   # centroid-split cells above the tip
-    return 2 * ni * npr * K + 6 * ni * nt * K + fill
+    return 2 * ni * npr * K + 6 * ni * nt * K
 
 
 def test_wing_counts_and_markers(gr):
@@ -26,7 +25,7 @@ def test_wing_default_size():
     import graphite_amd as gr
     gr.load()
     s = gr.Mesh.build(gr.Elements.wing()).sizes()
-    assert s.n_cells == wing_counts(160, 20, 23, 24, 14) == 1095568
+    assert s.n_cells == wing_counts(160, 20, 23, 24, 14) == 1082240
 
 
 def test_wing_geometry(gr, orc):
@@ -37,8 +36,9 @@ def test_wing_geometry(gr, orc):
     V = om.cell_volumes[:N]
     assert V.min() > 0
     # closure: the outward n A of a cell's faces sum to zero for planar faces (tetrahedra,
-    # exactly); prism side faces between twisted layers are slightly non-planar, and the
-    # reference's face area (sum of fan-triangle areas) then leaves a small residue
+    # exactly); prism side faces between twisted layers (most of all in the ridge over the
+    # tip) are non-planar, and the reference's face area (sum of fan-triangle areas) then
+    # leaves a residue
     x, kinds, starts, nodes, _ = e.arrays()
     cfs, cf, cc = om.cell_faces_starts, om.cell_faces, om.cell_connects
     fn, fa, fc, xc = om.face_normals, om.face_areas, om.face_centers, om.cell_centers
@@ -52,7 +52,7 @@ def test_wing_geometry(gr, orc):
         r = np.linalg.norm(acc) / fa[cf[cfs[i]:cfs[i + 1]]].sum()
         worst[int(kinds[i])] = max(worst[int(kinds[i])], r)
     assert worst[gr.GR_TET] < 1e-12
-    assert worst[gr.GR_WEDGE] < 1e-2
+    assert worst[gr.GR_WEDGE] < 5e-2
     # symmetry plane: planar at z = 0, area of the inscribed 32-gon minus the root section
     r = om.region("symmetry")
     g0, g1 = om.ghost_cell_starts[r], om.ghost_cell_starts[r + 1]
