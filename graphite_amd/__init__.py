@@ -16,7 +16,7 @@ import numpy as np
 
 from ._capi import (GR_BC_FARFIELD, GR_BC_SLIP, GR_F32, GR_F64, GR_HEX, GR_LIM_BARTH_JESPERSEN,
                     GR_LIM_MICHALAK, GR_LIM_VENKATAKRISHNAN, GR_OK, GR_PYRAMID, GR_QUAD, GR_TET,
-                    GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, WarmNorm, WingParams, load)
+                    GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, WarmNorm, WingParams, NsiConfig, load)
 
 __all__ = ["GraphiteError", "Elements", "Mesh", "Field", "Solver", "device_count", "load",
            "euler_init", "euler_bc_slip", "euler_bc_farfield", "euler_step_prepare", "euler_step",
@@ -548,13 +548,15 @@ class Batch:
         _check(load().gr_batch_get_states(self._h, _p(r, C.c_double), _p(u, C.c_double), _p(e, C.c_double)))
         return r, u, e
 
-    def warm_rows(self, wall_distance):
+    def warm_rows(self, wall_distance, origin):
         """Raw warm-start inputs (n_inst*N, 6) and targets (n_inst*N, 5) of the current states,
-        row = cell * n_inst + instance."""
+        row = cell * n_inst + instance; centre coordinates compressed about `origin`."""
         wd = _f64(wall_distance)
+        o = np.ascontiguousarray(origin, np.float32)
         rows = self.mesh.n_cells * self.n_inst
         x, t = np.zeros((rows, 6), np.float32), np.zeros((rows, 5), np.float32)
-        _check(load().gr_batch_warm_rows(self._h, _p(wd, C.c_double), _p(x, C.c_float), _p(t, C.c_float)))
+        _check(load().gr_batch_warm_rows(self._h, _p(wd, C.c_double), _p(o, C.c_float), _p(x, C.c_float),
+                                         _p(t, C.c_float)))
         return x, t
 
     def warm_start(self, net, wall_distance, norm, max_ratio: float = 1.0):
@@ -679,10 +681,13 @@ class Mlp:
         return loss.value
 
 
-def warm_norm(x, t) -> WarmNorm:
-    """Input/output standardisation of a warm-start training set (mean, std per column)."""
+def warm_norm(x, t, origin=(0.0, 0.0, 0.0)) -> WarmNorm:
+    """Input/output standardisation of a warm-start training set (mean, std per column); origin
+    is the point the rows' centre coordinates were compressed about."""
     x, t = np.asarray(x, np.float64), np.asarray(t, np.float64)
     n = WarmNorm()
+    for k in range(3):
+        n.origin[k] = origin[k]
     xm, xs = x.mean(0), x.std(0)
     tm, ts = t.mean(0), t.std(0)
     for c in range(6):
@@ -697,3 +702,61 @@ def normalise(x, t, norm: WarmNorm):
     xm, xs = np.array(norm.in_mean[:]), np.array(norm.in_std[:])
     tm, ts = np.array(norm.out_mean[:]), np.array(norm.out_std[:])
     return (((x - xm) / xs).astype(np.float32), ((t - tm) / ts).astype(np.float32))
+
+
+# ---- incompressible pressure-correction solver ----------------------------------------------
+
+class Nsi:
+    """Pressure-correction solver for incompressible flow (constant density 1) on a mesh."""
+
+    def __init__(self, mesh: Mesh, nu: float, dt: float, dtype: int = GR_F64, cg_max: int = 200,
+                 cg_tol: float = 1e-6):
+        cfg = NsiConfig(dtype, nu, dt, cg_max, cg_tol)
+        h = C.c_void_p()
+        _check(load().gr_nsi_create(mesh._h, C.byref(cfg), C.byref(h)))
+        self._h, self.mesh, self.dtype = h, mesh, dtype
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().gr_nsi_free(h)
+            self._h = None
+
+    def set_wall(self, region: str, u) -> None:
+        v = _f64(u)
+        _check(load().gr_nsi_set_wall(self._h, region.encode(), _p(v, C.c_double)))
+
+    def set_state(self, u=None, p=None, F=None) -> None:
+        args = []
+        for a in (u, p, F):
+            if a is None:
+                args.append(None)
+            else:
+                a = _f64(a).reshape(-1)
+                args.append(a)
+        _check(load().gr_nsi_set_state(self._h, *[None if a is None else _p(a, C.c_double) for a in args]))
+
+    def get_state(self):
+        s = self.mesh.sizes()
+        n = s.n_cells + s.n_ghosts
+        u, p, F = np.zeros((n, 3)), np.zeros(n), np.zeros(max(s.n_faces, 1))
+        _check(load().gr_nsi_get_state(self._h, _p(u, C.c_double), _p(p, C.c_double), _p(F, C.c_double)))
+        return u, p, F[:s.n_faces]
+
+    def step(self, n: int = 1):
+        """Returns (continuity residual, CG iterations) of each step."""
+        c = np.zeros(n)
+        it = (C.c_int * n)()
+        _check(load().gr_nsi_step(self._h, n, _p(c, C.c_double), it))
+        return c, np.array(it[:], dtype=int)
+
+    def iterate(self, max_steps: int, tol: float):
+        steps, ratio = C.c_int(), C.c_double()
+        _check(load().gr_nsi_iterate(self._h, max_steps, tol, C.byref(steps), C.byref(ratio)))
+        return steps.value, ratio.value
+
+    def history(self, n: int):
+        c = np.zeros(max(n, 1))
+        it = (C.c_int * max(n, 1))()
+        _check(load().gr_nsi_history(self._h, _p(c, C.c_double), it, n))
+        return c[:n], np.array(it[:n], dtype=int)

@@ -47,8 +47,12 @@ class WingParams(C.Structure):
                 ("first_height", _d), ("growth", _d), ("radius", _d)]
 
 
+class NsiConfig(C.Structure):
+    _fields_ = [("dtype", _i), ("nu", _d), ("dt", _d), ("cg_max", _i), ("cg_tol", _d)]
+
+
 class WarmNorm(C.Structure):
-    _fields_ = [("in_mean", C.c_float * 6), ("in_std", C.c_float * 6),
+    _fields_ = [("origin", C.c_float * 3), ("in_mean", C.c_float * 6), ("in_std", C.c_float * 6),
                 ("out_mean", C.c_float * 5), ("out_std", C.c_float * 5)]
 
 
@@ -132,6 +136,14 @@ SIGNATURES = {
     "gr_solver_force": (_i, [_vp, _s, _dp]),
     "gr_batch_force": (_i, [_vp, _s, _dp]),
     "gr_mesh_wall_distance": (_i, [_vp, C.POINTER(C.c_char_p), _u32, _dp]),
+    "gr_nsi_create": (_i, [_vp, C.POINTER(NsiConfig), C.POINTER(_vp)]),
+    "gr_nsi_free": (None, [_vp]),
+    "gr_nsi_set_wall": (_i, [_vp, _s, _dp]),
+    "gr_nsi_set_state": (_i, [_vp, _dp, _dp, _dp]),
+    "gr_nsi_get_state": (_i, [_vp, _dp, _dp, _dp]),
+    "gr_nsi_step": (_i, [_vp, _i, _dp, C.POINTER(_i)]),
+    "gr_nsi_iterate": (_i, [_vp, _i, _d, C.POINTER(_i), _dp]),
+    "gr_nsi_history": (_i, [_vp, _dp, C.POINTER(_i), _i]),
     "gr_mlp_create": (_i, [_u32, _u32, C.c_uint64, C.POINTER(_vp)]),
     "gr_mlp_free": (None, [_vp]),
     "gr_mlp_n_params": (_i, [_vp, C.POINTER(C.c_size_t)]),
@@ -144,7 +156,7 @@ SIGNATURES = {
     "gr_mlp_train": (_i, [_vp, _i, _i, C.c_float, C.c_float, C.c_uint64, _dp]),
     "gr_batch_set_states": (_i, [_vp, _dp, _dp, _dp]),
     "gr_batch_get_states": (_i, [_vp, _dp, _dp, _dp]),
-    "gr_batch_warm_rows": (_i, [_vp, _dp, _fp, _fp]),
+    "gr_batch_warm_rows": (_i, [_vp, _dp, _fp, _fp, _fp]),
     "gr_batch_warm_start": (_i, [_vp, _vp, _dp, C.POINTER(WarmNorm), _d, _dp, C.POINTER(_i)]),
 }
 

@@ -330,10 +330,14 @@ __global__ void k_warm_features(MeshView m, const double *__restrict__ wd, const
     double f = 0;
     if (c < 6) {
         const double mx = inst_mach[3 * k], my = inst_mach[3 * k + 1], mz = inst_mach[3 * k + 2];
+        // centre coordinates compressed about the origin: (x - o) / (1 + |x - o|) resolves the
+        // body's neighbourhood and keeps the far field bounded
+        const double ex = m.cx[i] - nrm.origin[0], ey = m.cy[i] - nrm.origin[1], ez = m.cz[i] - nrm.origin[2];
+        const double q = 1.0 + dsqrt((ex * ex + ey * ey) + ez * ez);
         switch (c) {
-        case 0: f = m.cx[i]; break;
-        case 1: f = m.cy[i]; break;
-        case 2: f = m.cz[i]; break;
+        case 0: f = ex / q; break;
+        case 1: f = ey / q; break;
+        case 2: f = ez / q; break;
         case 3: f = log(wd[i] + 1e-3); break;
         case 4: f = dsqrt((mx * mx + my * my) + mz * mz); break;
         default: f = atan2(my, mx); break;
@@ -971,9 +975,9 @@ int gr_batch_set_graph_iters(gr_batch *b, int iters) {
     });
 }
 
-int gr_batch_warm_rows(gr_batch *b, const double *wall_distance, float *x, float *t) {
+int gr_batch_warm_rows(gr_batch *b, const double *wall_distance, const float origin[3], float *x, float *t) {
     return guarded([&] {
-        require(b && wall_distance && x && t, "null argument");
+        require(b && wall_distance && origin && x && t, "null argument");
         gr_mesh *m = b->mesh;
         const uint32_t N = m->h.n_cells;
         const size_t rows = (size_t)N * b->n_inst;
@@ -983,7 +987,8 @@ int gr_batch_warm_rows(gr_batch *b, const double *wall_distance, float *x, float
         X.alloc(rows * 6);
         T.alloc(rows * 5);
         wd.upload(wall_distance, N, m->stream);
-        const gr_warm_norm none{};
+        gr_warm_norm none{};
+        for (int k = 0; k < 3; ++k) none.origin[k] = origin[k];
         hipLaunchKernelGGL(gr::k_warm_features<false>, dim3(gr::blocks_for(rows * 6)), dim3(gr::kBlock), 0, m->stream,
                            m->view(), wd.p, (const double *)b->inst_mach.p, b->n_inst, (size_t)0, (int)rows, none,
                            (void *)X.p);

@@ -44,6 +44,17 @@ def wall_distance(mesh: Mesh) -> np.ndarray:
     return mesh.wall_distance(["airfoil"])
 
 
+def wall_origin(mesh: Mesh, region: str = "airfoil") -> np.ndarray:
+    """Area-weighted centroid of a wall region's faces (the feature origin)."""
+    mt, c = mesh.metrics(), mesh.connectivity()
+    r = mesh.region_id(region)
+    N = mesh.n_cells
+    faces = [c["cell_faces"][c["cell_faces_starts"][N + g]]
+             for g in range(c["ghost_cell_starts"][r], c["ghost_cell_starts"][r + 1])]
+    a = mt["face_areas"][faces]
+    return (mt["face_centers"][faces] * a[:, None]).sum(0) / a.sum()
+
+
 @dataclass
 class SolveStats:
     iterations: np.ndarray     # per instance
@@ -70,12 +81,13 @@ def generate_training_set(mesh: Mesh, machs, tol: float, max_iter: int, dtype: i
     """Solve the instances in batches of 64 to `tol` (freestream-normalised residual) and
     return the raw rows (x [n*N, 6], t [n*N, 5]) and per-batch solve statistics."""
     wd = wall_distance(mesh)
+    origin = wall_origin(mesh)
     xs, ts, stats = [], [], []
     for s in range(0, len(machs), LANES):
         b = make_batch(mesh, machs[s:s + LANES], dtype)
         fs = b.residual_norms()
         st = solve(b, tol, fs, max_iter)
-        x, t = b.warm_rows(wd)
+        x, t = b.warm_rows(wd, origin)
         xs.append(x)
         ts.append(t)
         stats.append(st)
@@ -83,12 +95,12 @@ def generate_training_set(mesh: Mesh, machs, tol: float, max_iter: int, dtype: i
             log(f"training set: instances {s}..{s + len(machs[s:s + LANES]) - 1} solved in {st.batch_iterations} "
                 f"iterations, {st.seconds:.2f} s, converged {int(st.converged.sum())}")
         del b
-    return np.concatenate(xs), np.concatenate(ts), stats
+    return np.concatenate(xs), np.concatenate(ts), origin, stats
 
 
-def train(x, t, steps: int, batch: int = 8192, lr: float = 2e-3, lr_final: float = 2e-5, seed: int = 0):
+def train(x, t, origin, steps: int, batch: int = 32768, lr: float = 2e-3, lr_final: float = 2e-5, seed: int = 0):
     """Train the warm-start network on raw rows; returns (network, normalisation, losses, seconds)."""
-    norm = warm_norm(x, t)
+    norm = warm_norm(x, t, origin)
     xn, tn = normalise(x, t, norm)
     net = Mlp(6, 5, seed=seed)
     net.set_data(xn, tn)

@@ -98,10 +98,10 @@ int gr_gen_cgrid_naca(uint32_t n_i, uint32_t n_j, uint32_t n_wake, double thickn
  * at the origin, span along +z from the symmetry plane z = 0) in a half-sphere far field
  * centred on the root mid-chord.  n_prism wall layers of prisms (first_height, geometric
  * growth), then n_tet layers of tetrahedra out to the sphere; n_span stations along the wing
- * (clustered at the tip) and n_cap stations above the tip, where the section's interior is
- * filled with tetrahedra up to the sphere.  Markers wall (wing surface and tip), farfield,
- * symmetry.  Defaults giving about 1.1 M cells: 160, 20, 23, 24, 14, 0.12, 30, 0.56, 1.5,
- * 1e-3, 1.2, 25. */
+ * (clustered at the tip) and n_cap stations above the tip, where the section closes to a
+ * zero-thickness plate (upper and lower nodes merged) and the tip is capped by a low ridge.
+ * Markers wall (wing surface and tip ridge), farfield, symmetry.  Defaults giving about
+ * 1.08 M cells: 160, 20, 23, 24, 14, 0.12, 30, 0.56, 1.5, 1e-3, 1.2, 25. */
 typedef struct gr_wing_params {
     uint32_t n_around, n_prism, n_tet, n_span, n_cap;
     double thickness, sweep_deg, taper, semispan, first_height, growth, radius;
@@ -334,6 +334,39 @@ int gr_mesh_wall_distance(gr_mesh *m, const char *const *regions, uint32_t n_reg
  * Calibrates hardware byte counters and measures the achievable HBM rate. */
 int gr_stream_copy_probe(size_t bytes, int reps, double *gbps);
 
+/* ---- incompressible pressure-correction solver ----------------------------------------------
+ * Replaces the reference's PhysicsNSI (include/models/nsi.cuh:8-40), a stub whose
+ * src/models/nsi.cu does not compile and holds no arithmetic.  Collocated cell-centred
+ * scheme, constant density 1: ghosts (no-slip walls moving with the region's velocity, zero
+ * normal pressure gradient), least-squares gradients, predictor (second-order upwind
+ * convection, two-point diffusion, explicit pressure gradient), momentum-interpolated face
+ * fluxes, a pressure-correction Poisson equation solved by Jacobi-preconditioned conjugate
+ * gradients, and flux/velocity/pressure corrections.  State: velocity (N+G)x3 AoS, pressure
+ * N+G, face fluxes n_faces (volume flux from the face's lower- to its higher-numbered cell). */
+typedef struct gr_nsi gr_nsi;
+typedef struct gr_nsi_config {
+    int dtype;       /* GR_F64, or GR_F32: binary32 state and CG vectors, binary64 dot products */
+    double nu, dt;   /* kinematic viscosity, time step */
+    int cg_max;      /* CG iterations per step at most */
+    double cg_tol;   /* stop CG when |r| < cg_tol |b| (0: always cg_max iterations) */
+} gr_nsi_config;
+
+int gr_nsi_create(gr_mesh *m, const gr_nsi_config *cfg, gr_nsi **out);
+void gr_nsi_free(gr_nsi *s);
+/* Wall velocity of a boundary region (default: at rest). */
+int gr_nsi_set_wall(gr_nsi *s, const char *region, const double u[3]);
+/* Any of u, p, F may be NULL (zero). */
+int gr_nsi_set_state(gr_nsi *s, const double *u, const double *p, const double *F);
+int gr_nsi_get_state(gr_nsi *s, double *u, double *p, double *F);
+/* n_steps steps; cont / cg_iters (n_steps entries, may be NULL) receive each step's continuity
+ * residual (sum over cells of |net outflow of the predicted fluxes|) and CG iterations. */
+int gr_nsi_step(gr_nsi *s, int n_steps, double *cont, int *cg_iters);
+/* Steps until the continuity residual falls below tol times that of the first step of this
+ * call (or max_steps). */
+int gr_nsi_iterate(gr_nsi *s, int max_steps, double tol, int *steps, double *ratio);
+/* The last n steps' continuity residuals and CG iterations. */
+int gr_nsi_history(const gr_nsi *s, double *cont, int *cg_iters, int n);
+
 /* ---- warm-start network (no reference counterpart) -------------------------------------------
  * A 4-layer MLP n_in -> 256 -> 256 -> 256 -> n_out (tanh on the hidden layers) trained and run
  * with bfloat16 matrix-core products accumulated in binary32.  Parameters are binary32 in a
@@ -364,11 +397,13 @@ int gr_mlp_train(gr_mlp *n, int batch, int steps, float lr, float lr_final, uint
 
 /* ---- learned warm start of the batched solver (no reference counterpart) --------------------
  * Network rows are (cell, instance) pairs, row = cell * n_inst + instance, over interior cells.
- * Raw inputs (n_in = 6): cell centre x, y, z, log(d + 1e-3) with d the wall distance, the
- * instance's Mach number |M| and angle of attack atan2(M_y, M_x) in radians.  Raw targets
- * (n_out = 5): rho/rho_inf - 1, (rho u - rho_inf u_inf)/(rho_inf c_inf) (3 entries) and
+ * Raw inputs (n_in = 6): the cell centre x relative to an origin o, compressed as
+ * (x - o) / (1 + |x - o|) (3 entries), log(d + 1e-3) with d the wall distance, the instance's
+ * Mach number |M| and angle of attack atan2(M_y, M_x) in radians.  Raw targets (n_out = 5):
+ * rho/rho_inf - 1, (rho u - rho_inf u_inf)/(rho_inf c_inf) (3 entries) and
  * (rho E - rho_inf E_inf)/(rho_inf c_inf^2), with the far-field state of the instance. */
 typedef struct gr_warm_norm {
+    float origin[3];                 /* origin of the compressed centre coordinates */
     float in_mean[6], in_std[6];     /* network input = (raw - mean) / std */
     float out_mean[5], out_std[5];   /* raw target = network output * std + mean */
 } gr_warm_norm;
@@ -378,7 +413,7 @@ int gr_batch_set_states(gr_batch *b, const double *rho, const double *rhou, cons
 int gr_batch_get_states(gr_batch *b, double *rho, double *rhou, double *rhoe);
 /* Raw inputs x [n_inst*N][6] and targets t [n_inst*N][5] of the batch's current states
  * (a training set); wall_distance has n_cells entries. */
-int gr_batch_warm_rows(gr_batch *b, const double *wall_distance, float *x, float *t);
+int gr_batch_warm_rows(gr_batch *b, const double *wall_distance, const float origin[3], float *x, float *t);
 /* Resets every instance to its freestream (gr_batch_set_freestream must have run), records
  * the freestream residuals (fs_residual, n_inst entries: the normalisation of the solve that
  * follows), replaces the states by the network's prediction and then judges each prediction

@@ -52,6 +52,14 @@ class _OrcBC(C.Structure):
                 ("p", C.c_double), ("T", C.c_double)]
 
 
+class _NsiWall(C.Structure):
+    _fields_ = [("region", C.c_uint32), ("u", C.c_double * 3)]
+
+
+class _NsiParams(C.Structure):
+    _fields_ = [("nu", C.c_double), ("dt", C.c_double), ("cg_max", C.c_int), ("cg_tol", C.c_double)]
+
+
 def build() -> str:
     """Compile liboracle.so with the committed Makefile (gcc, no GPU needed)."""
     subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
@@ -98,6 +106,11 @@ def lib() -> C.CDLL:
         L.orc_euler_iterate.argtypes = [mp, C.POINTER(_OrcState), C.POINTER(_OrcBC), C.c_int,
                                         C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.c_double, _f64p, C.c_int]
+        L.orc_nsi_step.restype = C.c_int
+        L.orc_nsi_step.argtypes = [mp, _f64p, _f64p, _f64p, C.c_void_p, C.c_int, C.POINTER(_NsiParams),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.orc_nsi_poisson.restype = C.c_int
+        L.orc_nsi_poisson.argtypes = [mp, _f64p, _f64p, C.c_int, C.c_double]
         L.orc_mlp_n_params.restype = C.c_size_t
         L.orc_mlp_n_params.argtypes = []
         L.orc_bf16_round.restype = C.c_double
@@ -332,3 +345,40 @@ def mlp_loss_grad(params, x, t, emulate_bf16: bool = False):
                                C.byref(loss), g):
         raise ValueError("bad MLP dimensions")
     return loss.value, g
+
+
+# ---- incompressible pressure-correction (oracle_nsi.c) ----
+
+class NsiState:
+    """u (N+G, 3), p (N+G,), F (n_faces,) face fluxes from the lower- to the higher-numbered cell."""
+
+    def __init__(self, m: OracleMesh):
+        n = m.n_cells + m.n_ghosts
+        self.u = np.zeros((n, 3))
+        self.p = np.zeros(n)
+        self.F = np.zeros(m.n_faces)
+
+
+def nsi_step(m: OracleMesh, st: NsiState, walls, nu: float, dt: float, cg_max: int, cg_tol: float = 0.0):
+    """One pressure-correction step in place; walls = [(region, (ux, uy, uz))].
+    Returns (continuity residual of the predicted fluxes, CG iterations)."""
+    arr = (_NsiWall * max(1, len(walls)))()
+    for i, (r, u) in enumerate(walls):
+        arr[i].region = r
+        for k in range(3):
+            arr[i].u[k] = float(u[k])
+    prm = _NsiParams(nu, dt, cg_max, cg_tol)
+    cont, its = C.c_double(), C.c_int()
+    u = np.ascontiguousarray(st.u).reshape(-1)
+    lib().orc_nsi_step(m._p, u, st.p, st.F, C.cast(arr, C.c_void_p), len(walls), C.byref(prm), C.byref(cont),
+                       C.byref(its))
+    st.u = u.reshape(-1, 3)
+    return cont.value, its.value
+
+
+def nsi_poisson(m: OracleMesh, b, cg_max: int, cg_tol: float = 0.0):
+    """x with sum_f a_f (x_i - x_j) = b_i (zero-mean b and x); returns (x, iterations)."""
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(m.n_cells)
+    its = lib().orc_nsi_poisson(m._p, b, x, cg_max, cg_tol)
+    return x, its

@@ -170,6 +170,25 @@ int orc_mlp_forward(const double *params, int n_in, int n_out, const double *x, 
 int orc_mlp_loss_grad(const double *params, int n_in, int n_out, const double *x, const double *t, int M,
                       int emulate_bf16, double *loss, double *grad);
 
+/* Incompressible pressure-correction step (oracle_nsi.c); no reference arithmetic exists.
+ * u: 3(N+G) AoS velocity, p: N+G pressure, F: n_faces volume fluxes from the face's
+ * lower-numbered cell to the higher.  Walls: no slip with the region's velocity (regions not
+ * listed: at rest).  cont receives sum_i |net outflow of the predicted fluxes|. */
+typedef struct orc_nsi_wall {
+    uint32_t region;
+    double u[3];
+} orc_nsi_wall;
+typedef struct orc_nsi_params {
+    double nu, dt;
+    int cg_max;       /* conjugate-gradient iterations at most */
+    double cg_tol;    /* stop when |r| < cg_tol |b| (0: run cg_max iterations) */
+} orc_nsi_params;
+int orc_nsi_step(const orc_mesh *m, double *u, double *p, double *F, const orc_nsi_wall *walls, int n_walls,
+                 const orc_nsi_params *prm, double *cont, int *cg_iters);
+/* Solves sum_f a_f (x_i - x_j) = b_i (interior faces, a = A/D) by Jacobi-preconditioned CG from
+ * x = 0 with b and x shifted to zero mean; returns the iterations run. */
+int orc_nsi_poisson(const orc_mesh *m, const double *b, double *x, int cg_max, double cg_tol);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,13 +39,15 @@ def test_rows_at_freestream(gr, small):
     M = machs(3, seed=5)
     b = ws.make_batch(m, M)
     wd = ws.wall_distance(m)
-    x, t = b.warm_rows(wd)
+    o = np.array([0.5, 0.25, 0.0])
+    x, t = b.warm_rows(wd, o.astype(np.float32))
     N = m.n_cells
     assert x.shape == (3 * N, 6) and t.shape == (3 * N, 5)
-    cc = m.metrics()["cell_centers"][:N]
+    e = m.metrics()["cell_centers"][:N] - o
+    comp = e / (1 + np.linalg.norm(e, axis=1))[:, None]
     for k in range(3):
         xs = x[k::3]                                     # row = cell * n_inst + instance
-        assert np.array_equal(xs[:, :3], cc.astype(np.float32))
+        assert np.allclose(xs[:, :3], comp, rtol=1e-6, atol=1e-7)
         assert np.array_equal(xs[:, 3], np.log(wd + 1e-3).astype(np.float32))
         assert np.allclose(xs[:, 4], np.linalg.norm(M[k]), rtol=1e-7)
         assert np.allclose(xs[:, 5], np.arctan2(M[k, 1], M[k, 0]), rtol=1e-6, atol=1e-7)
