@@ -14,6 +14,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from ._capi import (GR_OP_BOUND_SCAL_GRAD_C, GR_OP_BOUND_VEC_GRAD_C, GR_OP_CONV_CONST_SCAL_U,  # noqa: F401
+                    GR_OP_CONV_CONST_SCAL_U2, GR_OP_CONV_SCAL_U, GR_OP_CONV_VEC_U, GR_OP_DIV_C,
+                    GR_OP_LAPLACIAN_C, GR_OP_SCAL_GRAD_C, GR_OP_SCAL_GRAD_C_CORRECT, GR_OP_VEC_GRAD_C)
 from ._capi import (GR_BC_FARFIELD, GR_BC_SLIP, GR_F32, GR_F64, GR_HEX, GR_LIM_BARTH_JESPERSEN,
                     GR_LIM_MICHALAK, GR_LIM_VENKATAKRISHNAN, GR_OK, GR_PYRAMID, GR_QUAD, GR_TET,
                     GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, WarmNorm, WingParams, NsiConfig, load)
@@ -349,6 +352,17 @@ class Field:
 
     def bc_zero_grad(self, region: int) -> None:
         _check(load().gr_field_bc_zero_grad(self._h, region))
+
+    def operator(self, op: int, phi_const=None, phi_field: "Field | None" = None) -> np.ndarray:
+        """Finite-volume operator `op` (GR_OP_*) of every interior cell:
+        (n_cells,), (n_cells, 3) or (n_cells, 9)."""
+        from ._capi import OP_OUTPUTS
+        n = self.mesh.n_cells
+        out = np.zeros(n * OP_OUTPUTS[op])
+        pc = None if phi_const is None else _f64(phi_const)
+        _check(load().gr_field_operator(self._h, op, None if pc is None else _p(pc, C.c_double),
+                                        None if phi_field is None else phi_field._h, _p(out, C.c_double)))
+        return out if OP_OUTPUTS[op] == 1 else out.reshape(n, -1)
 
 
 def euler_init(mesh, rho, rhou, rhoe, mach, p, T):

@@ -12,6 +12,10 @@ GR_F32, GR_F64 = 4, 8
 GR_TRI, GR_QUAD, GR_TET, GR_HEX, GR_WEDGE, GR_PYRAMID = 5, 9, 10, 12, 13, 14
 GR_LIM_MICHALAK, GR_LIM_BARTH_JESPERSEN, GR_LIM_VENKATAKRISHNAN = 0, 1, 2
 GR_BC_SLIP, GR_BC_FARFIELD = 0, 1
+(GR_OP_LAPLACIAN_C, GR_OP_SCAL_GRAD_C, GR_OP_SCAL_GRAD_C_CORRECT, GR_OP_VEC_GRAD_C, GR_OP_BOUND_SCAL_GRAD_C,
+ GR_OP_BOUND_VEC_GRAD_C, GR_OP_DIV_C, GR_OP_CONV_CONST_SCAL_U, GR_OP_CONV_CONST_SCAL_U2, GR_OP_CONV_SCAL_U,
+ GR_OP_CONV_VEC_U) = range(11)
+OP_OUTPUTS = (1, 3, 3, 9, 3, 9, 1, 1, 1, 1, 3)
 
 # Every function the header declares: name -> (restype, argtypes).
 _vp = C.c_void_p
@@ -93,6 +97,7 @@ SIGNATURES = {
     "gr_field_upload": (_i, [_vp, _dp]),
     "gr_field_download": (_i, [_vp, _dp, _dp, _dp]),
     "gr_field_init": (_i, [_vp, _dp]),
+    "gr_field_operator": (_i, [_vp, _i, _dp, _vp, _dp]),
     "gr_field_reduce_sum": (_i, [_vp, _dp]),
     "gr_field_compute_gradient": (_i, [_vp]),
     "gr_field_compute_limiter": (_i, [_vp, _i]),

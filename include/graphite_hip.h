@@ -181,6 +181,27 @@ void gr_field_free(gr_field *f);   /* FvmFieldFree, src/fvmfields.cu:49-64 */
  * (src/fvmfields.cu:85-101). */
 int gr_field_upload(gr_field *f, const double *value);
 int gr_field_download(const gr_field *f, double *value, double *gradient, double *limiter);
+/* Finite-volume operators of include/fvmops.cuh:96-219 (fvmLaplacianC, fvmScalGradC,
+ * fvmScalGradCCorrect, fvmVecGradC, fvmBoundScalGradC, fvmBoundVecGradC, fvmDivC,
+ * fvmConvConstScalU, fvmConvConstScalU2, fvmConvScalU, fvmConvVecU), each evaluated for every
+ * interior cell of field u (ghost entries as they stand, e.g. after gr_field_gradlim_bc).
+ * out: n_cells x (1 | 3 | 9) values (scalar, vector, or tensor as three columns), binary64.
+ * phi_const: the constant velocity of the ConvConst operators; phi_field: the vector field of
+ * ConvScalU / ConvVecU.  The operators reading gradients or limiters need alloc_grads. */
+enum gr_fvm_op {
+    GR_OP_LAPLACIAN_C = 0,
+    GR_OP_SCAL_GRAD_C = 1,
+    GR_OP_SCAL_GRAD_C_CORRECT = 2,
+    GR_OP_VEC_GRAD_C = 3,
+    GR_OP_BOUND_SCAL_GRAD_C = 4,
+    GR_OP_BOUND_VEC_GRAD_C = 5,
+    GR_OP_DIV_C = 6,
+    GR_OP_CONV_CONST_SCAL_U = 7,
+    GR_OP_CONV_CONST_SCAL_U2 = 8,
+    GR_OP_CONV_SCAL_U = 9,
+    GR_OP_CONV_VEC_U = 10
+};
+int gr_field_operator(const gr_field *u, int op, const double phi_const[3], const gr_field *phi_field, double *out);
 /* ScalarVarInit / VectorVarInit (src/fvmops.cu:8-43): value = v, gradient 0, limiter 1. */
 int gr_field_init(gr_field *f, const double *v);
 /* FvmFieldReduceSum (src/fvmops.cu:369-383): sum over interior cells, device reduction in

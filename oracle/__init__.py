@@ -106,6 +106,8 @@ def lib() -> C.CDLL:
         L.orc_euler_iterate.argtypes = [mp, C.POINTER(_OrcState), C.POINTER(_OrcBC), C.c_int,
                                         C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.c_double, _f64p, C.c_int]
+        L.orc_fvm_op.restype = C.c_int
+        L.orc_fvm_op.argtypes = [mp, C.c_int, _f64p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _f64p]
         L.orc_nsi_step.restype = C.c_int
         L.orc_nsi_step.argtypes = [mp, _f64p, _f64p, _f64p, C.c_void_p, C.c_int, C.POINTER(_NsiParams),
                                    C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -382,3 +384,26 @@ def nsi_poisson(m: OracleMesh, b, cg_max: int, cg_tol: float = 0.0):
     x = np.zeros(m.n_cells)
     its = lib().orc_nsi_poisson(m._p, b, x, cg_max, cg_tol)
     return x, its
+
+
+# ---- finite-volume operator library (oracle_ops.c) ----
+
+OP_LAPLACIAN_C, OP_SCAL_GRAD_C, OP_SCAL_GRAD_C_CORRECT, OP_VEC_GRAD_C, OP_BOUND_SCAL_GRAD_C, \
+    OP_BOUND_VEC_GRAD_C, OP_DIV_C, OP_CONV_CONST_SCAL_U, OP_CONV_CONST_SCAL_U2, OP_CONV_SCAL_U, OP_CONV_VEC_U = range(11)
+OP_OUT = {0: 1, 1: 3, 2: 3, 3: 9, 4: 3, 5: 9, 6: 1, 7: 1, 8: 1, 9: 1, 10: 3}
+
+
+def fvm_op(m: OracleMesh, op: int, val, grad=None, lim=None, phi_const=None, phi_field=None) -> np.ndarray:
+    """Operator `op` of every interior cell: (n_cells,) or (n_cells, 3) or (n_cells, 9)."""
+    def ptr(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, np.float64).reshape(-1)
+        keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+    keep = []
+    v = np.ascontiguousarray(val, np.float64).reshape(-1)
+    out = np.zeros(m.n_cells * OP_OUT[op])
+    if lib().orc_fvm_op(m._p, op, v, ptr(grad), ptr(lim), ptr(phi_const), ptr(phi_field), out):
+        raise ValueError("unknown operator")
+    return out.reshape(m.n_cells, -1).squeeze() if OP_OUT[op] > 1 else out

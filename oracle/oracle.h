@@ -170,6 +170,25 @@ int orc_mlp_forward(const double *params, int n_in, int n_out, const double *x, 
 int orc_mlp_loss_grad(const double *params, int n_in, int n_out, const double *x, const double *t, int M,
                       int emulate_bf16, double *loss, double *grad);
 
+/* Finite-volume operator library (oracle_ops.c), include/fvmops.cuh:96-219.  Inputs in the
+ * AoS layouts above (val/grad/lim of the input field over N+G; phi_const a vector; phi_field a
+ * vector field over N+G); out: 1, 3 or 9 values per interior cell. */
+enum {
+    ORC_OP_LAPLACIAN_C = 0,
+    ORC_OP_SCAL_GRAD_C = 1,
+    ORC_OP_SCAL_GRAD_C_CORRECT = 2,
+    ORC_OP_VEC_GRAD_C = 3,
+    ORC_OP_BOUND_SCAL_GRAD_C = 4,
+    ORC_OP_BOUND_VEC_GRAD_C = 5,
+    ORC_OP_DIV_C = 6,
+    ORC_OP_CONV_CONST_SCAL_U = 7,
+    ORC_OP_CONV_CONST_SCAL_U2 = 8,
+    ORC_OP_CONV_SCAL_U = 9,
+    ORC_OP_CONV_VEC_U = 10
+};
+int orc_fvm_op(const orc_mesh *m, int op, const double *val, const double *grad, const double *lim,
+               const double *phi_const, const double *phi_field, double *out);
+
 /* Incompressible pressure-correction step (oracle_nsi.c); no reference arithmetic exists.
  * u: 3(N+G) AoS velocity, p: N+G pressure, F: n_faces volume fluxes from the face's
  * lower-numbered cell to the higher.  Walls: no slip with the region's velocity (regions not
