@@ -774,3 +774,25 @@ class Nsi:
         it = (C.c_int * max(n, 1))()
         _check(load().gr_nsi_history(self._h, _p(c, C.c_double), it, n))
         return c[:n], np.array(it[:n], dtype=int)
+
+
+# ---- case files and VTU output ----------------------------------------------------------------
+
+def write_vtu(mesh: Mesh, path: str, fields: dict) -> None:
+    """ASCII VTU of the interior cells (polyhedra) with cell data; fields: name -> array of
+    shape (n, ), (n, 3) or (n, 9) with n >= n_cells."""
+    names = (C.c_char_p * len(fields))(*[k.encode() for k in fields])
+    arrs = [np.ascontiguousarray(v, np.float64) for v in fields.values()]
+    nc = np.array([1 if a.ndim == 1 else a.shape[1] for a in arrs], np.uint32)
+    vals = (C.POINTER(C.c_double) * len(arrs))(*[_p(a.reshape(-1), C.c_double) for a in arrs])
+    _check(load().gr_write_vtu(mesh._h, path.encode(), len(arrs), names, _p(nc, C.c_uint32), vals))
+
+
+def check_case(path: str) -> None:
+    _check(load().gr_case_check(path.encode()))
+
+
+def run_case(path: str, verbose: bool = False) -> SolveResult:
+    r = SolveResult()
+    _check(load().gr_case_run(path.encode(), int(verbose), C.byref(r)))
+    return r

@@ -141,6 +141,9 @@ SIGNATURES = {
     "gr_solver_force": (_i, [_vp, _s, _dp]),
     "gr_batch_force": (_i, [_vp, _s, _dp]),
     "gr_mesh_wall_distance": (_i, [_vp, C.POINTER(C.c_char_p), _u32, _dp]),
+    "gr_write_vtu": (_i, [_vp, _s, _u32, C.POINTER(C.c_char_p), _u32p, C.POINTER(_dp)]),
+    "gr_case_check": (_i, [_s]),
+    "gr_case_run": (_i, [_s, _i, C.POINTER(SolveResult)]),
     "gr_nsi_create": (_i, [_vp, C.POINTER(NsiConfig), C.POINTER(_vp)]),
     "gr_nsi_free": (None, [_vp]),
     "gr_nsi_set_wall": (_i, [_vp, _s, _dp]),

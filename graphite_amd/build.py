@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgraphite_hip.so")
+CLI = os.path.join(HERE, "graphite")                 # the case-file driver executable
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
@@ -56,6 +58,9 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + [o for _, o in jobs], verbose)
     os.replace(tmp, LIB)
+    _run(["g++", os.path.join(HERE, "cli", "graphite_main.cpp"), "-o", CLI + ".tmp", f"-I{INCLUDE}"] + CXX_FLAGS +
+         [f"-L{HERE}", "-lgraphite_hip", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

@@ -99,5 +99,7 @@ void gen_ogrid(uint32_t nt, uint32_t nr, uint32_t nz, double r0, double r1, doub
 void gen_cgrid(uint32_t ni, uint32_t nj, uint32_t nwake, double thick, double radius, double h0, double dz,
                gr_elements &e);
 void gen_wing(const gr_wing_params &p, gr_elements &e);
+// Host connectivity of a mesh handle (mesh_device.hip).
+const HostMesh &host_mesh(const gr_mesh *m);
 
 }  // namespace gr

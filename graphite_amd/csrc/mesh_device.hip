@@ -234,6 +234,10 @@ void gr_mesh::ensure_tiles(uint32_t max_slots, uint32_t max_cells) {
     GR_HIP(hipStreamSynchronize(stream));
 }
 
+namespace gr {
+const HostMesh &host_mesh(const gr_mesh *m) { return m->h; }
+}  // namespace gr
+
 gr::MeshView gr_mesh::view() const {
     gr::MeshView v;
     v.N = h.n_cells;

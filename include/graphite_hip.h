@@ -355,6 +355,21 @@ int gr_mesh_wall_distance(gr_mesh *m, const char *const *regions, uint32_t n_reg
  * Calibrates hardware byte counters and measures the achievable HBM rate. */
 int gr_stream_copy_probe(size_t bytes, int reps, double *gbps);
 
+/* ---- case driver and field output -----------------------------------------------------------
+ * ASCII VTU of the interior cells as polyhedra (VTK type 42: sorted unique nodes, face stream,
+ * face offsets) with cell data arrays, FieldsWriteVtu (src/fvmfieldsio.cu:5-287).  Field k has
+ * n_comp[k] (1, 3 or 9) values per cell in values[k] (at least n_cells*n_comp entries; ghost
+ * entries after them are ignored — the reference writes them past NumberOfCells). */
+int gr_write_vtu(const gr_mesh *m, const char *path, uint32_t n_fields, const char *const *names,
+                 const uint32_t *n_comp, const double *const *values);
+/* Runs a JSON case file as PhysicsEuler does (src/models/euler.cu:15-221): "mesh" (su2
+ * converted to .fvm beside it), "init" {mach{x,y,z}, pressure, temperature}, "bcs" {region:
+ * {type slip | farfield (+ mach, pressure, temperature)}}, "controls" {cfl, time "steady",
+ * tolerance; optional max_iterations, precision "f64" | "f32", renumber}, optional "save"
+ * {file} (VTU).  Every key is checked before any device work; gr_case_check only checks. */
+int gr_case_check(const char *case_path);
+int gr_case_run(const char *case_path, int verbose, gr_solve_result *result);
+
 /* ---- incompressible pressure-correction solver ----------------------------------------------
  * Replaces the reference's PhysicsNSI (include/models/nsi.cuh:8-40), a stub whose
  * src/models/nsi.cu does not compile and holds no arithmetic.  Collocated cell-centred
