@@ -1,16 +1,19 @@
 """Benchmark of the second-order FV residual path on one MI355X.
 
-Workload: 3-D Euler flow past a cylinder, O-grid hexes 256 (circumferential) x 128 (radial,
-geometric ratio 1.05 from r=0.5 to r=10) x 4 (span, dz = 2*pi*0.5/256) = 131,072 cells,
-M = 0.3, second order with the reference limiter, cells renumbered by reverse Cuthill-McKee.
-A step is one solver iteration (boundary conditions, LSQ gradient + limiter, flux + local
-time-step update, residual norm) with the state resident in HBM.
+Workload of `value`: 3-D Euler flow past a cylinder, O-grid hexes 256 (circumferential) x 128
+(radial, geometric ratio 1.05 from r=0.5 to r=10) x 4 (span, dz = 2*pi*0.5/256) = 131,072
+cells, M = 0.3, second order with the reference limiter, cells renumbered by reverse
+Cuthill-McKee.  A step is one solver iteration (boundary conditions, LSQ gradient + limiter,
+flux + local time-step update, residual norm) with the state resident in HBM.
 
 Prints one JSON line: cell-updates/s of the binary64 solver (value), the roofline of its
 dominant kernel measured live with HIP events on the library's stream, the CPU oracle on the
-host cores, the mixed-precision throughput and the time to a 1e-3 residual drop.
+host cores, the mixed-precision throughput, the time to a 1e-3 residual drop, the batched
+C-grid throughput (64 flows per launch), the learned warm start against cold batched and cold
+single-flow solves, and the incompressible 64^3 cavity.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-converge]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-converge] [--no-batch]
+                  [--no-warm] [--no-nsi] [--warm-flows F] [--warm-steps S] [--nsi-steps T]
 """
 from __future__ import annotations
 
@@ -87,38 +90,20 @@ def cpu_baseline(gr, e, budget_s=15.0):
                        f"{threads} OpenMP threads, {t:.1f} s")
 
 
-CGRID_BCS = [("airfoil", "slip"), ("farfield", "farfield"), ("zmin", "slip"), ("zmax", "slip")]
-
-
-def cgrid_machs(n, seed=2024):
-    """Mach ~ U[0.5, 0.85], angle of attack ~ U[-2, 6] degrees (seeded)."""
-    rng = np.random.default_rng(seed)
-    M = rng.uniform(0.5, 0.85, n)
-    a = np.deg2rad(rng.uniform(-2.0, 6.0, n))
-    return np.stack([M * np.cos(a), M * np.sin(a), np.zeros(n)], axis=1)
-
-
-def make_batch(gr, m, dtype, cfl, machs):
-    b = gr.Batch(m, len(machs), dtype=dtype, cfl=cfl)
-    for name, kind in CGRID_BCS:
-        b.set_bc(name, gr.GR_BC_FARFIELD if kind == "farfield" else gr.GR_BC_SLIP)
-    b.set_freestream(machs, P_INF, T_INF)
-    return b
-
-
 def batched_leg(gr, args):
     """64 instances on the NACA0012 C-grid (384x96x1 = 36,864 hexes) per launch."""
+    from graphite_amd import warmstart as ws
     log("bench: batched C-grid leg")
     m = gr.Mesh.build(gr.Elements.cgrid_naca())
     m.renumber_rcm()
     m.compute_metrics()
     s = m.sizes()
-    machs = cgrid_machs(64)
+    machs = ws.sample_instances(64, seed=2024)
     res = {"workload": "NACA0012 C-grid 384x96x1 (36,864 hexes) x 64 instances, Mach U[0.5,0.85], "
                        "AoA U[-2,6] deg", "instances": 64, "cells": s.n_cells}
     steps = max(20, args.steps // 4)
     for name, dtype in (("f64", gr.GR_F64), ("mixed", gr.GR_F32)):
-        b = make_batch(gr, m, dtype, args.cfl, machs)
+        b = ws.make_batch(m, machs, dtype, args.cfl)
         b.iterate(args.warmup)
         t0 = time.perf_counter()
         b.iterate(steps)
@@ -128,21 +113,119 @@ def batched_leg(gr, args):
         res[name] = {"value": s.n_cells * 64 * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / steps * 1e3,
                      "kernels": {k: {"ms": prof[k], "bytes": kb[k], "GBps": kb[k] / (prof[k] * 1e-3) / 1e9}
                                  for k in prof}}
-    if not args.no_converge:
-        b = make_batch(gr, m, gr.GR_F64, args.cfl, machs)
-        t0 = time.perf_counter()
-        its, conv, ran = b.iterate(20000, tol=1e-3)
-        wall = time.perf_counter() - t0
-        per_iter = wall / max(ran, 1)
-        amort = its * per_iter / 64.0
-        res["time_to_1e-3_f64_cold"] = {"batch_seconds": wall, "batch_iterations": int(ran),
-                                        "converged": int(conv.sum()),
-                                        "iterations_p50": float(np.percentile(its, 50)),
-                                        "iterations_p99": float(np.percentile(its, 99)),
-                                        "seconds_per_instance_p50": float(np.percentile(amort, 50)),
-                                        "seconds_per_instance_p99": float(np.percentile(amort, 99)),
-                                        "normalisation": "residual of iteration 0 per instance"}
     return res
+
+
+def warm_start_leg(gr, args):
+    """Learned warm start on the C-grid: 512 training flows generated by the batched solver,
+    BF16-MFMA training, then 64 new flows to 1e-3 of their freestream residual: warm-started
+    batched vs cold batched vs cold single-flow solves; lift/drag after deeper convergence."""
+    from graphite_amd import warmstart as ws
+    log("bench: warm-start leg (training set)")
+    m = gr.Mesh.build(gr.Elements.cgrid_naca())
+    m.renumber_rcm()
+    m.compute_metrics()
+    wd = ws.wall_distance(m)
+    N = m.n_cells
+    t0 = time.perf_counter()
+    n_flows = args.warm_flows
+    x, t, origin, _ = ws.generate_training_set(m, ws.sample_instances(n_flows, seed=1), tol=1e-5, max_iter=6000,
+                                               dtype=gr.GR_F32)
+    t_data = time.perf_counter() - t0
+    # oversample rows of cells within half a chord of the wall (x3)
+    rows, per = [], 64
+    for b0 in range(0, n_flows, per):
+        nb = min(per, n_flows - b0)
+        base = b0 * N
+        cells = np.repeat(np.arange(N), nb)
+        r = base + np.arange(N * nb)
+        near = r[wd[cells] < 0.5]
+        rows += [r, near, near]
+    idx = np.concatenate(rows)
+    log("bench: warm-start leg (training)")
+    net, norm, losses, t_train = ws.train(x[idx], t[idx], origin, steps=args.warm_steps, batch=32768, lr=2e-3,
+                                          lr_final=2e-5)
+    test = ws.sample_instances(64, seed=99)
+    log("bench: warm-start leg (cold and warm solves)")
+    cold = ws.make_batch(m, test)
+    fs = cold.residual_norms()
+    cs = ws.solve(cold, 1e-3, fs, 20000)
+    warm = ws.make_batch(m, test)
+    t0 = time.perf_counter()
+    _, used = warm.warm_start(net, wd, norm)
+    t_pred = time.perf_counter() - t0
+    wsol = ws.solve(warm, 1e-3, fs, 20000)
+    t_warm = t_pred + wsol.seconds
+    # cold single-flow solves (the plain solver, one flow at a time)
+    plain = []
+    for k in range(64):
+        sv = gr.Solver(m, gr.GR_F64, cfl=args.cfl)
+        for name, kind in ws.CGRID_BCS:
+            sv.set_bc(name, kind, test[k], ws.P_INF, ws.T_INF)
+        sv.init_freestream(test[k], ws.P_INF, ws.T_INF)
+        t1 = time.perf_counter()
+        r = sv.iterate(20000, tol=1e-3)
+        plain.append((time.perf_counter() - t1, r.iterations, bool(r.converged)))
+    t_plain = sum(p[0] for p in plain)
+    s_ref = 0.1                                     # chord 1 x extrusion depth 0.1
+
+    def coeffs(b):
+        F = b.force("airfoil")
+        return np.array([gr.lift_drag(F[k], test[k], 1.4, ws.P_INF, s_ref) for k in range(64)])
+    c3, w3 = coeffs(cold), coeffs(warm)
+    log("bench: warm-start leg (deeper convergence for lift/drag)")
+    cd = ws.solve(cold, 1e-5, fs, 20000)
+    wd5 = ws.solve(warm, 1e-5, fs, 20000)
+    c5, w5 = coeffs(cold), coeffs(warm)
+    both = cd.converged & wd5.converged
+    rel = lambda a, b: (np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)).tolist()
+    it_w, it_c = wsol.iterations, cs.iterations
+    return {
+        "workload": "NACA0012 C-grid 384x96x1, 64 new flows (Mach U[0.5,0.85], AoA U[-2,6] deg), binary64, "
+                    "to 1e-3 of each flow's freestream residual",
+        "training": {"flows": n_flows, "data_seconds": t_data, "rows": int(len(idx)), "steps": args.warm_steps,
+                     "batch": 32768,
+                     "train_seconds": t_train, "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                     "network": "4 layers 6-256-256-256-5, tanh, bf16 MFMA (v_mfma_f32_16x16x32_bf16), fp32 accumulation"},
+        "predictions_kept": int(used.sum()),
+        "prediction_seconds": t_pred,
+        "seconds": {"warm_batched": t_warm, "cold_batched": cs.seconds, "cold_single_flows": t_plain},
+        "speedup_vs_cold_single_flows": t_plain / t_warm,
+        "speedup_vs_cold_batched": cs.seconds / t_warm,
+        "iterations": {"warm_p50": float(np.percentile(it_w, 50)), "warm_p99": float(np.percentile(it_w, 99)),
+                       "cold_p50": float(np.percentile(it_c, 50)), "cold_p99": float(np.percentile(it_c, 99)),
+                       "warm_batch": wsol.batch_iterations, "cold_batch": cs.batch_iterations},
+        "seconds_per_flow": {"cold_single_p50": float(np.percentile([p[0] for p in plain], 50)),
+                             "cold_single_p99": float(np.percentile([p[0] for p in plain], 99)),
+                             "warm_batched_amortised": t_warm / 64, "cold_batched_amortised": cs.seconds / 64},
+        "converged": {"warm": int(wsol.converged.sum()), "cold": int(cs.converged.sum()),
+                      "single": int(sum(p[2] for p in plain))},
+        "lift_drag": {"s_ref": s_ref,
+                      "rel_diff_at_1e-3_warm_vs_cold": rel(w3, c3),
+                      "rel_diff_at_1e-5_warm_vs_cold": rel(w5[both], c5[both]) if both.any() else None,
+                      "flows_converged_to_1e-5": int(both.sum()),
+                      "cold_1e-3_vs_1e-5": rel(c3, c5), "warm_1e-3_vs_1e-5": rel(w3, w5)},
+        "normalisation": "residual of the freestream state of each flow, for warm and cold starts alike",
+    }
+
+
+def nsi_leg(gr, args):
+    """Lid-driven cavity 64^3 (262,144 hexes), Re 400, binary32 bulk with binary64 reductions."""
+    log("bench: incompressible cavity leg")
+    n = 64
+    m = gr.Mesh.build(gr.Elements.box(n, n, n))
+    m.renumber_rcm()
+    m.compute_metrics()
+    ns = gr.Nsi(m, nu=1.0 / 400.0, dt=0.005, dtype=gr.GR_F32, cg_max=400, cg_tol=1e-4)
+    ns.set_wall("ymax", (1.0, 0.0, 0.0))
+    ns.step(5)
+    t0 = time.perf_counter()
+    c, its = ns.step(args.nsi_steps)
+    dt = time.perf_counter() - t0
+    return {"workload": "lid-driven cavity 64^3 hexes, Re 400, dt 0.005, Jacobi-PCG to 1e-4 relative",
+            "cells": m.n_cells, "steps": args.nsi_steps, "seconds": dt,
+            "steps_per_second": args.nsi_steps / dt, "cell_updates_per_second": m.n_cells * args.nsi_steps / dt,
+            "cg_iterations_mean": float(np.mean(its)), "continuity_last": float(c[-1]), "dtype": "f32 bulk, f64 dots"}
 
 
 def main():
@@ -155,6 +238,11 @@ def main():
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=100)
+    ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-nsi", action="store_true")
+    ap.add_argument("--nsi-steps", type=int, default=200)
+    ap.add_argument("--warm-flows", type=int, default=512)
+    ap.add_argument("--warm-steps", type=int, default=8000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -228,6 +316,10 @@ def main():
 
     if not args.no_batch:
         out["batched"] = batched_leg(gr, args)
+    if not args.no_warm:
+        out["warm_start"] = warm_start_leg(gr, args)
+    if not args.no_nsi:
+        out["nsi_cavity"] = nsi_leg(gr, args)
 
     if not args.no_converge:
         log("bench: time to a 1e-3 residual drop (binary64, cold start)")

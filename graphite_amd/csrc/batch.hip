@@ -48,9 +48,11 @@ struct BatchArgs {
     size_t stride;
 };
 
-// Cell of this wave (wave-uniform, in a scalar register) and this lane's instance.
+// Cell of this wave (wave-uniform, in a scalar register).  Blocks are mapped XCD-aware: the
+// blocks one XCD runs (block index mod 8) take a contiguous range of cells, so the neighbour
+// rows a cell re-reads are in that XCD's L2.
 __device__ inline uint32_t batch_cell() {
-    return __builtin_amdgcn_readfirstlane(blockIdx.x * kCellsPerBlock + (threadIdx.x >> 6));
+    return __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * kCellsPerBlock + (threadIdx.x >> 6));
 }
 
 // Far-field region uses the instance's Mach vector: copy the region with it substituted.

@@ -139,3 +139,38 @@ def test_bulk_states_round_trip(gr, small):
     N = m.n_cells
     for x, y in zip(got, new):
         assert np.array_equal(x[:, :N], y[:, :N])
+
+
+def test_trained_network_cuts_iterations_and_keeps_lift_drag(gr, small):
+    """Train on 64 solver-generated subsonic flows of the coarse C-grid, then warm-start 16 new
+    flows: the warm start reaches 1e-3 of the freestream residual in far fewer iterations than
+    freestream starts, and once both are converged their lift and drag agree to 1e-4."""
+    from graphite_amd import warmstart as ws
+    m = small
+    rng = np.random.default_rng(42)
+
+    def flows(n):
+        M = rng.uniform(0.3, 0.5, n)
+        a = np.deg2rad(rng.uniform(-2.0, 6.0, n))
+        return np.stack([M * np.cos(a), M * np.sin(a), np.zeros(n)], axis=1)
+
+    x, t, origin, _ = ws.generate_training_set(m, flows(64), tol=1e-7, max_iter=30000)
+    net, norm, losses, _ = ws.train(x, t, origin, steps=4000, batch=16384)
+    assert losses[-1] < 0.05 * losses[0]
+    test = flows(16)
+    cs, wsx, _, used, (cold, warm), fs = ws.cold_and_warm(m, net, norm, test, tol=1e-3, max_iter=30000)
+    print("iterations to 1e-3: cold p50", np.median(cs.iterations), "warm p50", np.median(wsx.iterations),
+          "predictions kept", used.sum())
+    assert used.sum() >= 12
+    assert np.median(wsx.iterations) < 0.5 * np.median(cs.iterations)
+    # converge both deeply, then compare the forces
+    a = ws.solve(cold, 1e-9, fs, 60000)
+    b = ws.solve(warm, 1e-9, fs, 60000)
+    done = a.converged & b.converged
+    assert done.sum() >= 12
+    Fc, Fw = cold.force("airfoil")[done], warm.force("airfoil")[done]
+    lc = np.array([gr.lift_drag(Fc[k], test[done][k], RHO_INF, 1.0, 0.1) for k in range(done.sum())])
+    lw = np.array([gr.lift_drag(Fw[k], test[done][k], RHO_INF, 1.0, 0.1) for k in range(done.sum())])
+    scale = np.abs(lc).max(axis=0)
+    print("max |CL, CD difference| / max |CL, CD|:", np.abs(lw - lc).max(axis=0) / scale)
+    assert np.all(np.abs(lw - lc).max(axis=0) <= 1e-4 * scale)
