@@ -109,12 +109,12 @@ class Elements:
         return Elements(h)
 
     @staticmethod
-    def wing(n_around=160, n_prism=20, n_tet=23, n_span=24, n_cap=14, thickness=0.12, sweep_deg=30.0,
-             taper=0.56, semispan=1.5, first_height=1e-3, growth=1.2, radius=25.0):
-        """Swept NACA wing, prism layers + tetrahedra, half-sphere far field (about 1.1 M cells
-        with the defaults); markers wall, farfield, symmetry."""
+    def wing(n_around=88, n_prism=20, n_tet=70, n_span=20, n_cap=16, thickness=0.12, sweep_deg=30.0,
+             taper=0.56, semispan=1.5, first_height=3e-3, growth=1.2, radius=25.0, scale=1.0):
+        """Swept NACA wing, prism layers + tetrahedra, half-sphere far field (1.07 M cells with
+        the defaults); markers wall, farfield, symmetry.  Lengths in root chords times scale."""
         prm = WingParams(n_around, n_prism, n_tet, n_span, n_cap, thickness, sweep_deg, taper, semispan,
-                         first_height, growth, radius)
+                         first_height, growth, radius, scale)
         h = C.c_void_p()
         _check(load().gr_gen_wing(C.byref(prm), C.byref(h)))
         return Elements(h)

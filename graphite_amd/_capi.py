@@ -48,7 +48,7 @@ class SolveResult(C.Structure):
 class WingParams(C.Structure):
     _fields_ = [("n_around", _u32), ("n_prism", _u32), ("n_tet", _u32), ("n_span", _u32), ("n_cap", _u32),
                 ("thickness", _d), ("sweep_deg", _d), ("taper", _d), ("semispan", _d),
-                ("first_height", _d), ("growth", _d), ("radius", _d)]
+                ("first_height", _d), ("growth", _d), ("radius", _d), ("scale", _d)]
 
 
 class NsiConfig(C.Structure):

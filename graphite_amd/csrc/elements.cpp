@@ -262,7 +262,7 @@ static double naca_half_thickness(double x, double t) {
 // Ratio r with h0 * (r^n - 1)/(r - 1) = 1 (geometric spacing from h0 over unit length).
 static double stretch_ratio(double h0, uint32_t n) {
     if (h0 * n >= 1.0) return 1.0;
-    double lo = 1.0 + 1e-12, hi = 2.0;
+    double lo = 1.0 + 1e-12, hi = 4.0;
     for (int it = 0; it < 200; ++it) {
         double r = 0.5 * (lo + hi);
         double len = h0 * (std::pow(r, (double)n) - 1.0) / (r - 1.0);
@@ -413,7 +413,13 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             "wing: bad thickness, taper or semispan");
     require(p.sweep_deg > -60 && p.sweep_deg < 60, "wing: sweep must be within +-60 degrees");
     require(p.first_height > 0 && p.growth >= 1 && p.radius > 4 * (p.semispan + 1), "wing: bad layers or radius");
+    require(p.scale > 0 && std::isfinite(p.scale), "wing: scale must be positive");
     const uint32_t nj = np + nt, K = ns + nc;
+    // Stations above the tip over which the section's thickness tapers to the zero-thickness
+    // plate: the tip is capped by a short wedge rather than one steep ridge, whose faces (the
+    // section at the tip station against the plate one station up) twist strongly where the
+    // thickness changes fast, at the leading edge.
+    const uint32_t n_taper = std::max(1u, std::min(4u, nc / 3));
     const uint64_t n_grid = (uint64_t)ni * (nj + 1) * (K + 1);
     require(n_grid * 8 < 0xFFFFFFF0ull, "wing: mesh too large for 32-bit indices");
     const double pi = 3.14159265358979323846;
@@ -446,12 +452,15 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
     // wing stations
     // Far-field latitudes: the wing stations' radial lines end on the sphere at latitudes
     // spread evenly over [0, phi_b] (not at the station's own height), so cells near the far
-    // field are not flattened to the station spacing; the stations above the tip continue to
-    // phi_top.
-    const double phi_b = 20.0 * pi / 180.0, phi_top = 0.25 * pi;
+    // field are not flattened to the station spacing; the stations above the tip continue at
+    // the same latitude step up to phi_top.  Station heights are half uniform, half clustered
+    // towards the tip, so the tip spacing stays within a factor two of the mean.
+    const double phi_b = 60.0 * pi / 180.0;
+    const double phi_top = std::min(85.0 * pi / 180.0, phi_b * (1.0 + (double)nc / ns));
     require(R * std::sin(phi_b) > 2 * b, "wing: far field too close to the tip");
+    auto station_z = [&](uint32_t k) { return b * (0.5 * k / ns + 0.5 * std::sin(0.5 * pi * k / ns)); };
     for (uint32_t k = 0; k <= ns; ++k) {
-        const double z = b * std::sin(0.5 * pi * k / ns);
+        const double z = station_z(k);
         const double c = 1.0 - (1.0 - p.taper) * z / b, xle = z * tsw;
         const double phi_k = phi_b * k / ns;
         const double rho = R * std::cos(phi_k), zq = R * std::sin(phi_k);
@@ -480,15 +489,34 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             nys.swap(ay);
         }
         for (uint32_t i = 0; i < ni; ++i) {
-            const double nx = nxs[i], ny = nys[i];
-            for (uint32_t j = 0; j <= np; ++j) X[nid(i, j, k)] = {P[i].x + d[j] * nx, P[i].y + d[j] * ny, z};
+            // Off the symmetry plane the layers leave the surface along its normal in 3D: the
+            // section normal made orthogonal to the spanwise tangent dP/dz (sweep and taper).  A
+            // wall cell whose centroid sits off its wall face's normal line lets mass through a
+            // slip wall (the ghost takes the cell's unmirrored gradient), which on the swept
+            // leading edge fed a growing density spike.
+            double nx = nxs[i], ny = nys[i], nz = 0.0;
+            if (k > 0) {
+                const double tx = tsw - (1.0 - p.taper) / b * ux[i], ty = -(1.0 - p.taper) / b * uy[i], tz = 1.0;
+                const double a = (nx * tx + ny * ty) / (tx * tx + ty * ty + tz * tz);
+                nx -= a * tx;
+                ny -= a * ty;
+                nz = -a * tz;
+                // no wall-layer node may drop below the mid-height of the previous station
+                // (near the root the tilt would take trailing-edge layers under the symmetry plane)
+                const double lim = 0.5 * (z - station_z(k - 1)) / d[np] * std::sqrt(nx * nx + ny * ny + nz * nz);
+                nz = std::max(-lim, std::min(lim, nz));
+                const double l = std::sqrt(nx * nx + ny * ny + nz * nz);
+                nx /= l;
+                ny /= l;
+                nz /= l;
+            }
+            for (uint32_t j = 0; j <= np; ++j) X[nid(i, j, k)] = {P[i].x + d[j] * nx, P[i].y + d[j] * ny, z + d[j] * nz};
             const V3 Pp = X[nid(i, np, k)];
             const V3 Q = {cx0 + rho * std::cos(th[i]), cy0 + rho * std::sin(th[i]), zq};
-            const double L = std::sqrt((Q.x - Pp.x) * (Q.x - Pp.x) + (Q.y - Pp.y) * (Q.y - Pp.y) + (Q.z - z) * (Q.z - z));
+            const double L = std::sqrt((Q.x - Pp.x) * (Q.x - Pp.x) + (Q.y - Pp.y) * (Q.y - Pp.y) + (Q.z - Pp.z) * (Q.z - Pp.z));
             require(L > 2 * h_tet, "wing: far field too close to the prism layers");
             const double r = stretch_ratio(h_tet / L, nt);
-            double acc[64 + 1];
-            require(nt <= 64, "wing: at most 64 tetrahedral layers");
+            std::vector<double> acc(nt + 1);
             {
                 double h = h_tet / L;
                 acc[0] = 0;
@@ -499,7 +527,7 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             }
             for (uint32_t j = 1; j <= nt; ++j) {
                 const double s = j == nt ? 1.0 : acc[j] / acc[nt];
-                X[nid(i, np + j, k)] = {Pp.x + s * (Q.x - Pp.x), Pp.y + s * (Q.y - Pp.y), z + s * (Q.z - z)};
+                X[nid(i, np + j, k)] = {Pp.x + s * (Q.x - Pp.x), Pp.y + s * (Q.y - Pp.y), Pp.z + s * (Q.z - Pp.z)};
             }
         }
     }
@@ -514,28 +542,33 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
     {
         const double c_tip = p.taper, xm = b * tsw + 0.5 * c_tip;
         const double rho_top = R * std::cos(phi_top);
-        const double ae = 0.25 * rho_top, kCapFirst = 0.01 * rho_top;
+        // Top layout in elliptic coordinates about the plate: x = xm + ae cosh(mu) cos(nu),
+        // y = -ae sinh(mu) sin(nu), nu = the section parameter of node i, mu = j dmu.  The map is
+        // conformal, so with dmu equal to the mean step in nu the cells are close to square
+        // everywhere, around the plate's ends too; the plate's half width ae follows from that
+        // (at least half the tip chord).  The outermost ellipse is bent onto the circle of
+        // latitude phi_top, the correction growing as cosh(mu) - 1: the rings' spacing next to the
+        // plate's ends grows that way, so the correction never turns them back.
+        const double ae = std::max(0.5 * c_tip, rho_top / std::cosh(2.0 * pi * nj / ni));
+        require(rho_top > 2 * ae, "wing: far field too close to the tip");
+        const double mu_max = std::acosh(rho_top / ae);
+        std::vector<double> t(nj + 1, 0.0);
+        for (uint32_t j = 0; j <= nj; ++j) t[j] = ae * std::sinh(mu_max * j / nj);
         std::vector<V3> T((size_t)ni * (nj + 1));
         for (uint32_t i = 0; i < ni; ++i) {
-            const double cxp = xm + ae * std::cos(beta_of[i]), cyp = 0.0;
+            const double nu = beta_of[i];
             const double ox = cx0 + rho_top * std::cos(th[i]), oy = cy0 + rho_top * std::sin(th[i]);
-            const double L = std::sqrt((ox - cxp) * (ox - cxp) + (oy - cyp) * (oy - cyp));
-            require(L > 4 * kCapFirst, "wing: far field too close to the tip");
-            const double r = stretch_ratio(kCapFirst / L, nj);
-            std::vector<double> acc(nj + 1, 0.0);
-            double h = kCapFirst / L;
-            for (uint32_t j = 1; j <= nj; ++j) {
-                acc[j] = acc[j - 1] + h;
-                h *= r;
-            }
+            const double ex = xm + ae * std::cosh(mu_max) * std::cos(nu), ey = -ae * std::sinh(mu_max) * std::sin(nu);
             for (uint32_t j = 0; j <= nj; ++j) {
-                const double s = j == nj ? 1.0 : acc[j] / acc[nj];
-                const double x = cxp + s * (ox - cxp), y = cyp + s * (oy - cyp);
+                const double mu = mu_max * j / nj, w = (std::cosh(mu) - 1.0) / (std::cosh(mu_max) - 1.0);
+                const double x = xm + ae * std::cosh(mu) * std::cos(nu) + w * (ox - ex);
+                const double y = (j == 0 ? 0.0 : -ae * std::sinh(mu) * std::sin(nu)) + w * (oy - ey);
                 const double rr = (x - cx0) * (x - cx0) + (y - cy0) * (y - cy0);
                 T[(size_t)j * ni + i] = {x, y, j == nj ? R * std::sin(phi_top) : std::sqrt(std::max(0.0, R * R - rr))};
             }
         }
-        const double dz_tip = b * (1.0 - std::cos(0.5 * pi / ns));
+        // cap heights: geometric from the last span step up to the dome
+        const double dz_tip = station_z(ns) - station_z(ns - 1);
         const double H = R - b;
         const double r = stretch_ratio(dz_tip / H, nc);
         std::vector<double> sq(nc + 1, 0.0);
@@ -548,29 +581,59 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             }
             for (uint32_t q = 1; q <= nc; ++q) sq[q] /= sq[nc];
         }
-        for (uint32_t j = 0; j <= nj; ++j)
+        // Station q above the tip.  Ring j's in-plane layout is B + s (T - B), B the tip station,
+        // T the top layout, with s moving from the geometric cap fraction sq[q] at the plate (in
+        // step with the heights, so no station slides over the one below) to s_out(q) at the
+        // far-field ring, where s_out puts that ring (the blend of B's and T's
+        // circles) on the circle of latitude phi_b + (phi_top - phi_b) q / n_cap.  The weight
+        // w_j is ring j's mean fraction of the path from the section to the far field on the tip
+        // station, the same for every node of a ring, so every ring is a convex blend of two
+        // valid layouts with one weight and the lines across rings do not fold.  Heights rise
+        // from the tip station's by the geometric cap increment at the plate and by the
+        // latitude's increment at the far field, blended with w_j.  The top station keeps T, on
+        // the sphere.
+        std::vector<double> wj(nj + 1, 0.0);
+        for (uint32_t i = 0; i < ni; ++i) {
+            double acc = 0;
+            std::vector<double> l(nj + 1, 0.0);
+            for (uint32_t j = 1; j <= nj; ++j) {
+                const V3 a0 = X[nid(i, j - 1, ns)], a1 = X[nid(i, j, ns)];
+                acc += std::hypot(a1.x - a0.x, a1.y - a0.y);
+                l[j] = acc;
+            }
+            for (uint32_t j = 1; j <= nj; ++j) wj[j] += l[j] / acc / ni;
+        }
+        wj[nj] = 1.0;
+        const double rho_b = R * std::cos(phi_b);
+        for (uint32_t q = 1; q < nc; ++q) {
+            const double phi = phi_b + (phi_top - phi_b) * q / nc;
+            const double s_out = (rho_b - R * std::cos(phi)) / (rho_b - rho_top), fz = R * std::sin(phi);
+            // the section's thickness tapers to the plate over the first n_taper stations
+            const double taper = q < n_taper ? 1.0 - (double)q / n_taper : 0.0;
             for (uint32_t i = 0; i < ni; ++i) {
-                const V3 B = X[nid(i, j, ns)], D = T[(size_t)j * ni + i];
-                for (uint32_t q = 1; q <= nc; ++q) {
-                    const double s = sq[q];
-                    V3 Y;
-                    if (j == nj) {
-                        const double phi = phi_b + s * (phi_top - phi_b);
-                        Y = {cx0 + R * std::cos(phi) * std::cos(th[i]), cy0 + R * std::cos(phi) * std::sin(th[i]),
-                             R * std::sin(phi)};
-                    } else if (j == 0) {
-                        // on the plate: upper node i coincides with lower node ni - i
-                        const uint32_t il = 2 * i > ni ? ni - i : i;
-                        const V3 Bl = X[nid(il, 0, ns)], Dl = T[il];
-                        Y = {Bl.x + s * (Dl.x - Bl.x), 0.0, Bl.z + s * (Dl.z - Bl.z)};
-                    } else {
-                        Y = {B.x + s * (D.x - B.x), B.y + s * (D.y - B.y), B.z + s * (D.z - B.z)};
-                    }
-                    X[nid(i, j, ns + q)] = Y;
+                // on the plate, upper node i coincides with lower node ni - i
+                const uint32_t il = 2 * i > ni && q >= n_taper ? ni - i : i;
+                const V3 Bl = X[nid(il, 0, ns)], Dl = T[il];
+                const double zp = Bl.z + sq[q] * (Dl.z - Bl.z);
+                for (uint32_t j = 0; j <= nj; ++j) {
+                    const uint32_t ii = j == 0 ? il : i;
+                    const V3 B = X[nid(ii, j, ns)], D = T[(size_t)j * ni + ii];
+                    const double w = wj[j], sj = (1.0 - w) * sq[q] + w * s_out;
+                    const double y = j == 0 ? taper * (1.0 - sj) * B.y : B.y + sj * (D.y - B.y);
+                    // height: the tip station's plus an increment blended from the plate's to
+                    // the far-field ring's, but not below the plate's height: the stations are
+                    // level around the plate, where the cells are tall and narrow (a sloping
+                    // station there feeds the vertical gradient from the in-plane neighbours).
+                    // Both terms rise with q, so the stations never cross.
+                    const double dz = (1.0 - w) * (zp - Bl.z) + w * (fz - R * std::sin(phi_b));
+                    X[nid(i, j, ns + q)] = {B.x + sj * (D.x - B.x), y, std::max(zp, B.z + dz)};
                 }
             }
+        }
+        for (uint32_t j = 0; j <= nj; ++j)
+            for (uint32_t i = 0; i < ni; ++i) X[nid(i, j, K)] = T[(size_t)j * ni + i];
     }
-    // node ids: upper plate nodes (j = 0, k > n_span, i > ni/2) are the lower ones
+    // node ids: upper plate nodes (j = 0, k >= n_span + n_taper, i > ni/2) are the lower ones
     std::vector<uint32_t> remap(n_grid), canon;
     canon.reserve(n_grid);
     e = gr_elements();
@@ -579,15 +642,15 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
         for (uint32_t j = 0; j <= nj; ++j)
             for (uint32_t i = 0; i < ni; ++i) {
                 const uint32_t g = nid(i, j, k);
-                if (j == 0 && k > ns && 2 * i > ni) {
+                if (j == 0 && k >= ns + n_taper && 2 * i > ni) {
                     remap[g] = remap[nid(ni - i, 0, k)];
                     continue;
                 }
                 remap[g] = (uint32_t)(e.xyz.size() / 3);
                 canon.push_back(g);
-                e.xyz.push_back(X[g].x);
-                e.xyz.push_back(X[g].y);
-                e.xyz.push_back(X[g].z);
+                e.xyz.push_back(X[g].x * p.scale);
+                e.xyz.push_back(X[g].y * p.scale);
+                e.xyz.push_back(X[g].z * p.scale);
             }
     auto pos = [&](uint32_t n) { return V3{e.xyz[3 * n], e.xyz[3 * n + 1], e.xyz[3 * n + 2]}; };
     e.starts.push_back(0);
@@ -613,16 +676,49 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
         const uint32_t w[6] = {a, bb, c, dd, ee, f};
         add(GR_WEDGE, w, 6);
     };
+    // pyramid: base quad (a,b,c,d) in cyclic order, apex e on the positive side
+    auto add_pyramid = [&](uint32_t a, uint32_t bb, uint32_t c, uint32_t dd, uint32_t ee) {
+        const V3 A = pos(a);
+        if (det3(sub(pos(bb), A), sub(pos(dd), A), sub(pos(ee), A)) < 0) std::swap(bb, dd);
+        const uint32_t w[5] = {a, bb, c, dd, ee};
+        add(GR_PYRAMID, w, 5);
+    };
+    // Cells of hex (i, j, k), corner[a + 2b + 4c] at offsets a, b, c along i, j, k.  A stretched
+    // triangle, in a prism or a tetrahedron, is what makes the unlimited second-order update of
+    // large cells unstable; a long prism over a compact triangle is not.
+    //  - Below the tip (k < n_span): wall layers (j < n_prism) of prisms extruded along j, their
+    //    triangles on constant-j faces split along (i, k) -> (i+1, k+1); outside them the
+    //    6-tetrahedron split about the corner-0 -> corner-7 diagonal, whose face diagonals match
+    //    the prisms'.  In the last wall-layer row below the cap (k = n_span - 1) the two
+    //    tetrahedra on the bottom face merge into a pyramid on the prism's quad below, the other
+    //    four give the cap above triangles.
+    //  - Above the tip (k >= n_span): prisms extruded along k (upwards), their triangles on
+    //    constant-k faces split along (i, j) -> (i+1, j+1) as the tetrahedra below.
+    // The slices i = n_around/2 - 1 and i = n_around - 1 are built mirrored in i (every diagonal
+    // flipped except those on constant-i faces, so they still meet their neighbours): above the
+    // tip their j = 0 cells have a plate end, the plate and the ring beyond that end on one line,
+    // and the unmirrored triangle there would have three collinear corners.
     for (uint32_t k = 0; k < K; ++k)
         for (uint32_t j = 0; j < nj; ++j)
             for (uint32_t i = 0; i < ni; ++i) {
-                // corner[a + 2b + 4c]: offsets along i, j, k
+                const int mirror = (i + 1 == ni / 2 || i + 1 == ni) ? 1 : 0;
                 uint32_t cr[8];
-                for (int o = 0; o < 8; ++o) cr[o] = remap[nid(i + (o & 1), j + ((o >> 1) & 1), k + ((o >> 2) & 1))];
-                if (j < np) {
-                    // constant-j triangles (i,k),(i+1,k),(i+1,k+1) and (i,k),(i+1,k+1),(i,k+1)
+                for (int o = 0; o < 8; ++o) {
+                    const int m = o ^ mirror;
+                    cr[o] = remap[nid(i + (m & 1), j + ((m >> 1) & 1), k + ((m >> 2) & 1))];
+                }
+                if (k >= ns) {
+                    add_prism(cr[0], cr[1], cr[3], cr[4], cr[5], cr[7]);
+                    add_prism(cr[0], cr[3], cr[2], cr[4], cr[7], cr[6]);
+                } else if (j < np && k + 1 < ns) {
                     add_prism(cr[0], cr[1], cr[5], cr[2], cr[3], cr[7]);
                     add_prism(cr[0], cr[5], cr[4], cr[2], cr[7], cr[6]);
+                } else if (j < np) {
+                    add_pyramid(cr[0], cr[1], cr[3], cr[2], cr[7]);
+                    add_tet(cr[0], cr[1], cr[5], cr[7]);
+                    add_tet(cr[0], cr[4], cr[5], cr[7]);
+                    add_tet(cr[0], cr[2], cr[6], cr[7]);
+                    add_tet(cr[0], cr[4], cr[6], cr[7]);
                 } else {
                     static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
                     for (const auto &pm : perms) {

@@ -98,13 +98,14 @@ int gr_gen_cgrid_naca(uint32_t n_i, uint32_t n_j, uint32_t n_wake, double thickn
  * at the origin, span along +z from the symmetry plane z = 0) in a half-sphere far field
  * centred on the root mid-chord.  n_prism wall layers of prisms (first_height, geometric
  * growth), then n_tet layers of tetrahedra out to the sphere; n_span stations along the wing
- * (clustered at the tip) and n_cap stations above the tip, where the section closes to a
- * zero-thickness plate (upper and lower nodes merged) and the tip is capped by a low ridge.
- * Markers wall (wing surface and tip ridge), farfield, symmetry.  Defaults giving about
- * 1.08 M cells: 160, 20, 23, 24, 14, 0.12, 30, 0.56, 1.5, 1e-3, 1.2, 25. */
+ * and n_cap stations above the tip, over which the section tapers to a zero-thickness plate
+ * (upper and lower nodes merged; above it the cells are prisms).  Every length is multiplied
+ * by scale at the end (the reference's limiter threshold is dimensional, so the scale changes
+ * which cells it limits).  Markers wall, farfield, symmetry.  Defaults giving 1,068,320 cells:
+ * 88, 20, 70, 20, 16, 0.12, 30, 0.56, 1.5, 3e-3, 1.2, 25, 1. */
 typedef struct gr_wing_params {
     uint32_t n_around, n_prism, n_tet, n_span, n_cap;
-    double thickness, sweep_deg, taper, semispan, first_height, growth, radius;
+    double thickness, sweep_deg, taper, semispan, first_height, growth, radius, scale;
 } gr_wing_params;
 int gr_gen_wing(const gr_wing_params *p, gr_elements **out);
 

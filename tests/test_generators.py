@@ -7,8 +7,9 @@ WING_SMALL = dict(n_around=32, n_prism=4, n_tet=5, n_span=6, n_cap=4, radius=12.
 
 
 def wing_counts(ni, npr, nt, ns, nc):
-    K = ns + nc
-    return 2 * ni * npr * K + 6 * ni * nt * K
+    """Below the tip: 2 prisms per wall-layer hex and 6 tetrahedra per outer hex, except the
+    last wall-layer row (a pyramid and 4 tetrahedra per hex); above the tip 2 prisms per hex."""
+    return (ns - 1) * ni * (2 * npr + 6 * nt) + ni * (5 * npr + 6 * nt) + nc * ni * 2 * (npr + nt)
 
 
 def test_wing_counts_and_markers(gr):
@@ -18,14 +19,14 @@ def test_wing_counts_and_markers(gr):
     assert s.n_cells == wing_counts(32, 4, 5, 6, 4)
     assert m.region_names() == ["wall", "farfield", "symmetry"]
     x, kinds, starts, nodes, markers = e.arrays()
-    assert set(np.unique(kinds)) == {gr.GR_TET, gr.GR_WEDGE}
+    assert set(np.unique(kinds)) == {gr.GR_TET, gr.GR_WEDGE, gr.GR_PYRAMID}
 
 
 def test_wing_default_size():
     import graphite_amd as gr
     gr.load()
     s = gr.Mesh.build(gr.Elements.wing()).sizes()
-    assert s.n_cells == wing_counts(160, 20, 23, 24, 14) == 1082240
+    assert s.n_cells == wing_counts(88, 20, 70, 20, 16) == 1068320
 
 
 def test_wing_geometry(gr, orc):
@@ -42,7 +43,7 @@ def test_wing_geometry(gr, orc):
     x, kinds, starts, nodes, _ = e.arrays()
     cfs, cf, cc = om.cell_faces_starts, om.cell_faces, om.cell_connects
     fn, fa, fc, xc = om.face_normals, om.face_areas, om.face_centers, om.cell_centers
-    worst = {gr.GR_TET: 0.0, gr.GR_WEDGE: 0.0}
+    worst = {gr.GR_TET: 0.0, gr.GR_WEDGE: 0.0, gr.GR_PYRAMID: 0.0}
     for i in range(N):
         acc = np.zeros(3)
         for q in range(cfs[i], cfs[i + 1]):
@@ -53,6 +54,7 @@ def test_wing_geometry(gr, orc):
         worst[int(kinds[i])] = max(worst[int(kinds[i])], r)
     assert worst[gr.GR_TET] < 1e-12
     assert worst[gr.GR_WEDGE] < 5e-2
+    assert worst[gr.GR_PYRAMID] < 5e-2
     # symmetry plane: planar at z = 0, area of the inscribed 32-gon minus the root section
     r = om.region("symmetry")
     g0, g1 = om.ghost_cell_starts[r], om.ghost_cell_starts[r + 1]
