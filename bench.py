@@ -119,7 +119,7 @@ def batched_leg(gr, args):
 def warm_start_leg(gr, args):
     """Learned warm start on the C-grid: 512 training flows generated by the batched solver,
     BF16-MFMA training, then 64 new flows to 1e-3 of their freestream residual: warm-started
-    batched vs cold batched vs cold single-flow solves; lift/drag after deeper convergence."""
+    batched vs cold batched vs cold single-flow solves; lift/drag after equal further work."""
     from graphite_amd import warmstart as ws
     log("bench: warm-start leg (training set)")
     m = gr.Mesh.build(gr.Elements.cgrid_naca())
@@ -173,12 +173,9 @@ def warm_start_leg(gr, args):
         F = b.force("airfoil")
         return np.array([gr.lift_drag(F[k], test[k], 1.4, ws.P_INF, s_ref) for k in range(64)])
     c3, w3 = coeffs(cold), coeffs(warm)
-    log("bench: warm-start leg (deeper convergence for lift/drag)")
-    cd = ws.solve(cold, 1e-5, fs, 20000)
-    wd5 = ws.solve(warm, 1e-5, fs, 20000)
-    c5, w5 = coeffs(cold), coeffs(warm)
-    both = cd.converged & wd5.converged
-    rel = lambda a, b: (np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)).tolist()
+    log("bench: warm-start leg (equal further work for lift/drag)")
+    agree = ws.force_agreement(cold, warm, test, "airfoil", s_ref, 20000)
+    rel = lambda a, b: (np.abs(a - b).max(axis=0) / np.abs(b[:, 0]).max()).tolist()
     it_w, it_c = wsol.iterations, cs.iterations
     return {
         "workload": "NACA0012 C-grid 384x96x1, 64 new flows (Mach U[0.5,0.85], AoA U[-2,6] deg), binary64, "
@@ -200,11 +197,9 @@ def warm_start_leg(gr, args):
                              "warm_batched_amortised": t_warm / 64, "cold_batched_amortised": cs.seconds / 64},
         "converged": {"warm": int(wsol.converged.sum()), "cold": int(cs.converged.sum()),
                       "single": int(sum(p[2] for p in plain))},
-        "lift_drag": {"s_ref": s_ref,
+        "lift_drag": {"s_ref": s_ref, "scale": "largest |C_L| of the sweep",
                       "rel_diff_at_1e-3_warm_vs_cold": rel(w3, c3),
-                      "rel_diff_at_1e-5_warm_vs_cold": rel(w5[both], c5[both]) if both.any() else None,
-                      "flows_converged_to_1e-5": int(both.sum()),
-                      "cold_1e-3_vs_1e-5": rel(c3, c5), "warm_1e-3_vs_1e-5": rel(w3, w5)},
+                      "after_equal_further_work": agree},
         "normalisation": "residual of the freestream state of each flow, for warm and cold starts alike",
     }
 

@@ -615,11 +615,15 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
                 const uint32_t il = 2 * i > ni && q >= n_taper ? ni - i : i;
                 const V3 Bl = X[nid(il, 0, ns)], Dl = T[il];
                 const double zp = Bl.z + sq[q] * (Dl.z - Bl.z);
+                // the section's surface moves towards the plate by dy0; the rings next to it move
+                // with it (fading with w_j), so the wall layers keep their thickness and stay
+                // normal to the tapering surface
+                const double dy0 = (taper - 1.0) * X[nid(i, 0, ns)].y;
                 for (uint32_t j = 0; j <= nj; ++j) {
                     const uint32_t ii = j == 0 ? il : i;
                     const V3 B = X[nid(ii, j, ns)], D = T[(size_t)j * ni + ii];
                     const double w = wj[j], sj = (1.0 - w) * sq[q] + w * s_out;
-                    const double y = j == 0 ? taper * (1.0 - sj) * B.y : B.y + sj * (D.y - B.y);
+                    const double y = B.y + sj * (D.y - B.y) + (1.0 - sj) * (1.0 - w) * (j == 0 ? (taper - 1.0) * B.y : dy0);
                     // height: the tip station's plus an increment blended from the plate's to
                     // the far-field ring's, but not below the plate's height: the stations are
                     // level around the plate, where the cells are tall and narrow (a sloping
