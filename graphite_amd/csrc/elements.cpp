@@ -419,7 +419,7 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
     // plate: the tip is capped by a short wedge rather than one steep ridge, whose faces (the
     // section at the tip station against the plate one station up) twist strongly where the
     // thickness changes fast, at the leading edge.
-    const uint32_t n_taper = std::max(1u, std::min(4u, nc / 3));
+    const uint32_t n_taper = std::max(1u, std::min(8u, nc / 2));
     const uint64_t n_grid = (uint64_t)ni * (nj + 1) * (K + 1);
     require(n_grid * 8 < 0xFFFFFFF0ull, "wing: mesh too large for 32-bit indices");
     const double pi = 3.14159265358979323846;
