@@ -128,8 +128,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<
     for (int c = 0; c < 5; ++c) a.l[c * st + me] = lim[c];
 }
 
+// Optional occupancy target for the batched flux kernel (waves per SIMD; the compiler then
+// caps its registers, spilling if it must).  Unset: the compiler's choice.
+#ifdef GR_BATCH_FLUX_WAVES_PER_EU
+#define GR_BATCH_FLUX_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_BATCH_FLUX_WAVES_PER_EU, 8)))
+#else
+#define GR_BATCH_FLUX_OCCUPANCY
+#endif
+
 template <typename S, bool CORR>
-__global__ __launch_bounds__(kBlock) void k_batch_flux(MeshView m, BatchArgs<S> a, BcTable bc,
+__global__ __launch_bounds__(kBlock) GR_BATCH_FLUX_OCCUPANCY void k_batch_flux(MeshView m, BatchArgs<S> a, BcTable bc,
                                                        const double *__restrict__ inst_mach, double cfl,
                                                        double *__restrict__ partials, const BatchCtl *ctl) {
     if (ctl->stop) return;

@@ -20,6 +20,14 @@
 
 namespace gr {
 
+// Optional occupancy target for the flux kernel (waves per SIMD; the compiler then caps its
+// registers, spilling if it must).  Unset: the compiler's choice.
+#ifdef GR_FLUX_WAVES_PER_EU
+#define GR_FLUX_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_FLUX_WAVES_PER_EU, 8)))
+#else
+#define GR_FLUX_OCCUPANCY
+#endif
+
 constexpr int kTileSlots = 248;   // 20 binary64 rows x 248 slots = 39.7 KB LDS: 4 blocks per CU
 constexpr int kTileCells = 51;    // one thread per (component, cell) in the per-cell phases
 static_assert(5 * kTileCells <= kBlock, "per-cell phases need 5 threads per cell");
@@ -245,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
 // Flux + local-time-step update (fvmEulerSteadyOp + euler_steady_step,
 // src/models/euler.cu:236-381), Jacobi, with V*|R|^2 block partials.
 template <typename S, bool CORR>
-__global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
+__global__ __launch_bounds__(kBlock) GR_FLUX_OCCUPANCY void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
                                                       const uint32_t *__restrict__ tiles, uint32_t ntiles, double cfl,
                                                       double *partials, const SolverCtl *ctl) {
     if (ctl && ctl->stop) return;

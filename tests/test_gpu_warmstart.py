@@ -82,8 +82,9 @@ def test_prediction_is_applied(gr, small):
 
 
 def test_bad_prediction_falls_back_to_freestream(gr, small):
-    """A network that outputs NaN is rejected by the residual test: every instance restarts from
-    freestream, bit for bit the cold state."""
+    """A network that outputs NaN: the positivity guard puts every cell back at freestream, so
+    the prediction's residual equals the freestream residual and a strict judge (max_ratio 1)
+    rejects it; every instance restarts from freestream, bit for bit the cold state."""
     from graphite_amd import warmstart as ws
     m = small
     M = machs(4, seed=7)
@@ -97,7 +98,7 @@ def test_bad_prediction_falls_back_to_freestream(gr, small):
     for j in range(5):
         norm.out_mean[j], norm.out_std[j] = 0.0, 1.0
     warm = ws.make_batch(m, M)
-    fs, used = warm.warm_start(net, ws.wall_distance(m), norm)
+    fs, used = warm.warm_start(net, ws.wall_distance(m), norm, max_ratio=1.0)
     assert not used.any()
     cold = ws.make_batch(m, M)
     assert np.array_equal(fs, cold.residual_norms())
