@@ -54,6 +54,19 @@ def make_solver(gr, m, dtype, cfl):
     return sv
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 (FETCH_SIZE, WRITE_SIZE, calibrated
+    against a copy kernel), from the summaries under profiles/; None if none records it."""
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
+        with open(os.path.join(d, name)) as fh:
+            for line in fh:
+                f = line.split()
+                if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                    return float(f[2])
+    return None
+
+
 def iteration_bytes(s, su, sg=8):
     """Algorithmic bytes of one iteration in the reference's phase accounting (each array
     element counted once per phase: BC, gradient, limiter, ghost copy, flux+update)."""
@@ -294,7 +307,7 @@ def main():
                                "second order + reference limiter, plain FP64 solver",
                    "cells": s.n_cells, "ghosts": s.n_ghosts, "faces": s.n_faces, "cfl": args.cfl},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(dom),
                      "algorithmic_bytes_per_launch": kb[dom], "avg_launch_ms": prof[dom]},
         "kernels": per_kernel,
         "iteration_bytes_reference_phases": iteration_bytes(s, 8),

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk) {
             g[3 * c + kk] = out[kk];
-            a.g[(3 * c + kk) * st + me] = out[kk];
+            store_out(&a.g[(3 * c + kk) * st + me], out[kk]);
         }
     }
     S lim[5] = {S(2), S(2), S(2), S(2), S(2)};
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<
                                                   dy, dz, kind));
     }
 #pragma unroll
-    for (int c = 0; c < 5; ++c) a.l[c * st + me] = lim[c];
+    for (int c = 0; c < 5; ++c) store_out(&a.l[c * st + me], lim[c]);
 }
 
 // Optional occupancy target for the batched flux kernel (waves per SIMD; the compiler then
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBlock) GR_BATCH_FLUX_OCCUPANCY void k_batch_flux(M
         }
         if (ctl->done[k]) {   // frozen instance: carry its state over
 #pragma unroll
-            for (int c = 0; c < 5; ++c) a.un[c * st + me] = ui[c];
+            for (int c = 0; c < 5; ++c) store_out(&a.un[c * st + me], ui[c]);
         } else {
 #pragma unroll
             for (int q = 0; q < 15; ++q) gi[q] = a.g[q * st + me];
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kBlock) GR_BATCH_FLUX_OCCUPANCY void k_batch_flux(M
             }
             const S dt = fdiv((S)cfl, dts);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) a.un[c * st + me] = ui[c] + (-acc[c]) * dt;
+            for (int c = 0; c < 5; ++c) store_out(&a.un[c * st + me], ui[c] + (-acc[c]) * dt);
             const double r = acc[0], u0 = acc[1], u1 = acc[2], u2 = acc[3], en = acc[4];
             e_loc = ((r * r + ((u0 * u0 + u1 * u1) + u2 * u2)) + en * en) * V;
         }

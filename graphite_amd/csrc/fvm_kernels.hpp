@@ -15,6 +15,19 @@
 
 namespace gr {
 
+// Store of an iteration kernel's output (new state, gradient, limiter).  With GR_NT_STORES the
+// store is non-temporal: the outputs are consumed by the next kernel, long after the L2 has
+// turned over, and keeping them out of it leaves room for the neighbour gathers.
+template <typename T>
+__device__ inline void store_out(T *p, T v) {
+#ifdef GR_NT_STORES
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+
 // Device-side control block of the iteration loop (one per solver).
 struct SolverCtl {
     double ref;        // normalisation; <= 0 takes the first residual

@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         }
         lsq_apply<S>(m, ic, g3, bg);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.g[3 * bc_c + k][ic] = bg[k];
+        for (int k = 0; k < 3; ++k) store_out(&a.g[3 * bc_c + k][ic], bg[k]);
     }
     __syncthreads();
     if (in_b) {
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         const int se = (int)s_start[bc_l + 1];
         S l = S(2);
         for (int s = bsb; s < se; ++s) l = fmin(l, s_buf[15 + bc_c][s]);
-        a.l[bc_c][c0 + bc_l] = l;
+        store_out(&a.l[bc_c][c0 + bc_l], l);
     }
 }
 
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kBlock) GR_FLUX_OCCUPANCY void k_flux_tile(MeshView
             dts += s_e[s];
         }
         const S dt = fdiv((S)cfl, dts);
-        a.un[cc_][i] = a.u[cc_][i] + (-acc) * dt;
+        store_out(&a.un[cc_][i], a.u[cc_][i] + (-acc) * dt);
     }
     __syncthreads();
     if (in_b) s_d[cc_][cl] = acc;

@@ -134,20 +134,28 @@ def force_agreement(cold: Batch, warm: Batch, machs, region: str, s_ref: float, 
     fixed amount of work rather than at a residual level, and the cold solve's own drift over
     the last quarter of the run is reported beside the difference (its noise floor).  Both are
     relative to the largest |lift coefficient| of the sweep (subsonic inviscid drag is numerical
-    and near zero, so it is measured on the lift's scale).  Returns the two arrays' stats."""
+    and near zero, so it is measured on the lift's scale).  Flows that diverge on either side
+    are frozen, counted and left out.  Returns a dict of the statistics."""
     from . import lift_drag
     def coeffs(b):
         F = b.force(region)
         return np.array([lift_drag(F[k], machs[k], RHO_INF, P_INF, s_ref) for k in range(len(machs))])
     step = extra_iter // checkpoints
+    n = len(machs)
+    dead_c, dead_w = np.zeros(n, bool), np.zeros(n, bool)
     hist_c = []
     for _ in range(checkpoints):
-        cold.iterate(step)
-        warm.iterate(step)
+        # a flow that diverges (transonic flows can leave the limit cycle) is frozen and left
+        # out of the comparison, for whichever start it happened to
+        dead_c |= cold.iterate_masked(step, active=~dead_c)[2]
+        dead_w |= warm.iterate_masked(step, active=~dead_w)[2]
         hist_c.append(coeffs(cold))
-    c, w = hist_c[-1], coeffs(warm)
+    ok = ~(dead_c | dead_w)
+    c, w = hist_c[-1][ok], coeffs(warm)[ok]
     scale = np.abs(c[:, 0]).max()
     diff = np.abs(w - c).max(axis=0) / scale
-    drift = np.abs(hist_c[-1] - hist_c[-2]).max(axis=0) / scale
-    return {"extra_iterations": step * checkpoints, "max_rel_diff_cl_cd": diff.tolist(),
-            "cold_drift_last_quarter_cl_cd": drift.tolist(), "scale_max_abs_cl": float(scale)}
+    drift = np.abs(hist_c[-1][ok] - hist_c[-2][ok]).max(axis=0) / scale
+    return {"extra_iterations": step * checkpoints, "flows_compared": int(ok.sum()),
+            "diverged_cold": int(dead_c.sum()), "diverged_warm": int(dead_w.sum()),
+            "max_rel_diff_cl_cd": diff.tolist(), "cold_drift_last_quarter_cl_cd": drift.tolist(),
+            "scale_max_abs_cl": float(scale)}
