@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk) {
             g[3 * c + kk] = out[kk];
-            store_out(&a.g[(3 * c + kk) * st + me], out[kk]);
+            store_nt(&a.g[(3 * c + kk) * st + me], out[kk]);
         }
     }
     S lim[5] = {S(2), S(2), S(2), S(2), S(2)};
@@ -125,19 +125,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<
                                                   dy, dz, kind));
     }
 #pragma unroll
-    for (int c = 0; c < 5; ++c) store_out(&a.l[c * st + me], lim[c]);
+    for (int c = 0; c < 5; ++c) store_nt(&a.l[c * st + me], lim[c]);
 }
 
-// Optional occupancy target for the batched flux kernel (waves per SIMD; the compiler then
-// caps its registers, spilling if it must).  Unset: the compiler's choice.
-#ifdef GR_BATCH_FLUX_WAVES_PER_EU
-#define GR_BATCH_FLUX_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_BATCH_FLUX_WAVES_PER_EU, 8)))
-#else
-#define GR_BATCH_FLUX_OCCUPANCY
-#endif
-
+// At least 3 waves per SIMD: left to itself the compiler gives the binary64 kernel 174 VGPRs
+// (2 waves); capping it at 168 spills 6 registers and shortened the launch from 360 to 325 us
+// on the 64-flow C-grid (4 waves spill 45 and take 580 us).
 template <typename S, bool CORR>
-__global__ __launch_bounds__(kBlock) GR_BATCH_FLUX_OCCUPANCY void k_batch_flux(MeshView m, BatchArgs<S> a, BcTable bc,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_batch_flux(MeshView m, BatchArgs<S> a, BcTable bc,
                                                        const double *__restrict__ inst_mach, double cfl,
                                                        double *__restrict__ partials, const BatchCtl *ctl) {
     if (ctl->stop) return;
@@ -155,7 +150,7 @@ __global__ __launch_bounds__(kBlock) GR_BATCH_FLUX_OCCUPANCY void k_batch_flux(M
         }
         if (ctl->done[k]) {   // frozen instance: carry its state over
 #pragma unroll
-            for (int c = 0; c < 5; ++c) store_out(&a.un[c * st + me], ui[c]);
+            for (int c = 0; c < 5; ++c) store_nt(&a.un[c * st + me], ui[c]);
         } else {
 #pragma unroll
             for (int q = 0; q < 15; ++q) gi[q] = a.g[q * st + me];
@@ -221,7 +216,7 @@ __global__ __launch_bounds__(kBlock) GR_BATCH_FLUX_OCCUPANCY void k_batch_flux(M
             }
             const S dt = fdiv((S)cfl, dts);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) store_out(&a.un[c * st + me], ui[c] + (-acc[c]) * dt);
+            for (int c = 0; c < 5; ++c) store_nt(&a.un[c * st + me], ui[c] + (-acc[c]) * dt);
             const double r = acc[0], u0 = acc[1], u1 = acc[2], u2 = acc[3], en = acc[4];
             e_loc = ((r * r + ((u0 * u0 + u1 * u1) + u2 * u2)) + en * en) * V;
         }

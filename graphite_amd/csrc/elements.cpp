@@ -416,10 +416,11 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
     require(p.scale > 0 && std::isfinite(p.scale), "wing: scale must be positive");
     const uint32_t nj = np + nt, K = ns + nc;
     // Stations above the tip over which the section's thickness tapers to the zero-thickness
-    // plate: the tip is capped by a short wedge rather than one steep ridge, whose faces (the
-    // section at the tip station against the plate one station up) twist strongly where the
-    // thickness changes fast, at the leading edge.
-    const uint32_t n_taper = std::max(1u, std::min(8u, nc / 2));
+    // plate (set with the cap heights below): the tip is capped by a short wedge, about twice as
+    // tall as the tip section is thick, rather than one steep ridge, whose faces (the section at
+    // the tip station against the plate one station up) twist strongly where the thickness
+    // changes fast, at the leading edge.
+    uint32_t n_taper = 1;
     const uint64_t n_grid = (uint64_t)ni * (nj + 1) * (K + 1);
     require(n_grid * 8 < 0xFFFFFFF0ull, "wing: mesh too large for 32-bit indices");
     const double pi = 3.14159265358979323846;
@@ -581,6 +582,7 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
             }
             for (uint32_t q = 1; q <= nc; ++q) sq[q] /= sq[nc];
         }
+        while (n_taper + 1 <= nc / 2 && sq[n_taper + 1] * H <= 2.0 * p.thickness * p.taper) ++n_taper;
         // Station q above the tip.  Ring j's in-plane layout is B + s (T - B), B the tip station,
         // T the top layout, with s moving from the geometric cap fraction sq[q] at the plate (in
         // step with the heights, so no station slides over the one below) to s_out(q) at the
@@ -689,40 +691,57 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
     };
     // Cells of hex (i, j, k), corner[a + 2b + 4c] at offsets a, b, c along i, j, k.  A stretched
     // triangle, in a prism or a tetrahedron, is what makes the unlimited second-order update of
-    // large cells unstable; a long prism over a compact triangle is not.
-    //  - Below the tip (k < n_span): wall layers (j < n_prism) of prisms extruded along j, their
-    //    triangles on constant-j faces split along (i, k) -> (i+1, k+1); outside them the
-    //    6-tetrahedron split about the corner-0 -> corner-7 diagonal, whose face diagonals match
-    //    the prisms'.  In the last wall-layer row below the cap (k = n_span - 1) the two
-    //    tetrahedra on the bottom face merge into a pyramid on the prism's quad below, the other
-    //    four give the cap above triangles.
-    //  - Above the tip (k >= n_span): prisms extruded along k (upwards), their triangles on
-    //    constant-k faces split along (i, j) -> (i+1, j+1) as the tetrahedra below.
-    // The slices i = n_around/2 - 1 and i = n_around - 1 are built mirrored in i (every diagonal
-    // flipped except those on constant-i faces, so they still meet their neighbours): above the
-    // tip their j = 0 cells have a plate end, the plate and the ring beyond that end on one line,
-    // and the unmirrored triangle there would have three collinear corners.
+    // large cells unstable; a long prism over a compact triangle is not.  A thin tetrahedron on
+    // a wall is bad too: its centroid sits far off its wall face's normal line, which lets mass
+    // through a slip wall.
+    //  - Wall layers (j < n_prism) up to the top of the tip's taper (k < n_span + n_taper):
+    //    prisms extruded along j, triangles on constant-j faces split along (i, k) -> (i+1, k+1).
+    //  - Outside them below the tip (k < n_span): the 6-tetrahedron split about the
+    //    corner-0 -> corner-7 diagonal, whose face diagonals match the prisms'.
+    //  - Above the tip (k >= n_span, j > n_prism): prisms extruded along k (upwards), triangles
+    //    on constant-k faces split along (i, j) -> (i+1, j+1) as the tetrahedra below; cells
+    //    there are tall near the plate.  Their first layer (j = n_prism) up to the transition row
+    //    is a pyramid on the quad of the k-extruded prism outside it plus the four tetrahedra of
+    //    the split not touching that quad.
+    //  - The transition row k = n_span + n_taper (on the plate, no wall): wall-layer hexes as a
+    //    pyramid on the j-extruded prism's quad below plus four tetrahedra; above it every cell
+    //    is a k-extruded prism.
+    // A slice i built mirrored in i flips every diagonal except those on constant-i faces, so it
+    // still meets its neighbours.  The plate's lower column i and upper column n_around-1-i
+    // share their face on the transition row, so exactly one of each pair is mirrored: the
+    // upper half except the leading-edge slice n_around/2, plus the lower slice n_around/2 - 1
+    // -- the slices next to either end of the plate (n_around/2 - 1 and n_around - 1) must be
+    // mirrored, since there the unmirrored triangle (i, 0), (i+1, 0), (i+1, 1) of a k-extruded
+    // prism has its corners on the plate's line.
+    require(n_taper < nc, "wing: need more stations above the tip");
+    const uint32_t k_tr = ns + n_taper;
     for (uint32_t k = 0; k < K; ++k)
         for (uint32_t j = 0; j < nj; ++j)
             for (uint32_t i = 0; i < ni; ++i) {
-                const int mirror = (i + 1 == ni / 2 || i + 1 == ni) ? 1 : 0;
+                const int mirror = ((2 * i >= ni && 2 * i != ni) || i + 1 == ni / 2) ? 1 : 0;
                 uint32_t cr[8];
                 for (int o = 0; o < 8; ++o) {
                     const int m = o ^ mirror;
                     cr[o] = remap[nid(i + (m & 1), j + ((m >> 1) & 1), k + ((m >> 2) & 1))];
                 }
-                if (k >= ns) {
-                    add_prism(cr[0], cr[1], cr[3], cr[4], cr[5], cr[7]);
-                    add_prism(cr[0], cr[3], cr[2], cr[4], cr[7], cr[6]);
-                } else if (j < np && k + 1 < ns) {
+                if (j < np && k < k_tr) {
                     add_prism(cr[0], cr[1], cr[5], cr[2], cr[3], cr[7]);
                     add_prism(cr[0], cr[5], cr[4], cr[2], cr[7], cr[6]);
-                } else if (j < np) {
+                } else if (j < np && k == k_tr) {
                     add_pyramid(cr[0], cr[1], cr[3], cr[2], cr[7]);
                     add_tet(cr[0], cr[1], cr[5], cr[7]);
                     add_tet(cr[0], cr[4], cr[5], cr[7]);
                     add_tet(cr[0], cr[2], cr[6], cr[7]);
                     add_tet(cr[0], cr[4], cr[6], cr[7]);
+                } else if (j == np && k >= ns && k <= k_tr) {
+                    add_pyramid(cr[2], cr[3], cr[7], cr[6], cr[0]);
+                    add_tet(cr[0], cr[1], cr[3], cr[7]);
+                    add_tet(cr[0], cr[1], cr[5], cr[7]);
+                    add_tet(cr[0], cr[4], cr[5], cr[7]);
+                    add_tet(cr[0], cr[4], cr[6], cr[7]);
+                } else if (k >= ns) {
+                    add_prism(cr[0], cr[1], cr[3], cr[4], cr[5], cr[7]);
+                    add_prism(cr[0], cr[3], cr[2], cr[4], cr[7], cr[6]);
                 } else {
                     static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
                     for (const auto &pm : perms) {

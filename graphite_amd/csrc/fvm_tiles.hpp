@@ -20,14 +20,6 @@
 
 namespace gr {
 
-// Optional occupancy target for the flux kernel (waves per SIMD; the compiler then caps its
-// registers, spilling if it must).  Unset: the compiler's choice.
-#ifdef GR_FLUX_WAVES_PER_EU
-#define GR_FLUX_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GR_FLUX_WAVES_PER_EU, 8)))
-#else
-#define GR_FLUX_OCCUPANCY
-#endif
-
 constexpr int kTileSlots = 248;   // 20 binary64 rows x 248 slots = 39.7 KB LDS: 4 blocks per CU
 constexpr int kTileCells = 51;    // one thread per (component, cell) in the per-cell phases
 static_assert(5 * kTileCells <= kBlock, "per-cell phases need 5 threads per cell");
@@ -216,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         }
         lsq_apply<S>(m, ic, g3, bg);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) store_out(&a.g[3 * bc_c + k][ic], bg[k]);
+        for (int k = 0; k < 3; ++k) a.g[3 * bc_c + k][ic] = bg[k];
     }
     __syncthreads();
     if (in_b) {
@@ -246,14 +238,14 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         const int se = (int)s_start[bc_l + 1];
         S l = S(2);
         for (int s = bsb; s < se; ++s) l = fmin(l, s_buf[15 + bc_c][s]);
-        store_out(&a.l[bc_c][c0 + bc_l], l);
+        a.l[bc_c][c0 + bc_l] = l;
     }
 }
 
 // Flux + local-time-step update (fvmEulerSteadyOp + euler_steady_step,
 // src/models/euler.cu:236-381), Jacobi, with V*|R|^2 block partials.
 template <typename S, bool CORR>
-__global__ __launch_bounds__(kBlock) GR_FLUX_OCCUPANCY void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
+__global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
                                                       const uint32_t *__restrict__ tiles, uint32_t ntiles, double cfl,
                                                       double *partials, const SolverCtl *ctl) {
     if (ctl && ctl->stop) return;
@@ -352,7 +344,7 @@ __global__ __launch_bounds__(kBlock) GR_FLUX_OCCUPANCY void k_flux_tile(MeshView
             dts += s_e[s];
         }
         const S dt = fdiv((S)cfl, dts);
-        store_out(&a.un[cc_][i], a.u[cc_][i] + (-acc) * dt);
+        a.un[cc_][i] = a.u[cc_][i] + (-acc) * dt;
     }
     __syncthreads();
     if (in_b) s_d[cc_][cl] = acc;

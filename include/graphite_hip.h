@@ -101,7 +101,7 @@ int gr_gen_cgrid_naca(uint32_t n_i, uint32_t n_j, uint32_t n_wake, double thickn
  * and n_cap stations above the tip, over which the section tapers to a zero-thickness plate
  * (upper and lower nodes merged; above it the cells are prisms).  Every length is multiplied
  * by scale at the end (the reference's limiter threshold is dimensional, so the scale changes
- * which cells it limits).  Markers wall, farfield, symmetry.  Defaults giving 1,068,320 cells:
+ * which cells it limits).  Markers wall, farfield, symmetry.  Defaults giving 1,069,112 cells:
  * 88, 20, 70, 20, 16, 0.12, 30, 0.56, 1.5, 3e-3, 1.2, 25, 1. */
 typedef struct gr_wing_params {
     uint32_t n_around, n_prism, n_tet, n_span, n_cap;

@@ -6,27 +6,45 @@ import pytest
 WING_SMALL = dict(n_around=32, n_prism=4, n_tet=5, n_span=6, n_cap=4, radius=12.0)
 
 
-def wing_counts(ni, npr, nt, ns, nc):
-    """Below the tip: 2 prisms per wall-layer hex and 6 tetrahedra per outer hex, except the
-    last wall-layer row (a pyramid and 4 tetrahedra per hex); above the tip 2 prisms per hex."""
-    return (ns - 1) * ni * (2 * npr + 6 * nt) + ni * (5 * npr + 6 * nt) + nc * ni * 2 * (npr + nt)
+def wing_counts(ni, npr, nt, ns, nc, n_taper):
+    """Below the tip: 2 prisms per wall-layer hex and 6 tetrahedra per outer hex.  Above it, up to
+    the transition row k = ns + n_taper: 2 prisms per wall-layer hex, a pyramid and 4
+    tetrahedra at j = npr, 2 prisms per outer hex; on the transition row the wall-layer hexes
+    are a pyramid and 4 tetrahedra each; above it 2 prisms per hex."""
+    below = ns * ni * (2 * npr + 6 * nt)
+    taper = n_taper * ni * (2 * npr + 5 + 2 * (nt - 1))
+    transition = ni * (5 * npr + 5 + 2 * (nt - 1))
+    above = (nc - n_taper - 1) * ni * 2 * (npr + nt)
+    return below + taper + transition + above
+
+
+def taper_rows(kinds, ni, npr):
+    """Rows over which the tip tapers, from the pyramid count ni * (n_taper + 1 + npr)."""
+    n_pyr = int((np.asarray(kinds) == 14).sum())
+    assert n_pyr % ni == 0
+    return n_pyr // ni - 1 - npr
 
 
 def test_wing_counts_and_markers(gr):
     e = gr.Elements.wing(**WING_SMALL)
     m = gr.Mesh.build(e)
     s = m.sizes()
-    assert s.n_cells == wing_counts(32, 4, 5, 6, 4)
-    assert m.region_names() == ["wall", "farfield", "symmetry"]
     x, kinds, starts, nodes, markers = e.arrays()
+    nta = taper_rows(kinds, 32, 4)
+    assert 1 <= nta <= 2
+    assert s.n_cells == wing_counts(32, 4, 5, 6, 4, nta)
+    assert m.region_names() == ["wall", "farfield", "symmetry"]
     assert set(np.unique(kinds)) == {gr.GR_TET, gr.GR_WEDGE, gr.GR_PYRAMID}
 
 
 def test_wing_default_size():
     import graphite_amd as gr
     gr.load()
-    s = gr.Mesh.build(gr.Elements.wing()).sizes()
-    assert s.n_cells == wing_counts(88, 20, 70, 20, 16) == 1068320
+    e = gr.Elements.wing()
+    s = gr.Mesh.build(e).sizes()
+    nta = taper_rows(e.arrays()[1], 88, 20)
+    assert nta == 2
+    assert s.n_cells == wing_counts(88, 20, 70, 20, 16, nta) == 1069112
 
 
 def test_wing_geometry(gr, orc):
