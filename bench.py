@@ -236,6 +236,56 @@ def nsi_leg(gr, args):
             "cg_iterations_mean": float(np.mean(its)), "continuity_last": float(c[-1]), "dtype": "f32 bulk, f64 dots"}
 
 
+def wing_leg(gr, args):
+    """Transonic swept wing (1.07 M prisms, pyramids and tetrahedra in a half sphere of 25
+    chords), M 0.80, AoA 2.5 deg, lengths scaled by 0.01 (the limiter threshold is dimensional):
+    kernel rates over a fixed number of iterations in both precisions, then a mixed-precision
+    solve towards a 1e-3 residual drop with the drop and lift/drag reached (the case stalls
+    above 1e-3 so far)."""
+    log("bench: swept-wing leg")
+    scale = 0.01
+    m = gr.Mesh.build(gr.Elements.wing(scale=scale))
+    m.renumber_rcm()
+    m.compute_metrics()
+    s = m.sizes()
+    a = math.radians(2.5)
+    mach = (0.8 * math.cos(a), 0.8 * math.sin(a), 0.0)
+
+    def solver(dtype):
+        sv = gr.Solver(m, dtype=dtype, cfl=args.cfl)
+        for name, kind in (("wall", "slip"), ("symmetry", "slip"), ("farfield", "farfield")):
+            sv.set_bc(name, gr.GR_BC_FARFIELD if kind == "farfield" else gr.GR_BC_SLIP, mach, P_INF, T_INF)
+        sv.init_freestream(mach, P_INF, T_INF)
+        return sv
+
+    res = {"workload": "swept NACA0012 wing, 30 deg sweep, taper 0.56, semispan 1.5, prism/pyramid/tet mesh "
+                       "in a half sphere of 25 chords, lengths x0.01; M 0.80, AoA 2.5 deg",
+           "cells": s.n_cells, "faces": s.n_faces, "ghosts": s.n_ghosts}
+    steps = max(50, args.steps // 2)
+    for name, dtype in (("mixed", gr.GR_F32), ("f64", gr.GR_F64)):
+        sv = solver(dtype)
+        sv.iterate(args.warmup)
+        dt = timed_steps(sv, steps)
+        prof = sv.profile(20)
+        kb = sv.kernel_bytes()
+        res[name] = {"value": s.n_cells * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / steps * 1e3,
+                     "kernels": {k: {"ms": prof[k], "bytes": kb[k],
+                                     "GBps": kb[k] / (prof[k] * 1e-3) / 1e9 if prof[k] > 0 else None,
+                                     "frac_of_peak": kb[k] / (prof[k] * 1e-3) / 1e9 / HBM_PEAK_GBPS if prof[k] > 0 else None}
+                                 for k in prof}}
+    sv = solver(gr.GR_F32)
+    t0 = time.perf_counter()
+    r = sv.iterate(args.wing_iters, tol=1e-3)
+    t1 = time.perf_counter()
+    h = sv.history(min(r.iterations, 2000))
+    s_ref = 1.5 * 0.5 * (1.0 + 0.56) * scale * scale
+    cl, cd = gr.lift_drag(sv.force("wall"), mach, P_INF / T_INF, P_INF, s_ref)
+    res["solve_mixed"] = {"iterations": r.iterations, "seconds": t1 - t0, "converged_to_1e-3": bool(r.converged),
+                          "residual_drop_reached": r.ratio, "best_drop_last_2000": float(h.min() / r.reference) if len(h) and r.reference > 0 else None,
+                          "lift_drag": {"C_L": cl, "C_D": cd, "s_ref": s_ref}}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +298,8 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=100)
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-nsi", action="store_true")
+    ap.add_argument("--no-wing", action="store_true")
+    ap.add_argument("--wing-iters", type=int, default=6000)
     ap.add_argument("--nsi-steps", type=int, default=200)
     ap.add_argument("--warm-flows", type=int, default=512)
     ap.add_argument("--warm-steps", type=int, default=8000)
@@ -328,6 +380,8 @@ def main():
         out["warm_start"] = warm_start_leg(gr, args)
     if not args.no_nsi:
         out["nsi_cavity"] = nsi_leg(gr, args)
+    if not args.no_wing:
+        out["swept_wing"] = wing_leg(gr, args)
 
     if not args.no_converge:
         log("bench: time to a 1e-3 residual drop (binary64, cold start)")

@@ -549,7 +549,7 @@ struct gr_solver {
     gr_mesh *mesh = nullptr;
     gr_solver_config cfg{};
     size_t stride = 0;
-    gr::DevArray<char> ubuf[2], grad, lim;
+    gr::DevArray<char> ubuf[2], grad;   // grad: gradient + limiter records (kGL per cell)
     int cur = 0;         // buffer holding the current interior state
     int ghost_src = 0;   // buffer holding the latest ghost values
     struct Bc { bool set = false; int kind = 0; double mach[3] = {0, 0, 0}, p = 0, T = 0; };
@@ -578,17 +578,15 @@ struct gr_solver {
     }
     size_t es() const { return (size_t)cfg.dtype; }
     template <typename S> S *u(int b, int c) const { return (S *)ubuf[b].p + (size_t)c * stride; }
-    template <typename S> S *g(int k) const { return (S *)grad.p + (size_t)k * stride; }
-    template <typename S> S *l(int c) const { return (S *)lim.p + (size_t)c * stride; }
 
+    // State-only arguments (the boundary-condition kernel reads and writes states alone; the
+    // solver's gradients and limiters are per-cell records, reached through sargs).
     template <typename S> gr::EulerArgs<S> args(int b) const {
         gr::EulerArgs<S> a{};
         for (int c = 0; c < 5; ++c) {
             a.u[c] = u<S>(b, c);
             a.un[c] = u<S>(1 - b, c);
-            a.l[c] = l<S>(c);
         }
-        for (int k = 0; k < 15; ++k) a.g[k] = g<S>(k);
         return a;
     }
     template <typename S> gr::SolverArgs<S> sargs(int b) const {
@@ -596,9 +594,8 @@ struct gr_solver {
         for (int c = 0; c < 5; ++c) {
             a.u[c] = u<S>(b, c);
             a.un[c] = u<S>(1 - b, c);
-            a.l[c] = l<S>(c);
         }
-        for (int k = 0; k < 15; ++k) a.g[k] = g<S>(k);
+        a.gl = (S *)grad.p;
         return a;
     }
     gr::BcTable table() const {
@@ -750,10 +747,8 @@ int gr_solver_create(gr_mesh *m, const gr_solver_config *cfg, gr_solver **out) {
                 b.alloc(5 * blk);
                 GR_HIP(hipMemsetAsync(b.p, 0, b.n, m->stream));
             }
-            s->grad.alloc(15 * blk);
-            s->lim.alloc(5 * blk);
+            s->grad.alloc((size_t)gr::kGL * blk);
             GR_HIP(hipMemsetAsync(s->grad.p, 0, s->grad.n, m->stream));
-            GR_HIP(hipMemsetAsync(s->lim.p, 0, s->lim.n, m->stream));
             s->bcs.resize(m->h.n_regions);
             m->ensure_tiles(gr::kTileSlots, gr::kTileCells);
             s->ctl.alloc(1);

@@ -131,13 +131,53 @@ __device__ inline int tile_cell_of(const uint32_t *starts, int ncell, uint32_t s
     return lo;
 }
 
+// Gradient and limiter of a cell as one record: rows 3c+k of the gradient (component c,
+// direction k), then the five limiters.  A neighbour's record is 160 contiguous bytes in
+// binary64 (80 in binary32), fetched with 16-byte loads from two or three cache lines, where
+// twenty separate arrays would touch twenty lines for it.
+constexpr int kGL = 20;
+
 template <typename S>
 struct SolverArgs {
     const S *u[5];    // current state (cells; ghosts only for regions without a BC)
     S *un[5];         // next state
-    S *g[15];         // gradients
-    S *l[5];          // limiters
+    S *gl;            // gradient + limiter records of the interior cells, kGL values each
 };
+
+// The record of cell i: gradient rows into g[15], limiters into l[5].
+template <typename S>
+__device__ inline void load_gl(const S *__restrict__ gl, uint32_t i, S g[15], S l[5]) {
+    const S *p = gl + (size_t)i * kGL;
+    if constexpr (sizeof(S) == 8) {
+        const double2 *q = reinterpret_cast<const double2 *>(p);
+        S v[20];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+            const double2 t = q[k];
+            v[2 * k] = t.x;
+            v[2 * k + 1] = t.y;
+        }
+#pragma unroll
+        for (int k = 0; k < 15; ++k) g[k] = v[k];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) l[k] = v[15 + k];
+    } else {
+        const float4 *q = reinterpret_cast<const float4 *>(p);
+        S v[20];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const float4 t = q[k];
+            v[4 * k] = t.x;
+            v[4 * k + 1] = t.y;
+            v[4 * k + 2] = t.z;
+            v[4 * k + 3] = t.w;
+        }
+#pragma unroll
+        for (int k = 0; k < 15; ++k) g[k] = v[k];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) l[k] = v[15 + k];
+    }
+}
 
 // Gradient (LSQ) + neighbour min/max + limiter of the five conservative components
 // (FvmEulerStepPrepare, src/models/euler.cu:597-647).
@@ -207,8 +247,6 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
             bmax = fmax(bmax, s_buf[15 + bc_c][s]);
         }
         lsq_apply<S>(m, ic, g3, bg);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) a.g[3 * bc_c + k][ic] = bg[k];
     }
     __syncthreads();
     if (in_b) {
@@ -233,12 +271,20 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     }
     __syncthreads();
 
-    // D: one thread per (component, cell): minimum over the cell's faces, starting from 2
+    // D: one thread per (component, cell): minimum over the cell's faces, starting from 2.  Row
+    // 15+c of the cell's slots is read by this thread alone, so the limiter can go straight to
+    // the cell's first slot column, under its gradient rows.
     if (in_b) {
         const int se = (int)s_start[bc_l + 1];
         S l = S(2);
         for (int s = bsb; s < se; ++s) l = fmin(l, s_buf[15 + bc_c][s]);
-        a.l[bc_c][c0 + bc_l] = l;
+        s_buf[15 + bc_c][bsb] = l;
+    }
+    __syncthreads();
+    // E: the tile's gradient + limiter records, contiguous in memory, written coalesced
+    for (int k = t; k < kGL * ncell; k += blockDim.x) {
+        const int cl = k / kGL, r = k - cl * kGL;
+        a.gl[(size_t)c0 * kGL + k] = s_buf[r][s_start[cl]];
     }
 }
 
@@ -275,23 +321,15 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
         const S d1[3] = {(S)(fcx - m.cx[j]), (S)(fcy - m.cy[j]), (S)(fcz - m.cz[j])};
         S ui[5], gi[15], li[5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            ui[c] = a.u[c][i];
-            li[c] = a.l[c][i];
-        }
-#pragma unroll
-        for (int k = 0; k < 15; ++k) gi[k] = a.g[k][i];
+        for (int c = 0; c < 5; ++c) ui[c] = a.u[c][i];
+        load_gl<S>(a.gl, i, gi, li);
         S r0, ru0[3], e0, r1, ru1[3], e1;
         reconstruct<S, CORR>(ui, gi, li, d0, r0, ru0, e0);
         if (j < m.N) {
             S uj[5], gj[15], lj[5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) {
-                uj[c] = a.u[c][j];
-                lj[c] = a.l[c][j];
-            }
-#pragma unroll
-            for (int k = 0; k < 15; ++k) gj[k] = a.g[k][j];
+            for (int c = 0; c < 5; ++c) uj[c] = a.u[c][j];
+            load_gl<S>(a.gl, j, gj, lj);
             reconstruct<S, CORR>(uj, gj, lj, d1, r1, ru1, e1);
         } else {   // ghost: its state from the BC, gradient and limiter of cell i
             S ug[5];
