@@ -550,6 +550,7 @@ struct gr_solver {
     gr_solver_config cfg{};
     size_t stride = 0;
     gr::DevArray<char> ubuf[2], grad;   // grad: gradient + limiter records (kGL per cell)
+    gr::DevArray<char> slot_geo;        // flux, gradient and ghost geometry records
     int cur = 0;         // buffer holding the current interior state
     int ghost_src = 0;   // buffer holding the latest ghost values
     struct Bc { bool set = false; int kind = 0; double mach[3] = {0, 0, 0}, p = 0, T = 0; };
@@ -596,7 +597,23 @@ struct gr_solver {
             a.un[c] = u<S>(1 - b, c);
         }
         a.gl = (S *)grad.p;
+        const size_t slots = mesh->h.cell_faces_starts[mesh->h.n_cells];
+        a.frec = (const S *)slot_geo.p;
+        a.grec = a.frec + slots * gr::kFluxRec<S>;
+        a.gnrm = a.grec + slots * gr::kGradRec;
         return a;
+    }
+    // Per-slot geometry records of the tile kernels, from the mesh metrics.
+    template <typename S> void build_slot_geometry() {
+        gr_mesh *m = mesh;
+        const size_t slots = m->h.cell_faces_starts[m->h.n_cells];
+        slot_geo.alloc(std::max<size_t>(1, (slots * (gr::kFluxRec<S> + gr::kGradRec) + (size_t)m->h.n_ghosts * gr::kGhostRec) *
+                                              sizeof(S)));
+        const gr::SolverArgs<S> a = sargs<S>(0);
+        if (m->h.n_cells)
+            hipLaunchKernelGGL(gr::k_slot_geometry<S>, dim3(gr::blocks_for(m->h.n_cells)), dim3(gr::kBlock), 0, m->stream,
+                               m->view(), (S *)a.frec, (S *)a.grec, (S *)a.gnrm);
+        GR_HIP(hipGetLastError());
     }
     gr::BcTable table() const {
         gr::BcTable t{};
@@ -751,6 +768,8 @@ int gr_solver_create(gr_mesh *m, const gr_solver_config *cfg, gr_solver **out) {
             GR_HIP(hipMemsetAsync(s->grad.p, 0, s->grad.n, m->stream));
             s->bcs.resize(m->h.n_regions);
             m->ensure_tiles(gr::kTileSlots, gr::kTileCells);
+            if (cfg->dtype == GR_F64) s->build_slot_geometry<double>();
+            else s->build_slot_geometry<float>();
             s->ctl.alloc(1);
             s->hist_cap = 1 << 17;
             s->hist.alloc(s->hist_cap);

@@ -104,22 +104,6 @@ __device__ inline void outward_normal(const MeshView &m, uint32_t f, double xi, 
     n[2] = (S)(nfz * sg);
 }
 
-// Neighbour state across slot q of cell i: interior cells from memory, ghosts from the fused
-// boundary condition of their region (or from memory when their region has none).
-template <typename S>
-__device__ inline void neighbour_state(const MeshView &m, const BcTable &t, const S *const u[5], uint32_t i, uint32_t j,
-                                       uint32_t f, double xi, double yi, double zi, const S ui[5], S uj[5]) {
-    int r = j >= m.N ? bc_region(t, j - m.N) : -1;
-    if (r < 0) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) uj[c] = u[c][j];
-        return;
-    }
-    S n[3];
-    outward_normal<S>(m, f, xi, yi, zi, n);
-    ghost_state<S>(t.r[r], n, ui, uj);
-}
-
 // Locates the local cell of local slot s among the tile's starts (ncell+1 entries).
 __device__ inline int tile_cell_of(const uint32_t *starts, int ncell, uint32_t s) {
     int lo = 0, hi = ncell;   // starts[lo] <= s < starts[hi]
@@ -131,52 +115,110 @@ __device__ inline int tile_cell_of(const uint32_t *starts, int ncell, uint32_t s
     return lo;
 }
 
-// Gradient and limiter of a cell as one record: rows 3c+k of the gradient (component c,
-// direction k), then the five limiters.  A neighbour's record is 160 contiguous bytes in
-// binary64 (80 in binary32), fetched with 16-byte loads from two or three cache lines, where
-// twenty separate arrays would touch twenty lines for it.
-constexpr int kGL = 20;
-
-template <typename S>
-struct SolverArgs {
-    const S *u[5];    // current state (cells; ghosts only for regions without a BC)
-    S *un[5];         // next state
-    S *gl;            // gradient + limiter records of the interior cells, kGL values each
-};
-
-// The record of cell i: gradient rows into g[15], limiters into l[5].
-template <typename S>
-__device__ inline void load_gl(const S *__restrict__ gl, uint32_t i, S g[15], S l[5]) {
-    const S *p = gl + (size_t)i * kGL;
+// Loads W consecutive values with 16-byte loads (p 16-byte aligned, W values a whole number
+// of 16-byte words).
+template <typename S, int W>
+__device__ inline void load_vec(const S *__restrict__ p, S (&v)[W]) {
+    static_assert((W * sizeof(S)) % 16 == 0, "records are whole 16-byte words");
     if constexpr (sizeof(S) == 8) {
         const double2 *q = reinterpret_cast<const double2 *>(p);
-        S v[20];
 #pragma unroll
-        for (int k = 0; k < 10; ++k) {
+        for (int k = 0; k < W / 2; ++k) {
             const double2 t = q[k];
             v[2 * k] = t.x;
             v[2 * k + 1] = t.y;
         }
-#pragma unroll
-        for (int k = 0; k < 15; ++k) g[k] = v[k];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) l[k] = v[15 + k];
     } else {
         const float4 *q = reinterpret_cast<const float4 *>(p);
-        S v[20];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < W / 4; ++k) {
             const float4 t = q[k];
             v[4 * k] = t.x;
             v[4 * k + 1] = t.y;
             v[4 * k + 2] = t.z;
             v[4 * k + 3] = t.w;
         }
-#pragma unroll
-        for (int k = 0; k < 15; ++k) g[k] = v[k];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) l[k] = v[15 + k];
     }
+}
+
+// Gradient and limiter of a cell as one record: rows 3c+k of the gradient (component c,
+// direction k), then the five limiters.  A neighbour's record is 160 contiguous bytes in
+// binary64 (80 in binary32), fetched with 16-byte loads from two or three cache lines, where
+// twenty separate arrays would touch twenty lines for it.
+constexpr int kGL = 20;
+
+// Per-slot geometry of the iteration, one record per face slot q of an interior cell i
+// (neighbour j = cell_connects[q], face f = cell_faces[q]), computed once from the binary64
+// metrics and cast exactly as the kernels cast them, so reading it instead of gathering face
+// and cell metrics leaves every bit of the results unchanged.  The records are read in slot
+// order, one contiguous run per tile, where the metrics they replace are gathered by face
+// and neighbour id.
+//   flux record:     n = outer_normal(n_f, fc - x_i) (3), ds = A_f / V_i, d0 = fc - x_i (3),
+//                    d1 = fc - x_j (3), padding to 16 bytes
+//   gradient record: d = x_j - x_i (3), 1/|d|^2, d0 (3), padding
+//   ghost record:    n of the ghost's face seen from its cell (3), padding (fused BCs)
+template <typename S>
+constexpr int kFluxRec = sizeof(S) == 8 ? 10 : 12;
+constexpr int kGradRec = 8;
+constexpr int kGhostRec = 4;
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_rec, S *grad_rec, S *ghost_rec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.N) return;
+    const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+    for (uint32_t q = m.cfs[i]; q < m.cfs[i + 1]; ++q) {
+        const uint32_t f = m.cf[q], j = m.cc[q];
+        S n[3];
+        outward_normal<S>(m, f, xi, yi, zi, n);
+        const double fcx = m.fcx[f], fcy = m.fcy[f], fcz = m.fcz[f];
+        const S d0[3] = {(S)(fcx - xi), (S)(fcy - yi), (S)(fcz - zi)};
+        S *fr = flux_rec + (size_t)q * kFluxRec<S>;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fr[k] = n[k];
+        fr[3] = geo_ratio<S>(m.fa[f], m.cv[i]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fr[4 + k] = d0[k];
+        fr[7] = (S)(fcx - m.cx[j]);
+        fr[8] = (S)(fcy - m.cy[j]);
+        fr[9] = (S)(fcz - m.cz[j]);
+        for (int k = 10; k < kFluxRec<S>; ++k) fr[k] = S(0);
+        const S d[3] = {(S)(m.cx[j] - xi), (S)(m.cy[j] - yi), (S)(m.cz[j] - zi)};
+        S *gq = grad_rec + (size_t)q * kGradRec;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gq[k] = d[k];
+        gq[3] = fdiv(S(1), (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gq[4 + k] = d0[k];
+        gq[7] = S(0);
+        if (j >= m.N) {   // a ghost's only face slot belongs to this cell
+            S *g = ghost_rec + (size_t)(j - m.N) * kGhostRec;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g[k] = n[k];
+            g[3] = S(0);
+        }
+    }
+}
+
+template <typename S>
+struct SolverArgs {
+    const S *u[5];    // current state (cells; ghosts only for regions without a BC)
+    S *un[5];         // next state
+    S *gl;            // gradient + limiter records of the interior cells, kGL values each
+    const S *frec;    // flux records, kFluxRec<S> per interior face slot
+    const S *grec;    // gradient records, kGradRec per interior face slot
+    const S *gnrm;    // ghost records, kGhostRec per ghost
+};
+
+// The record of cell i: gradient rows into g[15], limiters into l[5].
+template <typename S>
+__device__ inline void load_gl(const S *__restrict__ gl, uint32_t i, S g[15], S l[5]) {
+    S v[kGL];
+    load_vec<S, kGL>(gl + (size_t)i * kGL, v);
+#pragma unroll
+    for (int k = 0; k < 15; ++k) g[k] = v[k];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) l[k] = v[15 + k];
 }
 
 // Gradient (LSQ) + neighbour min/max + limiter of the five conservative components
@@ -199,28 +241,32 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
 
     // A: per-slot gradient terms d*(u_j-u_i)/|d|^2 (rows 0..14) and u_j (rows 15..19)
     int lc = 0;
-    uint32_t i = 0, q = 0;
-    double xi = 0, yi = 0, zi = 0;
+    uint32_t q = 0;
     S ui[5];
     if (t < nslot) {
         lc = tile_cell_of(s_start, ncell, (uint32_t)t);
-        i = c0 + lc;
+        const uint32_t i = c0 + lc;
         q = q0 + t;
-        xi = m.cx[i];
-        yi = m.cy[i];
-        zi = m.cz[i];
 #pragma unroll
         for (int c = 0; c < 5; ++c) ui[c] = a.u[c][i];
         const uint32_t j = m.cc[q];
+        S dw[4];   // d = x_j - x_i, 1/|d|^2
+        load_vec<S, 4>(a.grec + (size_t)q * kGradRec, dw);
         S uj[5];
-        neighbour_state<S>(m, bc, a.u, i, j, m.cf[q], xi, yi, zi, ui, uj);
-        const S d[3] = {(S)(m.cx[j] - xi), (S)(m.cy[j] - yi), (S)(m.cz[j] - zi)};
-        const S w2 = fdiv(S(1), (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+        const int r = j >= m.N ? bc_region(bc, j - m.N) : -1;
+        if (r < 0) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) uj[c] = a.u[c][j];
+        } else {
+            S n[4];
+            load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, n);
+            ghost_state<S>(bc.r[r], n, ui, uj);
+        }
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const S du = uj[c] - ui[c];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) s_buf[3 * c + k][t] = (d[k] * du) * w2;
+            for (int k = 0; k < 3; ++k) s_buf[3 * c + k][t] = (dw[k] * du) * dw[3];
             s_buf[15 + c][t] = uj[c];
         }
     }
@@ -262,12 +308,12 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     // in B): disjoint, so no barrier is needed between the reads and the writes.
     if (t < nslot) {
         const int sb = (int)s_start[lc];
-        const uint32_t f = m.cf[q];
-        const S dx = (S)(m.fcx[f] - xi), dy = (S)(m.fcy[f] - yi), dz = (S)(m.fcz[f] - zi);
+        S d0[4];   // fc - x_i
+        load_vec<S, 4>(a.grec + (size_t)q * kGradRec + 4, d0);
 #pragma unroll
         for (int c = 0; c < 5; ++c)
             s_buf[15 + c][t] = limiter_face<S>(ui[c], s_buf[3 * c][sb], s_buf[3 * c + 1][sb], s_buf[3 * c + 2][sb],
-                                               s_buf[c][sb + 1], s_buf[5 + c][sb + 1], dx, dy, dz, kind);
+                                               s_buf[c][sb + 1], s_buf[5 + c][sb + 1], d0[0], d0[1], d0[2], kind);
     }
     __syncthreads();
 
@@ -311,14 +357,13 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
     if (t < nslot) {
         const int lc = tile_cell_of(s_start, ncell, (uint32_t)t);
         const uint32_t i = c0 + lc, q = q0 + t;
-        const uint32_t f = m.cf[q], j = m.cc[q];
-        const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
-        const double fcx = m.fcx[f], fcy = m.fcy[f], fcz = m.fcz[f];
-        S n[3];
-        outward_normal<S>(m, f, xi, yi, zi, n);
-        const S ds = geo_ratio<S>(m.fa[f], m.cv[i]);
-        const S d0[3] = {(S)(fcx - xi), (S)(fcy - yi), (S)(fcz - zi)};
-        const S d1[3] = {(S)(fcx - m.cx[j]), (S)(fcy - m.cy[j]), (S)(fcz - m.cz[j])};
+        const uint32_t j = m.cc[q];
+        S geo[kFluxRec<S>];
+        load_vec<S, kFluxRec<S>>(a.frec + (size_t)q * kFluxRec<S>, geo);
+        const S n[3] = {geo[0], geo[1], geo[2]};
+        const S ds = geo[3];
+        const S d0[3] = {geo[4], geo[5], geo[6]};
+        const S d1[3] = {geo[7], geo[8], geo[9]};
         S ui[5], gi[15], li[5];
 #pragma unroll
         for (int c = 0; c < 5; ++c) ui[c] = a.u[c][i];
