@@ -578,24 +578,13 @@ struct gr_solver {
         }
     }
     size_t es() const { return (size_t)cfg.dtype; }
-    template <typename S> S *u(int b, int c) const { return (S *)ubuf[b].p + (size_t)c * stride; }
+    // State records (kUR values per cell and ghost, `stride` records) of buffer b.
+    template <typename S> S *rec(int b) const { return (S *)ubuf[b].p; }
 
-    // State-only arguments (the boundary-condition kernel reads and writes states alone; the
-    // solver's gradients and limiters are per-cell records, reached through sargs).
-    template <typename S> gr::EulerArgs<S> args(int b) const {
-        gr::EulerArgs<S> a{};
-        for (int c = 0; c < 5; ++c) {
-            a.u[c] = u<S>(b, c);
-            a.un[c] = u<S>(1 - b, c);
-        }
-        return a;
-    }
     template <typename S> gr::SolverArgs<S> sargs(int b) const {
         gr::SolverArgs<S> a{};
-        for (int c = 0; c < 5; ++c) {
-            a.u[c] = u<S>(b, c);
-            a.un[c] = u<S>(1 - b, c);
-        }
+        a.u = rec<S>(b);
+        a.un = rec<S>(1 - b);
         a.gl = (S *)grad.p;
         const size_t slots = mesh->h.cell_faces_starts[mesh->h.n_cells];
         a.frec = (const S *)slot_geo.p;
@@ -661,11 +650,16 @@ struct gr_solver {
     // Writes the boundary-condition ghost states of buffer b's interior into buffer b (the
     // iteration kernels evaluate them on the fly and never store them).
     void materialise_ghosts(int b) {
-        gr_mesh *m = mesh;
-        const gr::BcTable t = table();
-        if (cfg.dtype == GR_F64) launch_bc<double>(m, args<double>(b), t, 0, m->h.n_ghosts, nullptr);
-        else launch_bc<float>(m, args<float>(b), t, 0, m->h.n_ghosts, nullptr);
+        if (cfg.dtype == GR_F64) ghost_states<double>(b);
+        else ghost_states<float>(b);
         GR_HIP(hipGetLastError());
+    }
+    template <typename S> void ghost_states(int b) {
+        gr_mesh *m = mesh;
+        if (!m->h.n_ghosts) return;
+        const gr::SolverArgs<S> a = sargs<S>(b);
+        hipLaunchKernelGGL(gr::k_ghost_states<S>, dim3(gr::blocks_for(m->h.n_ghosts)), dim3(gr::kBlock), 0, m->stream,
+                           m->view(), rec<S>(b), a.gnrm, table());
     }
     // Algorithmic bytes per launch: each array element a kernel reads or writes, once.
     void kernel_bytes(double out[4]) const {
@@ -709,14 +703,15 @@ namespace {
 template <typename S>
 void upload_state(gr_solver *s, const double *rho, const double *rhou, const double *rhoe) {
     const size_t n = (size_t)s->mesh->h.n_cells + s->mesh->h.n_ghosts;
-    std::vector<S> soa(5 * s->stride, S(0));
+    std::vector<S> rec(gr::kUR * s->stride, S(0));
     for (size_t i = 0; i < n; ++i) {
-        soa[i] = (S)rho[i];
-        for (int k = 0; k < 3; ++k) soa[(1 + k) * s->stride + i] = (S)rhou[3 * i + k];
-        soa[4 * s->stride + i] = (S)rhoe[i];
+        S *r = &rec[gr::kUR * i];
+        r[0] = (S)rho[i];
+        for (int k = 0; k < 3; ++k) r[1 + k] = (S)rhou[3 * i + k];
+        r[4] = (S)rhoe[i];
     }
     for (int b = 0; b < 2; ++b)
-        GR_HIP(hipMemcpyAsync(s->ubuf[b].p, soa.data(), soa.size() * sizeof(S), hipMemcpyHostToDevice,
+        GR_HIP(hipMemcpyAsync(s->ubuf[b].p, rec.data(), rec.size() * sizeof(S), hipMemcpyHostToDevice,
                               s->mesh->stream));
     GR_HIP(hipStreamSynchronize(s->mesh->stream));
 }
@@ -728,16 +723,16 @@ void download_state(gr_solver *s, double *rho, double *rhou, double *rhoe) {
     const gr_mesh *m = s->mesh;
     const size_t N = m->h.n_cells, n = N + m->h.n_ghosts;
     if (s->ghost_src != s->cur) s->materialise_ghosts(s->ghost_src);
-    std::vector<S> a(5 * s->stride), b(5 * s->stride);
+    std::vector<S> a(gr::kUR * s->stride), b(gr::kUR * s->stride);
     GR_HIP(hipMemcpyAsync(a.data(), s->ubuf[s->cur].p, a.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
     GR_HIP(hipMemcpyAsync(b.data(), s->ubuf[s->ghost_src].p, b.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
     GR_HIP(hipStreamSynchronize(m->stream));
     for (size_t i = 0; i < n; ++i) {
-        const std::vector<S> &src = i < N ? a : b;
-        if (rho) rho[i] = (double)src[i];
+        const S *r = &(i < N ? a : b)[gr::kUR * i];
+        if (rho) rho[i] = (double)r[0];
         if (rhou)
-            for (int k = 0; k < 3; ++k) rhou[3 * i + k] = (double)src[(1 + k) * s->stride + i];
-        if (rhoe) rhoe[i] = (double)src[4 * s->stride + i];
+            for (int k = 0; k < 3; ++k) rhou[3 * i + k] = (double)r[1 + k];
+        if (rhoe) rhoe[i] = (double)r[4];
     }
 }
 
@@ -761,7 +756,7 @@ int gr_solver_create(gr_mesh *m, const gr_solver_config *cfg, gr_solver **out) {
             s->stride = gr::pad64((size_t)m->h.n_cells + m->h.n_ghosts);
             const size_t blk = s->stride * s->es();
             for (auto &b : s->ubuf) {
-                b.alloc(5 * blk);
+                b.alloc(gr::kUR * blk);
                 GR_HIP(hipMemsetAsync(b.p, 0, b.n, m->stream));
             }
             s->grad.alloc((size_t)gr::kGL * blk);
@@ -808,12 +803,17 @@ int gr_solver_init_freestream(gr_solver *s, const double mach[3], double pressur
         require(s && mach, "null argument");
         double fs[5];
         freestream(mach, pressure, temperature, fs);
-        const size_t n = (size_t)s->mesh->h.n_cells + s->mesh->h.n_ghosts;
-        for (int b = 0; b < 2; ++b)
-            for (int c = 0; c < 5; ++c) {
-                if (s->cfg.dtype == GR_F64) fill<double>(s->u<double>(b, c), n, fs[c], s->mesh->stream);
-                else fill<float>(s->u<float>(b, c), n, (float)fs[c], s->mesh->stream);
-            }
+        const size_t n = s->stride;
+        const dim3 grid(gr::blocks_for(n * gr::kUR));
+        for (int b = 0; b < 2; ++b) {
+            if (s->cfg.dtype == GR_F64)
+                hipLaunchKernelGGL(gr::k_fill_states<double>, grid, dim3(gr::kBlock), 0, s->mesh->stream,
+                                   s->rec<double>(b), n, fs[0], fs[1], fs[2], fs[3], fs[4]);
+            else
+                hipLaunchKernelGGL(gr::k_fill_states<float>, grid, dim3(gr::kBlock), 0, s->mesh->stream,
+                                   s->rec<float>(b), n, (float)fs[0], (float)fs[1], (float)fs[2], (float)fs[3],
+                                   (float)fs[4]);
+        }
         s->ghost_src = s->cur;
         GR_HIP(hipGetLastError());
         GR_HIP(hipStreamSynchronize(s->mesh->stream));
@@ -946,7 +946,7 @@ int gr_solver_force(gr_solver *s, const char *region, double F[3]) {
     return guarded([&] {
         require(s && region && F, "null argument");
         const uint32_t r = region_of(s->mesh, region);
-        int rc = gr_internal_wall_force(s->mesh, s->ubuf[s->cur].p, s->cfg.dtype, s->stride, 1, 0, r, F);
+        int rc = gr_internal_wall_force(s->mesh, s->ubuf[s->cur].p, s->cfg.dtype, 1, gr::kUR, 0, r, F);
         if (rc != GR_OK) throw gr::Error(rc, gr_last_error());
     });
 }

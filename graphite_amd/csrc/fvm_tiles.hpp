@@ -200,10 +200,51 @@ __global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_re
     }
 }
 
+// The solver's state as one record per cell and ghost: rho, rho*u (3), rho*E, then padding to
+// kUR values (64 bytes in binary64, 32 in binary32), so a neighbour's state is one aligned
+// segment of a cache line where five arrays would touch five lines.
+constexpr int kUR = 8;
+
+template <typename S>
+__device__ inline void load_state(const S *__restrict__ u, uint32_t i, S v[5]) {
+    constexpr int W = sizeof(S) == 8 ? 6 : 8;
+    S w[W];
+    load_vec<S, W>(u + (size_t)i * kUR, w);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) v[c] = w[c];
+}
+
+// Ghost states of buffer u from its interior cells, for regions with a boundary condition (the
+// iteration evaluates them on the fly and never stores them; this serves state downloads).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_ghost_states(MeshView m, S *u, const S *__restrict__ ghost_rec, BcTable t) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= m.G) return;
+    const int r = bc_region(t, g);
+    if (r < 0) return;
+    const uint32_t i = m.cc[m.cfs[m.N + g]];
+    S ui[5], n[4], ug[5];
+    load_vec<S, 4>(ghost_rec + (size_t)g * kGhostRec, n);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ui[c] = u[(size_t)i * kUR + c];
+    ghost_state<S>(t.r[r], n, ui, ug);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) u[((size_t)m.N + g) * kUR + c] = ug[c];
+}
+
+// Fills n state records with the values v[0..4] (padding zero).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_fill_states(S *u, size_t n, S v0, S v1, S v2, S v3, S v4) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n * kUR) return;
+    const int r = (int)(k & (kUR - 1));
+    u[k] = r == 0 ? v0 : r == 1 ? v1 : r == 2 ? v2 : r == 3 ? v3 : r == 4 ? v4 : S(0);
+}
+
 template <typename S>
 struct SolverArgs {
-    const S *u[5];    // current state (cells; ghosts only for regions without a BC)
-    S *un[5];         // next state
+    const S *u;       // current state records (cells; ghosts only for regions without a BC)
+    S *un;            // next state records
     S *gl;            // gradient + limiter records of the interior cells, kGL values each
     const S *frec;    // flux records, kFluxRec<S> per interior face slot
     const S *grec;    // gradient records, kGradRec per interior face slot
@@ -247,16 +288,14 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         lc = tile_cell_of(s_start, ncell, (uint32_t)t);
         const uint32_t i = c0 + lc;
         q = q0 + t;
-#pragma unroll
-        for (int c = 0; c < 5; ++c) ui[c] = a.u[c][i];
+        load_state<S>(a.u, i, ui);
         const uint32_t j = m.cc[q];
         S dw[4];   // d = x_j - x_i, 1/|d|^2
         load_vec<S, 4>(a.grec + (size_t)q * kGradRec, dw);
         S uj[5];
         const int r = j >= m.N ? bc_region(bc, j - m.N) : -1;
         if (r < 0) {
-#pragma unroll
-            for (int c = 0; c < 5; ++c) uj[c] = a.u[c][j];
+            load_state<S>(a.u, j, uj);
         } else {
             S n[4];
             load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, n);
@@ -285,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         bsb = (int)s_start[bc_l];
         const int se = (int)s_start[bc_l + 1];
         S g3[3] = {S(0), S(0), S(0)};
-        bmin = bmax = a.u[bc_c][ic];
+        bmin = bmax = a.u[(size_t)ic * kUR + bc_c];
         for (int s = bsb; s < se; ++s) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) g3[k] += s_buf[3 * bc_c + k][s];
@@ -365,23 +404,20 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
         const S d0[3] = {geo[4], geo[5], geo[6]};
         const S d1[3] = {geo[7], geo[8], geo[9]};
         S ui[5], gi[15], li[5];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) ui[c] = a.u[c][i];
+        load_state<S>(a.u, i, ui);
         load_gl<S>(a.gl, i, gi, li);
         S r0, ru0[3], e0, r1, ru1[3], e1;
         reconstruct<S, CORR>(ui, gi, li, d0, r0, ru0, e0);
         if (j < m.N) {
             S uj[5], gj[15], lj[5];
-#pragma unroll
-            for (int c = 0; c < 5; ++c) uj[c] = a.u[c][j];
+            load_state<S>(a.u, j, uj);
             load_gl<S>(a.gl, j, gj, lj);
             reconstruct<S, CORR>(uj, gj, lj, d1, r1, ru1, e1);
         } else {   // ghost: its state from the BC, gradient and limiter of cell i
             S ug[5];
             const int r = bc_region(bc, j - m.N);
             if (r < 0) {
-#pragma unroll
-                for (int c = 0; c < 5; ++c) ug[c] = a.u[c][j];
+                load_state<S>(a.u, j, ug);
             } else {
                 ghost_state<S>(bc.r[r], n, ui, ug);
             }
@@ -414,10 +450,11 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
     __syncthreads();
 
     // one thread per (component, cell): accumulate the slots in order, update; the cell's
-    // residual components meet in LDS (row 0 of s_c, free after the reads) for V*|R|^2
+    // residual components (rows of s_d) and new state (rows of s_c) meet in LDS, free after
+    // the reads, for V*|R|^2 and for whole-record state writes
     const bool in_b = t < 5 * ncell;
     const int cc_ = in_b ? t / ncell : 0, cl = in_b ? t - cc_ * ncell : 0;
-    S acc = 0;
+    S acc = 0, unew = 0;
     if (in_b) {
         const uint32_t i = c0 + cl;
         const int sb = (int)s_start[cl], se = (int)s_start[cl + 1];
@@ -427,11 +464,19 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
             dts += s_e[s];
         }
         const S dt = fdiv((S)cfl, dts);
-        a.un[cc_][i] = a.u[cc_][i] + (-acc) * dt;
+        unew = a.u[(size_t)i * kUR + cc_] + (-acc) * dt;
     }
     __syncthreads();
-    if (in_b) s_d[cc_][cl] = acc;
+    if (in_b) {
+        s_d[cc_][cl] = acc;
+        s_c[cc_][cl] = unew;
+    }
     __syncthreads();
+    // the tile's new state records, contiguous in memory, written coalesced
+    for (int k = t; k < kUR * ncell; k += blockDim.x) {
+        const int r = k & (kUR - 1);
+        a.un[(size_t)c0 * kUR + k] = r < 5 ? s_c[r][k / kUR] : S(0);
+    }
     double e_loc = 0;
     if (t < ncell) {
         const double r = s_d[0][t], u0 = s_d[1][t], u1 = s_d[2][t], u2 = s_d[3][t], e = s_d[4][t];
