@@ -14,8 +14,16 @@ namespace gr {
 __global__ __launch_bounds__(kBlock) void k_finalize(const double *partials, uint32_t n, double *out, SolverCtl *ctl,
                                                      double *hist) {
     if (ctl && ctl->stop) return;
-    double acc = 0;
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) acc += partials[k];
+    // eight independent loads in flight per thread (a single running sum waits on each load in
+    // turn), the eight sums then combined in a fixed order
+    double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t st = blockDim.x;
+    uint32_t k = threadIdx.x;
+    for (; k + 7 * st < n; k += 8 * st)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a8[u] += partials[k + u * st];
+    for (; k < n; k += st) a8[0] += partials[k];
+    const double acc = ((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]));
     const double tot = block_sum(acc);
     if (threadIdx.x != 0) return;
     if (!ctl) {
