@@ -559,6 +559,7 @@ struct gr_solver {
     size_t stride = 0;
     gr::DevArray<char> ubuf[2], grad;   // grad: gradient + limiter records (kGL per cell)
     gr::DevArray<char> slot_geo;        // flux, gradient and ghost geometry records
+    gr::DevArray<float> lsq32;          // binary32 LSQ matrices (binary64 solvers use the mesh's)
     int cur = 0;         // buffer holding the current interior state
     int ghost_src = 0;   // buffer holding the latest ghost values
     struct Bc { bool set = false; int kind = 0; double mach[3] = {0, 0, 0}, p = 0, T = 0; };
@@ -598,6 +599,13 @@ struct gr_solver {
         a.frec = (const S *)slot_geo.p;
         a.grec = a.frec + slots * gr::kFluxRec<S>;
         a.gnrm = a.grec + slots * gr::kGradRec;
+        if constexpr (sizeof(S) == 8) {
+            a.lsq = (const S *)mesh->mat.p;
+            a.lsq_stride = std::max<size_t>(64, mesh->mstride());
+        } else {
+            a.lsq = (const S *)lsq32.p;
+            a.lsq_stride = std::max<size_t>(64, mesh->mstride());
+        }
         return a;
     }
     // Per-slot geometry records of the tile kernels, from the mesh metrics.
@@ -606,7 +614,11 @@ struct gr_solver {
         const size_t slots = m->h.cell_faces_starts[m->h.n_cells];
         slot_geo.alloc(std::max<size_t>(1, (slots * (gr::kFluxRec<S> + gr::kGradRec) + (size_t)m->h.n_ghosts * gr::kGhostRec) *
                                               sizeof(S)));
+        if (sizeof(S) == 4) lsq32.alloc(6 * std::max<size_t>(64, m->mstride()));
         const gr::SolverArgs<S> a = sargs<S>(0);
+        if (m->h.n_cells && sizeof(S) == 4)
+            hipLaunchKernelGGL(gr::k_cast_lsq<float>, dim3(gr::blocks_for(m->h.n_cells)), dim3(gr::kBlock), 0,
+                               m->stream, m->view(), lsq32.p, a.lsq_stride);
         if (m->h.n_cells)
             hipLaunchKernelGGL(gr::k_slot_geometry<S>, dim3(gr::blocks_for(m->h.n_cells)), dim3(gr::kBlock), 0, m->stream,
                                m->view(), (S *)a.frec, (S *)a.grec, (S *)a.gnrm);

@@ -241,6 +241,15 @@ __global__ __launch_bounds__(kBlock) void k_fill_states(S *u, size_t n, S v0, S 
     u[k] = r == 0 ? v0 : r == 1 ? v1 : r == 2 ? v2 : r == 3 ? v3 : r == 4 ? v4 : S(0);
 }
 
+// LSQ matrices in S (6 blocks of stride values): the cast the gradient applies, done once.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_cast_lsq(MeshView m, S *out, size_t stride) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.N) return;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k * stride + i] = (S)m.mat[k * m.mat_stride + i];
+}
+
 template <typename S>
 struct SolverArgs {
     const S *u;       // current state records (cells; ghosts only for regions without a BC)
@@ -249,6 +258,8 @@ struct SolverArgs {
     const S *frec;    // flux records, kFluxRec<S> per interior face slot
     const S *grec;    // gradient records, kGradRec per interior face slot
     const S *gnrm;    // ghost records, kGhostRec per ghost
+    const S *lsq;     // LSQ matrices of the interior cells in S, 6 blocks: xx,yy,zz,xy,xz,yz
+    size_t lsq_stride;
 };
 
 // The record of cell i: gradient rows into g[15], limiters into l[5].
@@ -277,6 +288,19 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     const uint32_t q0 = m.cfs[c0];
     const int nslot = (int)(m.cfs[c1] - q0);
     const int t = threadIdx.x;
+    // loads of the per-(component, cell) phase B (the cell's LSQ matrix and value), issued first
+    // so that their latency hides under the slot gathers of phase A
+    const bool in_b = t < 5 * ncell;
+    const int bc_c = in_b ? t / ncell : 0, bc_l = in_b ? t - bc_c * ncell : 0;
+    S bmat[6], bu = S(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bmat[k] = S(0);
+    if (in_b) {
+        const uint32_t ic = c0 + bc_l;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) bmat[k] = a.lsq[k * a.lsq_stride + ic];
+        bu = a.u[(size_t)ic * kUR + bc_c];
+    }
     for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = m.cfs[c0 + k] - q0;
     __syncthreads();
 
@@ -315,23 +339,23 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     // gradient goes to the cell's first slot column (rows 3c..3c+2), min/max to its second
     // (rows c and 5+c) -- rows other threads of the cell still read, hence the barrier between
     // the reads and the writes.
-    const bool in_b = t < 5 * ncell;
-    const int bc_c = in_b ? t / ncell : 0, bc_l = in_b ? t - bc_c * ncell : 0;
     S bg[3] = {S(0), S(0), S(0)}, bmin = S(0), bmax = S(0);
     int bsb = 0;
     if (in_b) {
-        const uint32_t ic = c0 + bc_l;
         bsb = (int)s_start[bc_l];
         const int se = (int)s_start[bc_l + 1];
         S g3[3] = {S(0), S(0), S(0)};
-        bmin = bmax = a.u[(size_t)ic * kUR + bc_c];
+        bmin = bmax = bu;
         for (int s = bsb; s < se; ++s) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) g3[k] += s_buf[3 * bc_c + k][s];
             bmin = fmin(bmin, s_buf[15 + bc_c][s]);
             bmax = fmax(bmax, s_buf[15 + bc_c][s]);
         }
-        lsq_apply<S>(m, ic, g3, bg);
+        // M * g with M symmetric (xx,yy,zz,xy,xz,yz), as lsq_apply
+        bg[0] = (bmat[0] * g3[0] + bmat[3] * g3[1]) + bmat[4] * g3[2];
+        bg[1] = (bmat[3] * g3[0] + bmat[1] * g3[1]) + bmat[5] * g3[2];
+        bg[2] = (bmat[4] * g3[0] + bmat[5] * g3[1]) + bmat[2] * g3[2];
     }
     __syncthreads();
     if (in_b) {
@@ -390,6 +414,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
     const uint32_t q0 = m.cfs[c0];
     const int nslot = (int)(m.cfs[c1] - q0);
     const int t = threadIdx.x;
+    const bool in_b = t < 5 * ncell;   // one thread per (component, cell) in the update
+    const int cc_ = in_b ? t / ncell : 0, cl = in_b ? t - cc_ * ncell : 0;
     for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = m.cfs[c0 + k] - q0;
     __syncthreads();
 
@@ -452,11 +478,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
     // one thread per (component, cell): accumulate the slots in order, update; the cell's
     // residual components (rows of s_d) and new state (rows of s_c) meet in LDS, free after
     // the reads, for V*|R|^2 and for whole-record state writes
-    const bool in_b = t < 5 * ncell;
-    const int cc_ = in_b ? t / ncell : 0, cl = in_b ? t - cc_ * ncell : 0;
     S acc = 0, unew = 0;
     if (in_b) {
-        const uint32_t i = c0 + cl;
         const int sb = (int)s_start[cl], se = (int)s_start[cl + 1];
         S dts = 0;
         for (int s = sb; s < se; ++s) {
@@ -464,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
             dts += s_e[s];
         }
         const S dt = fdiv((S)cfl, dts);
-        unew = a.u[(size_t)i * kUR + cc_] + (-acc) * dt;
+        unew = a.u[(size_t)(c0 + cl) * kUR + cc_] + (-acc) * dt;
     }
     __syncthreads();
     if (in_b) {
