@@ -590,6 +590,11 @@ struct gr_solver {
     // State records (kUR values per cell and ghost, `stride` records) of buffer b.
     template <typename S> S *rec(int b) const { return (S *)ubuf[b].p; }
 
+    // Values taken by the flux records, rounded up so the gradient records start 16-byte aligned.
+    template <typename S> static size_t flux_span(size_t slots) {
+        const size_t w = 16 / sizeof(S);
+        return (slots * gr::kFluxRec + w - 1) / w * w;
+    }
     template <typename S> gr::SolverArgs<S> sargs(int b) const {
         gr::SolverArgs<S> a{};
         a.u = rec<S>(b);
@@ -597,7 +602,7 @@ struct gr_solver {
         a.gl = (S *)grad.p;
         const size_t slots = mesh->h.cell_faces_starts[mesh->h.n_cells];
         a.frec = (const S *)slot_geo.p;
-        a.grec = a.frec + slots * gr::kFluxRec<S>;
+        a.grec = a.frec + flux_span<S>(slots);
         a.gnrm = a.grec + slots * gr::kGradRec;
         if constexpr (sizeof(S) == 8) {
             a.lsq = (const S *)mesh->mat.p;
@@ -612,7 +617,7 @@ struct gr_solver {
     template <typename S> void build_slot_geometry() {
         gr_mesh *m = mesh;
         const size_t slots = m->h.cell_faces_starts[m->h.n_cells];
-        slot_geo.alloc(std::max<size_t>(1, (slots * (gr::kFluxRec<S> + gr::kGradRec) + (size_t)m->h.n_ghosts * gr::kGhostRec) *
+        slot_geo.alloc(std::max<size_t>(1, (flux_span<S>(slots) + slots * gr::kGradRec + (size_t)m->h.n_ghosts * gr::kGhostRec) *
                                               sizeof(S)));
         if (sizeof(S) == 4) lsq32.alloc(6 * std::max<size_t>(64, m->mstride()));
         const gr::SolverArgs<S> a = sargs<S>(0);

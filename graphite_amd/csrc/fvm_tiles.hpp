@@ -115,29 +115,19 @@ __device__ inline int tile_cell_of(const uint32_t *starts, int ncell, uint32_t s
     return lo;
 }
 
-// Loads W consecutive values with 16-byte loads (p 16-byte aligned, W values a whole number
-// of 16-byte words).
-template <typename S, int W>
+// Loads W consecutive values with WB-byte loads (p WB-byte aligned, W values a whole number of
+// WB-byte words; WB 16 or 8).
+template <typename S, int W, int WB = 16>
 __device__ inline void load_vec(const S *__restrict__ p, S (&v)[W]) {
-    static_assert((W * sizeof(S)) % 16 == 0, "records are whole 16-byte words");
-    if constexpr (sizeof(S) == 8) {
-        const double2 *q = reinterpret_cast<const double2 *>(p);
+    constexpr int L = WB / (int)sizeof(S);
+    static_assert(L >= 1 && W % L == 0, "records are whole load words");
+    using word = S __attribute__((ext_vector_type(L)));
+    const word *q = reinterpret_cast<const word *>(p);
 #pragma unroll
-        for (int k = 0; k < W / 2; ++k) {
-            const double2 t = q[k];
-            v[2 * k] = t.x;
-            v[2 * k + 1] = t.y;
-        }
-    } else {
-        const float4 *q = reinterpret_cast<const float4 *>(p);
+    for (int k = 0; k < W / L; ++k) {
+        const word t = q[k];
 #pragma unroll
-        for (int k = 0; k < W / 4; ++k) {
-            const float4 t = q[k];
-            v[4 * k] = t.x;
-            v[4 * k + 1] = t.y;
-            v[4 * k + 2] = t.z;
-            v[4 * k + 3] = t.w;
-        }
+        for (int e = 0; e < L; ++e) v[L * k + e] = t[e];
     }
 }
 
@@ -154,11 +144,11 @@ constexpr int kGL = 20;
 // order, one contiguous run per tile, where the metrics they replace are gathered by face
 // and neighbour id.
 //   flux record:     n = outer_normal(n_f, fc - x_i) (3), ds = A_f / V_i, d0 = fc - x_i (3),
-//                    d1 = fc - x_j (3), padding to 16 bytes
+//                    d1 = fc - x_j (3): 80 bytes in binary64 (16-byte loads), 40 in binary32
+//                    (8-byte loads)
 //   gradient record: d = x_j - x_i (3), 1/|d|^2, d0 (3), padding
 //   ghost record:    n of the ghost's face seen from its cell (3), padding (fused BCs)
-template <typename S>
-constexpr int kFluxRec = sizeof(S) == 8 ? 10 : 12;
+constexpr int kFluxRec = 10;
 constexpr int kGradRec = 8;
 constexpr int kGhostRec = 4;
 
@@ -173,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_re
         outward_normal<S>(m, f, xi, yi, zi, n);
         const double fcx = m.fcx[f], fcy = m.fcy[f], fcz = m.fcz[f];
         const S d0[3] = {(S)(fcx - xi), (S)(fcy - yi), (S)(fcz - zi)};
-        S *fr = flux_rec + (size_t)q * kFluxRec<S>;
+        S *fr = flux_rec + (size_t)q * kFluxRec;
 #pragma unroll
         for (int k = 0; k < 3; ++k) fr[k] = n[k];
         fr[3] = geo_ratio<S>(m.fa[f], m.cv[i]);
@@ -182,7 +172,6 @@ __global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_re
         fr[7] = (S)(fcx - m.cx[j]);
         fr[8] = (S)(fcy - m.cy[j]);
         fr[9] = (S)(fcz - m.cz[j]);
-        for (int k = 10; k < kFluxRec<S>; ++k) fr[k] = S(0);
         const S d[3] = {(S)(m.cx[j] - xi), (S)(m.cy[j] - yi), (S)(m.cz[j] - zi)};
         S *gq = grad_rec + (size_t)q * kGradRec;
 #pragma unroll
@@ -255,7 +244,7 @@ struct SolverArgs {
     const S *u;       // current state records (cells; ghosts only for regions without a BC)
     S *un;            // next state records
     S *gl;            // gradient + limiter records of the interior cells, kGL values each
-    const S *frec;    // flux records, kFluxRec<S> per interior face slot
+    const S *frec;    // flux records, kFluxRec per interior face slot
     const S *grec;    // gradient records, kGradRec per interior face slot
     const S *gnrm;    // ghost records, kGhostRec per ghost
     const S *lsq;     // LSQ matrices of the interior cells in S, 6 blocks: xx,yy,zz,xy,xz,yz
@@ -423,8 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
         const int lc = tile_cell_of(s_start, ncell, (uint32_t)t);
         const uint32_t i = c0 + lc, q = q0 + t;
         const uint32_t j = m.cc[q];
-        S geo[kFluxRec<S>];
-        load_vec<S, kFluxRec<S>>(a.frec + (size_t)q * kFluxRec<S>, geo);
+        S geo[kFluxRec];
+        load_vec<S, kFluxRec, 2 * sizeof(S) == 16 ? 16 : 8>(a.frec + (size_t)q * kFluxRec, geo);
         const S n[3] = {geo[0], geo[1], geo[2]};
         const S ds = geo[3];
         const S d0[3] = {geo[4], geo[5], geo[6]};
