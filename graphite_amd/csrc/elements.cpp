@@ -455,9 +455,13 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
     // spread evenly over [0, phi_b] (not at the station's own height), so cells near the far
     // field are not flattened to the station spacing; the stations above the tip continue at
     // the same latitude step up to phi_top.  Station heights are half uniform, half clustered
-    // towards the tip, so the tip spacing stays within a factor two of the mean.
+    // towards the tip, so the tip spacing stays within a factor two of the mean.  Above phi_top
+    // the dome holds the top layout of every ring (below); at 70 degrees its disc has a radius
+    // of 0.34 R, room for rings a fair fraction of the cap's top steps apart (at 85 degrees, 0.09
+    // R, the column's top cells were slivers with face-area ratios above 1e5, whose residual
+    // oscillated without end and held the solve at a drop of 1e-2).
     const double phi_b = 60.0 * pi / 180.0;
-    const double phi_top = std::min(85.0 * pi / 180.0, phi_b * (1.0 + (double)nc / ns));
+    const double phi_top = std::min(70.0 * pi / 180.0, phi_b * (1.0 + (double)nc / ns));
     require(R * std::sin(phi_b) > 2 * b, "wing: far field too close to the tip");
     auto station_z = [&](uint32_t k) { return b * (0.5 * k / ns + 0.5 * std::sin(0.5 * pi * k / ns)); };
     for (uint32_t k = 0; k <= ns; ++k) {
@@ -544,43 +548,74 @@ void gen_wing(const gr_wing_params &p, gr_elements &e) {
         const double c_tip = p.taper, xm = b * tsw + 0.5 * c_tip;
         const double rho_top = R * std::cos(phi_top);
         // Top layout in elliptic coordinates about the plate: x = xm + ae cosh(mu) cos(nu),
-        // y = -ae sinh(mu) sin(nu), nu = the section parameter of node i, mu = j dmu.  The map is
-        // conformal, so with dmu equal to the mean step in nu the cells are close to square
-        // everywhere, around the plate's ends too; the plate's half width ae follows from that
-        // (at least half the tip chord).  The outermost ellipse is bent onto the circle of
-        // latitude phi_top, the correction growing as cosh(mu) - 1: the rings' spacing next to the
-        // plate's ends grows that way, so the correction never turns them back.
+        // y = -ae sinh(mu) sin(nu), nu = the section parameter of node i.  The map is conformal,
+        // so the cells' shape follows the steps in mu against those in nu; the plate's half width
+        // ae is at least half the tip chord.  Ring j sits on the ellipse at distance t_j from the
+        // plate (ae sinh(mu_j) = t_j), the t_j growing geometrically from 0.07 tip chords to the
+        // outermost ellipse: evenly spaced mu would put the first ring 0.015 chords from the
+        // plate, under cap layers chords tall.  The outermost ellipse is bent onto the circle of
+        // latitude phi_top, the correction growing as cosh(mu) - 1: the rings' spacing next to
+        // the plate's ends grows that way, so the correction never turns them back.
         const double ae = std::max(0.5 * c_tip, rho_top / std::cosh(2.0 * pi * nj / ni));
         require(rho_top > 2 * ae, "wing: far field too close to the tip");
         const double mu_max = std::acosh(rho_top / ae);
-        std::vector<double> t(nj + 1, 0.0);
-        for (uint32_t j = 0; j <= nj; ++j) t[j] = ae * std::sinh(mu_max * j / nj);
+        std::vector<double> mu_j(nj + 1, 0.0);
+        {
+            const double tn = ae * std::sinh(mu_max), t1 = std::min(0.07 * c_tip, 0.5 * tn / nj);
+            const double g = stretch_ratio(t1 / tn, nj);
+            std::vector<double> tt(nj + 1, 0.0);
+            double acc = 0, h = t1;
+            for (uint32_t j = 1; j <= nj; ++j) {
+                acc += h;
+                h *= g;
+                tt[j] = acc;
+            }
+            for (uint32_t j = 1; j < nj; ++j) mu_j[j] = std::asinh(tn * (tt[j] / tt[nj]) / ae);
+            mu_j[nj] = mu_max;
+        }
         std::vector<V3> T((size_t)ni * (nj + 1));
         for (uint32_t i = 0; i < ni; ++i) {
             const double nu = beta_of[i];
             const double ox = cx0 + rho_top * std::cos(th[i]), oy = cy0 + rho_top * std::sin(th[i]);
             const double ex = xm + ae * std::cosh(mu_max) * std::cos(nu), ey = -ae * std::sinh(mu_max) * std::sin(nu);
             for (uint32_t j = 0; j <= nj; ++j) {
-                const double mu = mu_max * j / nj, w = (std::cosh(mu) - 1.0) / (std::cosh(mu_max) - 1.0);
+                const double mu = mu_j[j], w = (std::cosh(mu) - 1.0) / (std::cosh(mu_max) - 1.0);
                 const double x = xm + ae * std::cosh(mu) * std::cos(nu) + w * (ox - ex);
                 const double y = (j == 0 ? 0.0 : -ae * std::sinh(mu) * std::sin(nu)) + w * (oy - ey);
                 const double rr = (x - cx0) * (x - cx0) + (y - cy0) * (y - cy0);
                 T[(size_t)j * ni + i] = {x, y, j == nj ? R * std::sin(phi_top) : std::sqrt(std::max(0.0, R * R - rr))};
             }
         }
-        // cap heights: geometric from the last span step up to the dome
+        // Cap heights: steps growing by 1.6 from the last span step up to a largest step hm,
+        // which the remaining steps keep, hm chosen so the steps reach the dome (a plain
+        // geometric progression over n_cap steps ends in one several times the mean, and the
+        // column's top cells with it).  Fractions sq[q] of the height H from the tip to the dome.
         const double dz_tip = station_z(ns) - station_z(ns - 1);
         const double H = R - b;
-        const double r = stretch_ratio(dz_tip / H, nc);
         std::vector<double> sq(nc + 1, 0.0);
         {
-            double acc = 0, h = dz_tip / H;
-            for (uint32_t q = 1; q <= nc; ++q) {
-                acc += h;
-                h *= r;
-                sq[q] = acc;
+            const double g = 1.6;
+            auto steps = [&](double hm, bool store) {
+                double acc = 0, h = dz_tip;
+                for (uint32_t q = 1; q <= nc; ++q) {
+                    acc += std::min(h, hm);
+                    h *= g;
+                    if (store) sq[q] = acc;
+                }
+                return acc;
+            };
+            double hm = H;   // too few stations to reach the dome with steps growing by g: no cap
+            if (steps(H, false) > H) {
+                double lo = dz_tip, hi = H;
+                for (int it = 0; it < 200; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    (steps(mid, false) > H ? hi : lo) = mid;
+                }
+                hm = 0.5 * (lo + hi);
             }
-            for (uint32_t q = 1; q <= nc; ++q) sq[q] /= sq[nc];
+            steps(hm, true);
+            for (uint32_t q = 1; q < nc; ++q) sq[q] /= sq[nc];
+            sq[nc] = 1.0;
         }
         while (n_taper + 1 <= nc / 2 && sq[n_taper + 1] * H <= 2.0 * p.thickness * p.taper) ++n_taper;
         // Station q above the tip.  Ring j's in-plane layout is B + s (T - B), B the tip station,

@@ -47,6 +47,27 @@ def test_wing_default_size():
     assert s.n_cells == wing_counts(88, 20, 70, 20, 16, nta) == 1069112
 
 
+def test_wing_cap_has_no_slivers(gr, orc):
+    """The column of prisms above the tip reaches the dome in steps a fair fraction of its
+    rings' spacing: its cells' largest/smallest face-area ratio stays below 1e4 (tall slivers
+    there, ratios to 3e5, kept the transonic solve from converging)."""
+    e = gr.Elements.wing()
+    om = orc.OracleMesh(gr.ElementArrays.of(e))
+    N = om.n_cells
+    cfs, cf, fa = om.cell_faces_starts, om.cell_faces, om.face_areas
+    cell = np.repeat(np.arange(N), np.diff(cfs[:N + 1]).astype(np.int64))
+    a = fa[cf[:cfs[N]]]
+    amax = np.zeros(N)
+    np.maximum.at(amax, cell, a)
+    amin = np.full(N, np.inf)
+    np.minimum.at(amin, cell, a)
+    ratio = amax / amin
+    above = om.cell_centers[:N, 2] > 1.6     # above the tip (semispan 1.5)
+    assert om.cell_volumes[:N].min() > 0
+    assert ratio[above].max() < 1e4
+    assert np.percentile(ratio[above], 99) < 2e3
+
+
 def test_wing_geometry(gr, orc):
     R = WING_SMALL["radius"]
     e = gr.Elements.wing(**WING_SMALL)
