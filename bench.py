@@ -270,7 +270,8 @@ def wing_leg(gr, args):
         res[name] = {"value": s.n_cells * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / steps * 1e3,
                      "kernels": {k: {"ms": prof[k], "bytes": kb[k],
                                      "GBps": kb[k] / (prof[k] * 1e-3) / 1e9 if prof[k] > 0 else None,
-                                     "frac_of_peak": kb[k] / (prof[k] * 1e-3) / 1e9 / HBM_PEAK_GBPS if prof[k] > 0 else None}
+                                     "frac_of_peak": kb[k] / (prof[k] * 1e-3) / 1e9 / HBM_PEAK_GBPS if prof[k] > 0 else None,
+                                     "traffic": measured_traffic(f"wing_{name}:{k}")}
                                  for k in prof}}
     sv = solver(gr.GR_F32)
     t0 = time.perf_counter()
@@ -298,6 +299,7 @@ def main():
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-nsi", action="store_true")
     ap.add_argument("--no-wing", action="store_true")
+    ap.add_argument("--only-wing", action="store_true", help="run the swept-wing leg alone (profiling)")
     ap.add_argument("--wing-iters", type=int, default=6000)
     ap.add_argument("--nsi-steps", type=int, default=200)
     ap.add_argument("--warm-flows", type=int, default=512)
@@ -311,6 +313,9 @@ def main():
     import graphite_amd as gr
     if gr.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
+    if args.only_wing:
+        print(json.dumps({"swept_wing": wing_leg(gr, args)}), flush=True)
+        return
 
     log("bench: building O-grid mesh")
     e = elements(gr)

@@ -453,7 +453,8 @@ class Solver:
     KERNELS = ("bc", "grad_lim", "flux_update", "residual_reduce")
 
     def profile(self, n_iter: int) -> dict:
-        """Mean per-kernel duration (ms) over n_iter iterations, HIP events on the library's stream."""
+        """Mean launch duration (ms) of each kernel over n_iter back-to-back launches on the
+        current state (not advanced), HIP events on the library's stream."""
         ms = np.zeros(4)
         _check(load().gr_solver_profile(self._h, n_iter, _p(ms, C.c_double)))
         return dict(zip(self.KERNELS, ms.tolist()))

@@ -496,31 +496,34 @@ struct gr_batch {
         }
         return t;
     }
-    template <typename S> void enqueue_iteration(int b, gr::BatchCtl *c, hipEvent_t *ev) {
+    // Stage 0: gradient+limiter, 1: flux+update, 2: per-instance reduction, of an iteration
+    // reading buffer b.
+    template <typename S> void launch_stage(int stage, int b, gr::BatchCtl *c) {
         gr_mesh *m = mesh;
-        const gr::BcTable t = table();
-        const auto a = args<S>(b);
         const dim3 grid(nblocks()), blk(gr::kBlock);
-        if (ev) GR_HIP(hipEventRecord(ev[0], m->stream));
-        hipLaunchKernelGGL(gr::k_batch_gradlim<S>, grid, blk, 0, m->stream, m->view(), a, t, inst_mach.p,
-                           cfg.limiter, (const gr::BatchCtl *)c);
-        if (ev) GR_HIP(hipEventRecord(ev[1], m->stream));
-        if (cfg.corrected)
-            hipLaunchKernelGGL((gr::k_batch_flux<S, true>), grid, blk, 0, m->stream, m->view(), a, t, inst_mach.p,
-                               cfg.cfl, partials.p, (const gr::BatchCtl *)c);
-        else
-            hipLaunchKernelGGL((gr::k_batch_flux<S, false>), grid, blk, 0, m->stream, m->view(), a, t, inst_mach.p,
-                               cfg.cfl, partials.p, (const gr::BatchCtl *)c);
-        if (ev) GR_HIP(hipEventRecord(ev[2], m->stream));
-        hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream, partials.p,
-                           nblocks(), (const gr::BatchCtl *)c, partials2.p);
-        hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,
-                           partials2.p, gr::kReduceBlocks, c, hist.p, out.p);
-        if (ev) GR_HIP(hipEventRecord(ev[3], m->stream));
+        if (stage == 0) {
+            hipLaunchKernelGGL(gr::k_batch_gradlim<S>, grid, blk, 0, m->stream, m->view(), args<S>(b), table(),
+                               inst_mach.p, cfg.limiter, (const gr::BatchCtl *)c);
+        } else if (stage == 1) {
+            if (cfg.corrected)
+                hipLaunchKernelGGL((gr::k_batch_flux<S, true>), grid, blk, 0, m->stream, m->view(), args<S>(b),
+                                   table(), inst_mach.p, cfg.cfl, partials.p, (const gr::BatchCtl *)c);
+            else
+                hipLaunchKernelGGL((gr::k_batch_flux<S, false>), grid, blk, 0, m->stream, m->view(), args<S>(b),
+                                   table(), inst_mach.p, cfg.cfl, partials.p, (const gr::BatchCtl *)c);
+        } else {
+            hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream, partials.p,
+                               nblocks(), (const gr::BatchCtl *)c, partials2.p);
+            hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,
+                               partials2.p, gr::kReduceBlocks, c, hist.p, out.p);
+        }
     }
-    void enqueue(int b, gr::BatchCtl *c, hipEvent_t *ev = nullptr) {
-        if (cfg.dtype == GR_F64) enqueue_iteration<double>(b, c, ev);
-        else enqueue_iteration<float>(b, c, ev);
+    void enqueue_stage(int stage, int b, gr::BatchCtl *c) {
+        if (cfg.dtype == GR_F64) launch_stage<double>(stage, b, c);
+        else launch_stage<float>(stage, b, c);
+    }
+    void enqueue(int b, gr::BatchCtl *c) {
+        for (int stage = 0; stage < 3; ++stage) enqueue_stage(stage, b, c);
     }
     hipGraphExec_t graph(int b) {
         if (exec[b]) return exec[b];
@@ -916,35 +919,37 @@ int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]) {
         require(b && ms, "null argument");
         require(n_iter > 0 && n_iter <= 10000, "n_iter must be in 1..10000");
         gr_mesh *m = b->mesh;
-        std::vector<hipEvent_t> ev(4 * (size_t)n_iter, nullptr);
+        // Each stage n_iter times back to back on the current state, events around each run
+        // only (per-launch events would add the cost of an event pair to every duration).  The
+        // control block counts the reductions' iterations and is reset by the next iterate.
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         struct Guard {
-            std::vector<hipEvent_t> &e;
+            hipEvent_t *e;
             ~Guard() {
-                for (auto x : e)
-                    if (x) (void)hipEventDestroy(x);
+                for (int k = 0; k < 4; ++k)
+                    if (e[k]) (void)hipEventDestroy(e[k]);
             }
         } guard{ev};
         for (auto &e : ev) GR_HIP(hipEventCreate(&e));
         gr::BatchCtl &c0 = b->ctl_host[0];
         std::memset(&c0, 0, sizeof c0);
-        c0.max_iter = n_iter;
+        c0.max_iter = n_iter + 1;
         c0.n_inst = (int)b->n_inst;
         c0.hist_cap = b->hist_cap;
         for (int k = 0; k < gr::kBatchLanes; ++k) c0.done[k] = k < (int)b->n_inst ? 0 : 1;
         GR_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
-        for (int k = 0; k < n_iter; ++k) b->enqueue((b->cur + k) % 2, b->ctl.p, &ev[4 * (size_t)k]);
+        GR_HIP(hipEventRecord(ev[0], m->stream));
+        for (int stage = 0; stage < 3; ++stage) {
+            for (int k = 0; k < n_iter; ++k) b->enqueue_stage(stage, b->cur, b->ctl.p);
+            GR_HIP(hipEventRecord(ev[stage + 1], m->stream));
+        }
         GR_HIP(hipGetLastError());
         GR_HIP(hipStreamSynchronize(m->stream));
-        double acc[3] = {0, 0, 0};
-        for (int k = 0; k < n_iter; ++k)
-            for (int j = 0; j < 3; ++j) {
-                float t = 0;
-                GR_HIP(hipEventElapsedTime(&t, ev[4 * (size_t)k + j], ev[4 * (size_t)k + j + 1]));
-                acc[j] += t;
-            }
-        for (int j = 0; j < 3; ++j) ms[j] = acc[j] / n_iter;
-        b->ghost_src = (b->cur + n_iter - 1) % 2;
-        b->cur = (b->cur + n_iter) % 2;
+        for (int j = 0; j < 3; ++j) {
+            float t = 0;
+            GR_HIP(hipEventElapsedTime(&t, ev[j], ev[j + 1]));
+            ms[j] = t / n_iter;
+        }
         b->last_iters = 0;
     });
 }

@@ -643,34 +643,35 @@ struct gr_solver {
         }
         return t;
     }
-    // One iteration reading buffer b: gradient+limiter, flux+update into 1-b, norm, with the
-    // boundary conditions evaluated inside both kernels.  ev (optional) receives 5 events:
-    // [0]-[1] the (fused, empty) BC stage, [1]-[2] gradient+limiter, [2]-[3] flux, [3]-[4] norm.
-    template <typename S> void enqueue_iteration(int b, gr::SolverCtl *c, hipEvent_t *ev) {
+    // An iteration reading buffer b: gradient+limiter, flux+update into 1-b, norm, with the
+    // boundary conditions evaluated inside both kernels.
+    template <typename S> void launch_gradlim(int b, gr::SolverCtl *c) {
         gr_mesh *m = mesh;
-        const gr::BcTable t = table();
-        const gr::SolverArgs<S> a = sargs<S>(b);
-        const dim3 grid(m->ntiles), blk(gr::kBlock);
-        if (ev) GR_HIP(hipEventRecord(ev[0], m->stream));
-        if (ev) GR_HIP(hipEventRecord(ev[1], m->stream));
-        hipLaunchKernelGGL(gr::k_gradlim_tile<S>, grid, blk, 0, m->stream, m->view(), a, t, m->tiles.p, m->ntiles,
-                           cfg.limiter, c);
-        if (ev) GR_HIP(hipEventRecord(ev[2], m->stream));
-        if (cfg.corrected)
-            hipLaunchKernelGGL((gr::k_flux_tile<S, true>), grid, blk, 0, m->stream, m->view(), a, t, m->tiles.p,
-                               m->ntiles, cfg.cfl, m->partials.p, c);
-        else
-            hipLaunchKernelGGL((gr::k_flux_tile<S, false>), grid, blk, 0, m->stream, m->view(), a, t, m->tiles.p,
-                               m->ntiles, cfg.cfl, m->partials.p, c);
-        if (ev) GR_HIP(hipEventRecord(ev[3], m->stream));
-        if (c)   // without a control block the caller reduces the partials itself
-            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, m->partials.p, m->ntiles,
-                               (double *)nullptr, c, hist.p);
-        if (ev) GR_HIP(hipEventRecord(ev[4], m->stream));
+        hipLaunchKernelGGL(gr::k_gradlim_tile<S>, dim3(m->ntiles), dim3(gr::kBlock), 0, m->stream, m->view(), sargs<S>(b),
+                           table(), m->tiles.p, m->ntiles, cfg.limiter, c);
     }
-    void enqueue(int b, gr::SolverCtl *c, hipEvent_t *ev = nullptr) {
-        if (cfg.dtype == GR_F64) enqueue_iteration<double>(b, c, ev);
-        else enqueue_iteration<float>(b, c, ev);
+    template <typename S> void launch_flux(int b, gr::SolverCtl *c) {
+        gr_mesh *m = mesh;
+        const dim3 grid(m->ntiles), blk(gr::kBlock);
+        if (cfg.corrected)
+            hipLaunchKernelGGL((gr::k_flux_tile<S, true>), grid, blk, 0, m->stream, m->view(), sargs<S>(b), table(),
+                               m->tiles.p, m->ntiles, cfg.cfl, m->partials.p, c);
+        else
+            hipLaunchKernelGGL((gr::k_flux_tile<S, false>), grid, blk, 0, m->stream, m->view(), sargs<S>(b), table(),
+                               m->tiles.p, m->ntiles, cfg.cfl, m->partials.p, c);
+    }
+    // One iteration reading buffer b; without a control block the caller reduces the partials.
+    void enqueue(int b, gr::SolverCtl *c) {
+        if (cfg.dtype == GR_F64) {
+            launch_gradlim<double>(b, c);
+            launch_flux<double>(b, c);
+        } else {
+            launch_gradlim<float>(b, c);
+            launch_flux<float>(b, c);
+        }
+        if (c)
+            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, mesh->stream, mesh->partials.p,
+                               mesh->ntiles, (double *)nullptr, c, hist.p);
     }
     // Writes the boundary-condition ghost states of buffer b's interior into buffer b (the
     // iteration kernels evaluate them on the fly and never store them).
@@ -707,7 +708,7 @@ struct gr_solver {
         hipGraph_t g = nullptr;
         GR_HIP(hipStreamBeginCapture(mesh->stream, hipStreamCaptureModeRelaxed));
         try {
-            for (int k = 0; k < graph_iters; ++k) enqueue((b + k) % 2, ctl.p, nullptr);
+            for (int k = 0; k < graph_iters; ++k) enqueue((b + k) % 2, ctl.p);
         } catch (...) {
             (void)hipStreamEndCapture(mesh->stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -883,7 +884,7 @@ int gr_solver_iterate(gr_solver *s, int max_iter, double tol, double reference, 
             if (s->cfg.use_graph) {
                 GR_HIP(hipGraphLaunch(s->graph(s->cur), m->stream));
             } else {
-                for (int k = 0; k < K; ++k) s->enqueue((s->cur + k) % 2, s->ctl.p, nullptr);
+                for (int k = 0; k < K; ++k) s->enqueue((s->cur + k) % 2, s->ctl.p);
                 GR_HIP(hipGetLastError());
             }
             gr::SolverCtl *poll = &s->ctl_host[1 + (L & 1)];
@@ -924,7 +925,7 @@ int gr_solver_history(const gr_solver *s, double *hist, int n) {
 int gr_solver_residual_norm(gr_solver *s, double *out) {
     return guarded([&] {
         require(s && out, "null argument");
-        s->enqueue(s->cur, nullptr, nullptr);   // BCs, gradient+limiter, flux partials; state not swapped
+        s->enqueue(s->cur, nullptr);   // gradient+limiter, flux partials; state not swapped
         GR_HIP(hipGetLastError());
         *out = finalize_sum(s->mesh, s->mesh->ntiles);
         s->ghost_src = s->cur;
@@ -936,34 +937,42 @@ int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]) {
         require(s && ms, "null argument");
         require(n_iter > 0 && n_iter <= 100000, "n_iter must be in 1..100000");
         gr_mesh *m = s->mesh;
-        std::vector<hipEvent_t> ev(5 * (size_t)n_iter, nullptr);
+        // Each kernel n_iter times back to back on the current state (its launches read the same
+        // inputs and write the same outputs), with events around each run only: per-launch
+        // events would add the cost of an event pair (about 5 us) to every duration.
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         struct Guard {
-            std::vector<hipEvent_t> &e;
+            hipEvent_t *e;
             ~Guard() {
-                for (auto x : e)
-                    if (x) (void)hipEventDestroy(x);
+                for (int k = 0; k < 4; ++k)
+                    if (e[k]) (void)hipEventDestroy(e[k]);
             }
         } guard{ev};
         for (auto &e : ev) GR_HIP(hipEventCreate(&e));
-        gr::SolverCtl &c0 = s->ctl_host[0];
-        std::memset(&c0, 0, sizeof c0);
-        c0.max_iter = n_iter;
-        c0.hist_cap = s->hist_cap;
-        GR_HIP(hipMemcpyAsync(s->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
-        for (int k = 0; k < n_iter; ++k) s->enqueue((s->cur + k) % 2, s->ctl.p, &ev[5 * (size_t)k]);
+        const int b = s->cur;
+        GR_HIP(hipEventRecord(ev[0], m->stream));
+        for (int k = 0; k < n_iter; ++k) {
+            if (s->cfg.dtype == GR_F64) s->launch_gradlim<double>(b, nullptr);
+            else s->launch_gradlim<float>(b, nullptr);
+        }
+        GR_HIP(hipEventRecord(ev[1], m->stream));
+        for (int k = 0; k < n_iter; ++k) {
+            if (s->cfg.dtype == GR_F64) s->launch_flux<double>(b, nullptr);
+            else s->launch_flux<float>(b, nullptr);
+        }
+        GR_HIP(hipEventRecord(ev[2], m->stream));
+        for (int k = 0; k < n_iter; ++k)
+            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, m->partials.p, m->ntiles,
+                               m->scalar_out.p, (gr::SolverCtl *)nullptr, (double *)nullptr);
+        GR_HIP(hipEventRecord(ev[3], m->stream));
         GR_HIP(hipGetLastError());
         GR_HIP(hipStreamSynchronize(m->stream));
-        double acc[4] = {0, 0, 0, 0};
-        for (int k = 0; k < n_iter; ++k)
-            for (int j = 0; j < 4; ++j) {
-                float t = 0;
-                GR_HIP(hipEventElapsedTime(&t, ev[5 * (size_t)k + j], ev[5 * (size_t)k + j + 1]));
-                acc[j] += t;
-            }
-        for (int j = 0; j < 4; ++j) ms[j] = acc[j] / n_iter;
-        s->ghost_src = (s->cur + n_iter - 1) % 2;
-        s->cur = (s->cur + n_iter) % 2;
-        s->last_iters = 0;
+        ms[0] = 0.0;   // boundary conditions are evaluated inside the two kernels
+        for (int j = 0; j < 3; ++j) {
+            float t = 0;
+            GR_HIP(hipEventElapsedTime(&t, ev[j], ev[j + 1]));
+            ms[1 + j] = t / n_iter;
+        }
     });
 }
 

@@ -293,9 +293,10 @@ int gr_solver_iterate(gr_solver *s, int max_iter, double tol, double reference,
 int gr_solver_history(const gr_solver *s, double *hist, int n);
 /* Residual of the current state without updating it (one iteration's BC+prepare+flux). */
 int gr_solver_residual_norm(gr_solver *s, double *out);
-/* Runs n_iter iterations (advancing the state) as stream launches with HIP events recorded
- * around every kernel on the library's stream; ms receives the mean duration in milliseconds
- * of [0] boundary conditions, [1] gradient+limiter, [2] flux+update, [3] residual reduction. */
+/* Mean launch duration in milliseconds of each kernel of an iteration, from n_iter back-to-back
+ * launches of it on the current state (the state is not advanced; HIP events on the library's
+ * stream around each run of launches): [0] boundary conditions (0: evaluated inside [1] and
+ * [2]), [1] gradient+limiter, [2] flux+update, [3] residual reduction. */
 int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]);
 /* Algorithmic bytes one launch of each kernel moves (every array element it reads or writes
  * counted once), same order as gr_solver_profile. */
@@ -334,8 +335,9 @@ int gr_batch_reset_freestream(gr_batch *b, const int *mask);
 int gr_batch_history(const gr_batch *b, uint32_t inst, double *hist, int n);
 /* Residual sum of each instance's current state (n_inst entries), state unchanged. */
 int gr_batch_residual_norms(gr_batch *b, double *out);
-/* Mean kernel durations (ms) over n_iter iterations: [0] gradient+limiter, [1] flux+update,
- * [2] per-instance residual reduction; and the algorithmic bytes of one launch of each. */
+/* Mean launch durations (ms) from n_iter back-to-back launches of each stage on the current
+ * state (not advanced): [0] gradient+limiter, [1] flux+update, [2] per-instance residual
+ * reduction; and the algorithmic bytes of one launch of each. */
 int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]);
 int gr_batch_kernel_bytes(const gr_batch *b, double bytes[3]);
 /* Iterations captured per HIP graph (even, 2..200; default 10). */
