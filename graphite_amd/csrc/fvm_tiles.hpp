@@ -146,11 +146,23 @@ constexpr int kGL = 20;
 //   flux record:     n = outer_normal(n_f, fc - x_i) (3), ds = A_f / V_i, d0 = fc - x_i (3),
 //                    d1 = fc - x_j (3): 80 bytes in binary64 (16-byte loads), 40 in binary32
 //                    (8-byte loads)
-//   gradient record: d = x_j - x_i (3), 1/|d|^2, d0 (3), padding
+//   gradient record: d = x_j - x_i (3), 1/|d|^2, d0 (3), the neighbour id j (its bits)
 //   ghost record:    n of the ghost's face seen from its cell (3), padding (fused BCs)
 constexpr int kFluxRec = 10;
 constexpr int kGradRec = 8;
 constexpr int kGhostRec = 4;
+
+// An index stored in (read from) a value slot of a record: its bits, never used in arithmetic.
+template <typename S>
+__device__ inline S id_bits(uint32_t j) {
+    if constexpr (sizeof(S) == 8) return __longlong_as_double((long long)j);
+    else return __uint_as_float(j);
+}
+template <typename S>
+__device__ inline uint32_t bits_id(S v) {
+    if constexpr (sizeof(S) == 8) return (uint32_t)__double_as_longlong(v);
+    else return __float_as_uint(v);
+}
 
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_rec, S *grad_rec, S *ghost_rec) {
@@ -179,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_re
         gq[3] = fdiv(S(1), (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) gq[4 + k] = d0[k];
-        gq[7] = S(0);
+        gq[7] = id_bits<S>(j);
         if (j >= m.N) {   // a ghost's only face slot belongs to this cell
             S *g = ghost_rec + (size_t)(j - m.N) * kGhostRec;
 #pragma unroll
@@ -295,16 +307,17 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
 
     // A: per-slot gradient terms d*(u_j-u_i)/|d|^2 (rows 0..14) and u_j (rows 15..19)
     int lc = 0;
-    uint32_t q = 0;
-    S ui[5];
+    S ui[5], d0[3] = {S(0), S(0), S(0)};
     if (t < nslot) {
         lc = tile_cell_of(s_start, ncell, (uint32_t)t);
-        const uint32_t i = c0 + lc;
-        q = q0 + t;
+        const uint32_t i = c0 + lc, q = q0 + t;
         load_state<S>(a.u, i, ui);
-        const uint32_t j = m.cc[q];
-        S dw[4];   // d = x_j - x_i, 1/|d|^2
-        load_vec<S, 4>(a.grec + (size_t)q * kGradRec, dw);
+        S dw[kGradRec];   // d = x_j - x_i, 1/|d|^2, fc - x_i (phase C), j
+        load_vec<S, kGradRec>(a.grec + (size_t)q * kGradRec, dw);
+        const uint32_t j = bits_id<S>(dw[7]);
+        d0[0] = dw[4];
+        d0[1] = dw[5];
+        d0[2] = dw[6];
         S uj[5];
         const int r = j >= m.N ? bc_region(bc, j - m.N) : -1;
         if (r < 0) {
@@ -360,8 +373,6 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     // in B): disjoint, so no barrier is needed between the reads and the writes.
     if (t < nslot) {
         const int sb = (int)s_start[lc];
-        S d0[4];   // fc - x_i
-        load_vec<S, 4>(a.grec + (size_t)q * kGradRec + 4, d0);
 #pragma unroll
         for (int c = 0; c < 5; ++c)
             s_buf[15 + c][t] = limiter_face<S>(ui[c], s_buf[3 * c][sb], s_buf[3 * c + 1][sb], s_buf[3 * c + 2][sb],
