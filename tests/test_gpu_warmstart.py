@@ -166,10 +166,9 @@ def test_trained_network_cuts_iterations_and_keeps_lift_drag(gr, small):
           "predictions kept", used.sum())
     assert used.sum() >= 12
     assert np.median(wsx.iterations) < 0.5 * np.median(cs.iterations)
-    # the forces after the same further work on both: the scheme ends in a limit cycle (squared
-    # residual near 1e-5 here), so the comparison is at equal work, against the cold solve's own
-    # drift, not at a residual level
-    agree = ws.force_agreement(cold, warm, test, "airfoil", 0.1, 20000)
+    # converged forces: the scheme ends in a limit cycle (squared residual near 1e-5 here), so
+    # both are settled by the same further work and averaged over a window of iterations
+    agree = ws.force_agreement(cold, warm, test, "airfoil", 0.1, 20000, 4000, 250)
     print("force agreement:", agree)
-    dcl, dcd = agree["max_rel_diff_cl_cd"]
-    assert dcl < 5e-3 and dcd < 5e-3
+    assert agree["samples"] == 16 and agree["lifting_flows"] >= 1
+    assert agree["mean_cl_rel_diff_p50_max"][1] < 5e-3 and agree["mean_cd_rel_diff_p50_max"][1] < 2e-2
