@@ -46,11 +46,13 @@ def elements(gr):
     return gr.Elements.ogrid(w["n_theta"], w["n_r"], w["n_z"], w["r0"], w["r1"], w["ratio"], w["dz"])
 
 
-def make_solver(gr, m, dtype, cfl):
+def make_solver(gr, m, dtype, cfl, tune=True):
+    """Solver of the O-grid case; with tune, the tile size is chosen by the autotuner."""
     sv = gr.Solver(m, dtype=dtype, cfl=cfl)
     for name, kind in BCS:
         sv.set_bc(name, gr.GR_BC_FARFIELD if kind == "farfield" else gr.GR_BC_SLIP, MACH, P_INF, T_INF)
     sv.init_freestream(MACH, P_INF, T_INF)
+    sv.tile_cells = sv.autotune(20) if tune else 51
     return sv
 
 
@@ -256,6 +258,7 @@ def wing_leg(gr, args):
         for name, kind in (("wall", "slip"), ("symmetry", "slip"), ("farfield", "farfield")):
             sv.set_bc(name, gr.GR_BC_FARFIELD if kind == "farfield" else gr.GR_BC_SLIP, mach, P_INF, T_INF)
         sv.init_freestream(mach, P_INF, T_INF)
+        sv.tile_cells = sv.autotune(10)
         return sv
 
     res = {"workload": "swept NACA0012 wing, 30 deg sweep, taper 0.56, semispan 1.5, prism/pyramid/tet mesh "
@@ -269,6 +272,7 @@ def wing_leg(gr, args):
         prof = sv.profile(20)
         kb = sv.kernel_bytes()
         res[name] = {"value": s.n_cells * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / steps * 1e3,
+                     "tile_cells": sv.tile_cells,
                      "kernels": {k: {"ms": prof[k], "bytes": kb[k],
                                      "GBps": kb[k] / (prof[k] * 1e-3) / 1e9 if prof[k] > 0 else None,
                                      "frac_of_peak": kb[k] / (prof[k] * 1e-3) / 1e9 / HBM_PEAK_GBPS if prof[k] > 0 else None,
@@ -365,7 +369,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cylinder O-grid 256x128x4 hexes (131,072 cells), 3-D Euler M=0.3, "
                                "second order + reference limiter, plain FP64 solver",
-                   "cells": s.n_cells, "ghosts": s.n_ghosts, "faces": s.n_faces, "cfl": args.cfl},
+                   "cells": s.n_cells, "ghosts": s.n_ghosts, "faces": s.n_faces, "cfl": args.cfl,
+                   "tile_cells": sv.tile_cells},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(dom),
                      "algorithmic_bytes_per_launch": kb[dom], "avg_launch_ms": prof[dom]},

@@ -452,6 +452,13 @@ class Solver:
 
     KERNELS = ("bc", "grad_lim", "flux_update", "residual_reduce")
 
+    def autotune(self, n_iter: int = 20) -> int:
+        """Chooses the tile size (cells per block) whose kernels run fastest on this mesh;
+        returns it.  The state is not advanced."""
+        c = C.c_uint32()
+        _check(load().gr_solver_autotune(self._h, n_iter, C.byref(c)))
+        return c.value
+
     def profile(self, n_iter: int) -> dict:
         """Mean launch duration (ms) of each kernel over n_iter back-to-back launches on the
         current state (not advanced), HIP events on the library's stream."""

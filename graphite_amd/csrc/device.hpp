@@ -86,11 +86,8 @@ struct gr_mesh {
     gr::DevArray<double> partials;   // reduction scratch, one per block of the largest launch
     gr::DevArray<double> scalar_out; // one double
     double *host_scalar = nullptr;   // pinned, one double
-    gr::DevArray<uint32_t> tiles;    // first cell of each slot tile, ntiles+1 entries
-    uint32_t ntiles = 0;
     ~gr_mesh();
     void ensure_device();            // stream + connectivity upload
-    void ensure_tiles(uint32_t max_slots, uint32_t max_cells);
     gr::MeshView view() const;
     size_t fstride() const { return gr::pad64(h.n_faces); }
     size_t cstride() const { return gr::pad64((size_t)h.n_cells + h.n_ghosts); }

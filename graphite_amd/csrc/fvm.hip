@@ -195,8 +195,8 @@ void launch_gradlim5(gr_mesh *m, const gr::GradLimArgs<S, 5> &a, int limiter, co
                        m->stream, m->view(), a, limiter, ctl);
 }
 
-double finalize_sum(gr_mesh *m, uint32_t nparts) {
-    hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, m->partials.p, nparts,
+double finalize_sum(gr_mesh *m, uint32_t nparts, const double *partials = nullptr) {
+    hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, partials ? partials : m->partials.p, nparts,
                        m->scalar_out.p, (gr::SolverCtl *)nullptr, (double *)nullptr);
     GR_HIP(hipGetLastError());
     GR_HIP(hipMemcpyAsync(m->host_scalar, m->scalar_out.p, sizeof(double), hipMemcpyDeviceToHost, m->stream));
@@ -560,6 +560,9 @@ struct gr_solver {
     gr::DevArray<char> ubuf[2], grad;   // grad: gradient + limiter records (kGL per cell)
     gr::DevArray<char> slot_geo;        // flux, gradient and ghost geometry records
     gr::DevArray<float> lsq32;          // binary32 LSQ matrices (binary64 solvers use the mesh's)
+    gr::DevArray<uint32_t> tiles;       // first cell of each tile, ntiles+1 entries
+    uint32_t ntiles = 0, tile_cells = 0;
+    gr::DevArray<double> partials;      // one V*|R|^2 partial per tile
     int cur = 0;         // buffer holding the current interior state
     int ghost_src = 0;   // buffer holding the latest ghost values
     struct Bc { bool set = false; int kind = 0; double mach[3] = {0, 0, 0}, p = 0, T = 0; };
@@ -613,6 +616,68 @@ struct gr_solver {
         }
         return a;
     }
+    // Greedy cut of the interior cells into runs of at most kTileSlots face slots and max_cells
+    // cells (one block of the tile kernels each).  Captured graphs hold the old tiles: dropped.
+    void build_tiles(uint32_t max_cells) {
+        gr_mesh *m = mesh;
+        require(max_cells >= 1 && max_cells <= (uint32_t)gr::kTileCells, "tile cell count out of range");
+        std::vector<uint32_t> t{0};
+        const auto &st = m->h.cell_faces_starts;
+        uint32_t first = 0;
+        for (uint32_t c = 0; c < m->h.n_cells; ++c) {
+            require(st[c + 1] - st[c] <= (uint32_t)gr::kTileSlots, "cell with more face slots than a tile holds");
+            if (st[c + 1] - st[first] > (uint32_t)gr::kTileSlots || c + 1 - first > max_cells) {
+                t.push_back(c);
+                first = c;
+            }
+        }
+        if (t.back() != m->h.n_cells) t.push_back(m->h.n_cells);
+        GR_HIP(hipStreamSynchronize(m->stream));   // no launch may still read the old tiles
+        drop_graphs();
+        ntiles = (uint32_t)t.size() - 1;
+        tile_cells = max_cells;
+        tiles.alloc(t.size());
+        tiles.upload(t.data(), t.size(), m->stream);
+        partials.alloc(ntiles + 64);
+        GR_HIP(hipStreamSynchronize(m->stream));
+    }
+    // Mean durations (ms) of n back-to-back launches of the gradient and the flux kernel on the
+    // current state (not advanced), HIP events on the library's stream around each run.
+    void time_kernels(int n, double ms[3]) {
+        gr_mesh *m = mesh;
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        struct Guard {
+            hipEvent_t *e;
+            ~Guard() {
+                for (int k = 0; k < 4; ++k)
+                    if (e[k]) (void)hipEventDestroy(e[k]);
+            }
+        } guard{ev};
+        for (auto &e : ev) GR_HIP(hipEventCreate(&e));
+        const int b = cur;
+        GR_HIP(hipEventRecord(ev[0], m->stream));
+        for (int k = 0; k < n; ++k) {
+            if (cfg.dtype == GR_F64) launch_gradlim<double>(b, nullptr);
+            else launch_gradlim<float>(b, nullptr);
+        }
+        GR_HIP(hipEventRecord(ev[1], m->stream));
+        for (int k = 0; k < n; ++k) {
+            if (cfg.dtype == GR_F64) launch_flux<double>(b, nullptr);
+            else launch_flux<float>(b, nullptr);
+        }
+        GR_HIP(hipEventRecord(ev[2], m->stream));
+        for (int k = 0; k < n; ++k)
+            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, partials.p, ntiles,
+                               m->scalar_out.p, (gr::SolverCtl *)nullptr, (double *)nullptr);
+        GR_HIP(hipEventRecord(ev[3], m->stream));
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(m->stream));
+        for (int j = 0; j < 3; ++j) {
+            float t = 0;
+            GR_HIP(hipEventElapsedTime(&t, ev[j], ev[j + 1]));
+            ms[j] = t / n;
+        }
+    }
     // Per-slot geometry records of the tile kernels, from the mesh metrics.
     template <typename S> void build_slot_geometry() {
         gr_mesh *m = mesh;
@@ -647,18 +712,18 @@ struct gr_solver {
     // boundary conditions evaluated inside both kernels.
     template <typename S> void launch_gradlim(int b, gr::SolverCtl *c) {
         gr_mesh *m = mesh;
-        hipLaunchKernelGGL(gr::k_gradlim_tile<S>, dim3(m->ntiles), dim3(gr::kBlock), 0, m->stream, m->view(), sargs<S>(b),
-                           table(), m->tiles.p, m->ntiles, cfg.limiter, c);
+        hipLaunchKernelGGL(gr::k_gradlim_tile<S>, dim3(ntiles), dim3(gr::kBlock), 0, m->stream, m->view(), sargs<S>(b),
+                           table(), tiles.p, ntiles, cfg.limiter, c);
     }
     template <typename S> void launch_flux(int b, gr::SolverCtl *c) {
         gr_mesh *m = mesh;
-        const dim3 grid(m->ntiles), blk(gr::kBlock);
+        const dim3 grid(ntiles), blk(gr::kBlock);
         if (cfg.corrected)
             hipLaunchKernelGGL((gr::k_flux_tile<S, true>), grid, blk, 0, m->stream, m->view(), sargs<S>(b), table(),
-                               m->tiles.p, m->ntiles, cfg.cfl, m->partials.p, c);
+                               tiles.p, ntiles, cfg.cfl, partials.p, c);
         else
             hipLaunchKernelGGL((gr::k_flux_tile<S, false>), grid, blk, 0, m->stream, m->view(), sargs<S>(b), table(),
-                               m->tiles.p, m->ntiles, cfg.cfl, m->partials.p, c);
+                               tiles.p, ntiles, cfg.cfl, partials.p, c);
     }
     // One iteration reading buffer b; without a control block the caller reduces the partials.
     void enqueue(int b, gr::SolverCtl *c) {
@@ -670,8 +735,8 @@ struct gr_solver {
             launch_flux<float>(b, c);
         }
         if (c)
-            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, mesh->stream, mesh->partials.p,
-                               mesh->ntiles, (double *)nullptr, c, hist.p);
+            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, mesh->stream, partials.p, ntiles,
+                               (double *)nullptr, c, hist.p);
     }
     // Writes the boundary-condition ghost states of buffer b's interior into buffer b (the
     // iteration kernels evaluate them on the fly and never store them).
@@ -701,7 +766,7 @@ struct gr_solver {
         // flux+update: U, gradient, limiter of cells, centres of cells and ghosts, volumes,
         // face centre+normal+area, CSR; new U written.
         out[2] = su * (25 * N) + sg * (3 * (N + G) + N + 7 * F) + 4.0 * (N + 1 + 2 * slots) + su * (5 * N);
-        out[3] = 8.0 * mesh->ntiles;
+        out[3] = 8.0 * ntiles;
     }
     hipGraphExec_t graph(int b) {
         if (exec[b]) return exec[b];
@@ -788,7 +853,7 @@ int gr_solver_create(gr_mesh *m, const gr_solver_config *cfg, gr_solver **out) {
             s->grad.alloc((size_t)gr::kGL * blk);
             GR_HIP(hipMemsetAsync(s->grad.p, 0, s->grad.n, m->stream));
             s->bcs.resize(m->h.n_regions);
-            m->ensure_tiles(gr::kTileSlots, gr::kTileCells);
+            s->build_tiles(gr::kTileCells);
             if (cfg->dtype == GR_F64) s->build_slot_geometry<double>();
             else s->build_slot_geometry<float>();
             s->ctl.alloc(1);
@@ -927,7 +992,7 @@ int gr_solver_residual_norm(gr_solver *s, double *out) {
         require(s && out, "null argument");
         s->enqueue(s->cur, nullptr);   // gradient+limiter, flux partials; state not swapped
         GR_HIP(hipGetLastError());
-        *out = finalize_sum(s->mesh, s->mesh->ntiles);
+        *out = finalize_sum(s->mesh, s->ntiles, s->partials.p);
         s->ghost_src = s->cur;
     });
 }
@@ -936,43 +1001,36 @@ int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]) {
     return guarded([&] {
         require(s && ms, "null argument");
         require(n_iter > 0 && n_iter <= 100000, "n_iter must be in 1..100000");
-        gr_mesh *m = s->mesh;
         // Each kernel n_iter times back to back on the current state (its launches read the same
         // inputs and write the same outputs), with events around each run only: per-launch
         // events would add the cost of an event pair (about 5 us) to every duration.
-        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-        struct Guard {
-            hipEvent_t *e;
-            ~Guard() {
-                for (int k = 0; k < 4; ++k)
-                    if (e[k]) (void)hipEventDestroy(e[k]);
-            }
-        } guard{ev};
-        for (auto &e : ev) GR_HIP(hipEventCreate(&e));
-        const int b = s->cur;
-        GR_HIP(hipEventRecord(ev[0], m->stream));
-        for (int k = 0; k < n_iter; ++k) {
-            if (s->cfg.dtype == GR_F64) s->launch_gradlim<double>(b, nullptr);
-            else s->launch_gradlim<float>(b, nullptr);
-        }
-        GR_HIP(hipEventRecord(ev[1], m->stream));
-        for (int k = 0; k < n_iter; ++k) {
-            if (s->cfg.dtype == GR_F64) s->launch_flux<double>(b, nullptr);
-            else s->launch_flux<float>(b, nullptr);
-        }
-        GR_HIP(hipEventRecord(ev[2], m->stream));
-        for (int k = 0; k < n_iter; ++k)
-            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, m->stream, m->partials.p, m->ntiles,
-                               m->scalar_out.p, (gr::SolverCtl *)nullptr, (double *)nullptr);
-        GR_HIP(hipEventRecord(ev[3], m->stream));
-        GR_HIP(hipGetLastError());
-        GR_HIP(hipStreamSynchronize(m->stream));
         ms[0] = 0.0;   // boundary conditions are evaluated inside the two kernels
-        for (int j = 0; j < 3; ++j) {
-            float t = 0;
-            GR_HIP(hipEventElapsedTime(&t, ev[j], ev[j + 1]));
-            ms[1 + j] = t / n_iter;
+        s->time_kernels(n_iter, ms + 1);
+    });
+}
+
+int gr_solver_autotune(gr_solver *s, int n_iter, uint32_t *tile_cells) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        require(n_iter > 0 && n_iter <= 10000, "n_iter must be in 1..10000");
+        // Candidate tile sizes (cells per block, at most kTileCells, each within kTileSlots
+        // slots): smaller tiles mean more, shorter blocks, which balances a small mesh's few
+        // rounds of resident blocks over the CUs; larger ones amortise each block's set-up.
+        const uint32_t cand[] = {(uint32_t)gr::kTileCells, 44, 40, 36, 32, 28, 24};
+        uint32_t best = (uint32_t)gr::kTileCells;
+        double best_ms = 1e300;
+        for (uint32_t c : cand) {
+            s->build_tiles(c);
+            double ms[3];
+            s->time_kernels(n_iter, ms);
+            const double tot = ms[0] + ms[1] + ms[2];
+            if (tot < best_ms) {
+                best_ms = tot;
+                best = c;
+            }
         }
+        s->build_tiles(best);
+        if (tile_cells) *tile_cells = best;
     });
 }
 

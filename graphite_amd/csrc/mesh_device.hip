@@ -211,29 +211,6 @@ void gr_mesh::ensure_device() {
     on_device = true;
 }
 
-// Greedy cut of the interior cells into runs of at most max_slots face slots and max_cells
-// cells (one block of the slot-parallel kernels each).
-void gr_mesh::ensure_tiles(uint32_t max_slots, uint32_t max_cells) {
-    if (tiles.p) return;
-    ensure_device();
-    std::vector<uint32_t> t{0};
-    const auto &s = h.cell_faces_starts;
-    uint32_t first = 0;
-    for (uint32_t c = 0; c < h.n_cells; ++c) {
-        gr::require(s[c + 1] - s[c] <= max_slots, "cell with more face slots than a tile holds");
-        if (s[c + 1] - s[first] > max_slots || c + 1 - first > max_cells) {
-            t.push_back(c);
-            first = c;
-        }
-    }
-    if (t.back() != h.n_cells) t.push_back(h.n_cells);
-    ntiles = (uint32_t)t.size() - 1;
-    tiles.alloc(t.size());
-    tiles.upload(t.data(), t.size(), stream);
-    if (partials.n < ntiles + 64) partials.alloc(ntiles + 64);
-    GR_HIP(hipStreamSynchronize(stream));
-}
-
 namespace gr {
 const HostMesh &host_mesh(const gr_mesh *m) { return m->h; }
 }  // namespace gr

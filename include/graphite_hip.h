@@ -298,6 +298,12 @@ int gr_solver_residual_norm(gr_solver *s, double *out);
  * stream around each run of launches): [0] boundary conditions (0: evaluated inside [1] and
  * [2]), [1] gradient+limiter, [2] flux+update, [3] residual reduction. */
 int gr_solver_profile(gr_solver *s, int n_iter, double ms[4]);
+/* Tile-size autotuner (no counterpart in the reference): for each candidate tile size (cells per
+ * block of the gradient and flux kernels) times n_iter back-to-back launches of the iteration's
+ * kernels and keeps the fastest for this solver; tile_cells (optional) receives it.  The state
+ * is not advanced, and cell results do not depend on the tiling (residual sums add the same
+ * cell terms in another grouping). */
+int gr_solver_autotune(gr_solver *s, int n_iter, uint32_t *tile_cells);
 /* Algorithmic bytes one launch of each kernel moves (every array element it reads or writes
  * counted once), same order as gr_solver_profile. */
 int gr_solver_kernel_bytes(const gr_solver *s, double bytes[4]);

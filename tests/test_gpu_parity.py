@@ -354,3 +354,29 @@ def test_ogrid_c2_residual_matches_oracle(gr, orc):
     assert normwise(sv.history(3), ohist) <= TOL
     r, u, e_ = sv.get_state()
     assert normwise(r, s.rho) <= TOL and normwise(u, s.rhou) <= TOL and normwise(e_, s.rhoe) <= TOL
+
+
+def test_autotuned_tiles_leave_states_unchanged(gr):
+    """The autotuner only changes how cells are grouped into blocks: after the same iterations
+    the states are bitwise those of the default tiling."""
+    m = gr.Mesh.build(gr.Elements.ogrid(64, 16, 2, 0.5, 10.0, 1.1, 0.05))
+    m.renumber_rcm()
+    m.compute_metrics()
+    mach = (0.3, 0.05, 0.0)
+
+    def run(tune):
+        sv = gr.Solver(m, gr.GR_F64, cfl=0.5)
+        for name in m.region_names():
+            kind = gr.GR_BC_FARFIELD if name == "farfield" else gr.GR_BC_SLIP
+            sv.set_bc(name, kind, mach, P_INF, T_INF)
+        sv.init_freestream(mach, P_INF, T_INF)
+        cells = sv.autotune(3) if tune else 51
+        sv.iterate(40)
+        return cells, sv.get_state()
+
+    c0, s0 = run(False)
+    c1, s1 = run(True)
+    print("autotuned tile size:", c1)
+    assert 1 <= c1 <= 51
+    for a, b in zip(s0, s1):
+        assert np.array_equal(a, b)
