@@ -390,10 +390,25 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         s_buf[15 + bc_c][bsb] = l;
     }
     __syncthreads();
-    // E: the tile's gradient + limiter records, contiguous in memory, written coalesced
-    for (int k = t; k < kGL * ncell; k += blockDim.x) {
-        const int cl = k / kGL, r = k - cl * kGL;
-        a.gl[(size_t)c0 * kGL + k] = s_buf[r][s_start[cl]];
+    // E: the tile's gradient + limiter records, contiguous in memory, written coalesced: in
+    // binary32 as 16-byte words (93.5 against 96 us on the 1.07 M-cell wing), in binary64 value
+    // by value (16-byte words measured 1% slower there)
+    if constexpr (sizeof(S) == 4) {
+        constexpr int WG = kGL / 4;
+        using word = float __attribute__((ext_vector_type(4)));
+        word *dst = reinterpret_cast<word *>(a.gl + (size_t)c0 * kGL);
+        for (int k = t; k < WG * ncell; k += blockDim.x) {
+            const int cl = k / WG, r0 = (k - cl * WG) * 4, col = (int)s_start[cl];
+            word v;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = s_buf[r0 + e][col];
+            dst[k] = v;
+        }
+    } else {
+        for (int k = t; k < kGL * ncell; k += blockDim.x) {
+            const int cl = k / kGL, r = k - cl * kGL;
+            a.gl[(size_t)c0 * kGL + k] = s_buf[r][s_start[cl]];
+        }
     }
 }
 
