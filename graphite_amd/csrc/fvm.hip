@@ -607,13 +607,9 @@ struct gr_solver {
         a.frec = (const S *)slot_geo.p;
         a.grec = a.frec + flux_span<S>(slots);
         a.gnrm = a.grec + slots * gr::kGradRec;
-        if constexpr (sizeof(S) == 8) {
-            a.lsq = (const S *)mesh->mat.p;
-            a.lsq_stride = std::max<size_t>(64, mesh->mstride());
-        } else {
-            a.lsq = (const S *)lsq32.p;
-            a.lsq_stride = std::max<size_t>(64, mesh->mstride());
-        }
+        if constexpr (sizeof(S) == 8) a.lsq = (const S *)mesh->mat.p;
+        else a.lsq = (const S *)lsq32.p;
+        a.lsq_stride = std::max<size_t>(64, mesh->mstride());
         return a;
     }
     // Greedy cut of the interior cells into runs of at most kTileSlots face slots and max_cells
@@ -641,8 +637,9 @@ struct gr_solver {
         partials.alloc(ntiles + 64);
         GR_HIP(hipStreamSynchronize(m->stream));
     }
-    // Mean durations (ms) of n back-to-back launches of the gradient and the flux kernel on the
-    // current state (not advanced), HIP events on the library's stream around each run.
+    // Mean durations (ms) of n back-to-back launches of the gradient, the flux and the residual
+    // sum kernels on the current state (not advanced), HIP events on the library's stream around
+    // each run.
     void time_kernels(int n, double ms[3]) {
         gr_mesh *m = mesh;
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
