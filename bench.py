@@ -189,7 +189,7 @@ def warm_start_leg(gr, args):
         return np.array([gr.lift_drag(F[k], test[k], 1.4, ws.P_INF, s_ref) for k in range(64)])
     c3, w3 = coeffs(cold), coeffs(warm)
     log(f"bench: warm-start leg (converged lift/drag: {args.warm_settle} + {args.warm_window} further iterations)")
-    agree = ws.force_agreement(cold, warm, test, "airfoil", s_ref, args.warm_settle, args.warm_window, 500)
+    agree = ws.force_agreement(cold, warm, test, "airfoil", s_ref, args.warm_settle, args.warm_window, 500, log=log)
     rel = lambda a, b: (np.abs(a - b).max(axis=0) / np.abs(b[:, 0]).max()).tolist()
     it_w, it_c = wsol.iterations, cs.iterations
     return {

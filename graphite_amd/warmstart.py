@@ -128,7 +128,7 @@ def cold_and_warm(mesh: Mesh, net: Mlp, norm, machs, tol: float, max_iter: int, 
 
 
 def force_agreement(cold: Batch, warm: Batch, machs, region: str, s_ref: float, settle_iter: int,
-                    window_iter: int = 0, sample_every: int = 500) -> dict:
+                    window_iter: int = 0, sample_every: int = 500, log=None) -> dict:
     """Lift and drag of warm- and cold-started solves of the same flows once both have run the
     plain solver for settle_iter further iterations.  The reference's scheme ends in a limit
     cycle on the C-grid (squared residual 1e-6 to 1e-4, lift oscillating by about 0.3%
@@ -136,7 +136,8 @@ def force_agreement(cold: Batch, warm: Batch, machs, region: str, s_ref: float, 
     sampled every sample_every; with window_iter 0 the forces at the end are compared.  A flow
     that diverges restarts from freestream (the plain solver is the fallback) and is counted.
     Lift differences are relative to each flow's mean |C_L| (flows with |C_L| of at least
-    0.05), drag differences to each flow's mean C_D.  Returns a dict of the statistics."""
+    0.05), drag differences to each flow's mean C_D.  log(msg), if given, reports progress.
+    Returns a dict of the statistics."""
     from . import lift_drag
     n = len(machs)
 
@@ -157,6 +158,8 @@ def force_agreement(cold: Batch, warm: Batch, machs, region: str, s_ref: float, 
             done += step
             if samples is not None:
                 samples.append(coeffs(b))
+            if log and done % 20000 == 0:
+                log(f"force agreement: {tag} +{done}")
 
     for tag, b in (("cold", cold), ("warm", warm)):
         run(tag, b, settle_iter, 10000)
