@@ -557,7 +557,7 @@ struct gr_solver {
     gr_mesh *mesh = nullptr;
     gr_solver_config cfg{};
     size_t stride = 0;
-    gr::DevArray<char> ubuf[2], grad;   // grad: gradient + limiter records (kGL per cell)
+    gr::DevArray<char> ubuf[2], grad;   // grad: gradient + limiter records (kGL<S> per cell)
     gr::DevArray<char> slot_geo;        // flux, gradient and ghost geometry records
     gr::DevArray<float> lsq32;          // binary32 LSQ matrices (binary64 solvers use the mesh's)
     gr::DevArray<uint32_t> tiles;       // first cell of each tile, ntiles+1 entries
@@ -710,7 +710,7 @@ struct gr_solver {
     template <typename S> void launch_gradlim(int b, gr::SolverCtl *c) {
         gr_mesh *m = mesh;
         hipLaunchKernelGGL(gr::k_gradlim_tile<S>, dim3(ntiles), dim3(gr::kBlock), 0, m->stream, m->view(), sargs<S>(b),
-                           table(), tiles.p, ntiles, cfg.limiter, c);
+                           table(), tiles.p, ntiles, cfg.limiter, cfg.corrected, c);
     }
     template <typename S> void launch_flux(int b, gr::SolverCtl *c) {
         gr_mesh *m = mesh;
@@ -847,7 +847,7 @@ int gr_solver_create(gr_mesh *m, const gr_solver_config *cfg, gr_solver **out) {
                 b.alloc(gr::kUR * blk);
                 GR_HIP(hipMemsetAsync(b.p, 0, b.n, m->stream));
             }
-            s->grad.alloc((size_t)gr::kGL * blk);
+            s->grad.alloc((size_t)(cfg->dtype == GR_F64 ? gr::kGL<double> : gr::kGL<float>) * blk);
             GR_HIP(hipMemsetAsync(s->grad.p, 0, s->grad.n, m->stream));
             s->bcs.resize(m->h.n_regions);
             s->build_tiles(gr::kTileCells);

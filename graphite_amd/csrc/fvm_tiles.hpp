@@ -131,11 +131,21 @@ __device__ inline void load_vec(const S *__restrict__ p, S (&v)[W]) {
     }
 }
 
-// Gradient and limiter of a cell as one record: rows 3c+k of the gradient (component c,
-// direction k), then the five limiters.  A neighbour's record is 160 contiguous bytes in
-// binary64 (80 in binary32), fetched with 16-byte loads from two or three cache lines, where
-// twenty separate arrays would touch twenty lines for it.
-constexpr int kGL = 20;
+// Gradient and limiter of a cell as one record.  binary64: rows 3c+k of the gradient (component
+// c, direction k), then the five limiters -- the reference's operands, 160 contiguous bytes,
+// fetched with 16-byte loads from two or three cache lines where twenty separate arrays would
+// touch twenty lines.  binary32: the 15 rows, each multiplied by the limiter the reconstruction
+// applies to it, and one value of padding (64 bytes): (g.d) l becomes (g l).d, the same
+// quantity rounded differently, and a neighbour costs 64 bytes instead of 80.
+template <typename S>
+constexpr int kGL = sizeof(S) == 8 ? 20 : 16;
+
+// The limiter (component) that multiplies gradient row r in the reconstruction: rho and rho E
+// their own; momentum direction k of every component the limiter of component 1 + k (the
+// reference's tensor * vector), or the row's own component with the corrected form.
+__device__ inline int row_limiter(int r, int corr) {
+    return r < 3 ? 0 : r >= 12 ? 4 : corr ? r / 3 : 1 + r % 3;
+}
 
 // Per-slot geometry of the iteration, one record per face slot q of an interior cell i
 // (neighbour j = cell_connects[q], face f = cell_faces[q]), computed once from the binary64
@@ -255,7 +265,7 @@ template <typename S>
 struct SolverArgs {
     const S *u;       // current state records (cells; ghosts only for regions without a BC)
     S *un;            // next state records
-    S *gl;            // gradient + limiter records of the interior cells, kGL values each
+    S *gl;            // gradient + limiter records of the interior cells, kGL<S> values each
     const S *frec;    // flux records, kFluxRec per interior face slot
     const S *grec;    // gradient records, kGradRec per interior face slot
     const S *gnrm;    // ghost records, kGhostRec per ghost
@@ -263,15 +273,25 @@ struct SolverArgs {
     size_t lsq_stride;
 };
 
-// The record of cell i: gradient rows into g[15], limiters into l[5].
-template <typename S>
-__device__ inline void load_gl(const S *__restrict__ gl, uint32_t i, S g[15], S l[5]) {
-    S v[kGL];
-    load_vec<S, kGL>(gl + (size_t)i * kGL, v);
+// Reconstructed state at offset d of cell c from its state u and its gradient+limiter record.
+template <typename S, bool CORR>
+__device__ inline void recon_at(const S *__restrict__ gl, uint32_t c, const S u[5], const S d[3], S &r, S ru[3],
+                                S &re) {
+    S v[kGL<S>];
+    load_vec<S, kGL<S>>(gl + (size_t)c * kGL<S>, v);
+    if constexpr (sizeof(S) == 8) {
+        reconstruct<S, CORR>(u, v, v + 15, d, r, ru, re);
+    } else {   // rows premultiplied by their limiters
+        r = u[0] + ((v[0] * d[0] + v[1] * d[1]) + v[2] * d[2]);
 #pragma unroll
-    for (int k = 0; k < 15; ++k) g[k] = v[k];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) l[k] = v[15 + k];
+        for (int k = 0; k < 3; ++k) {
+            S gd;
+            if (CORR) gd = (v[3 * (1 + k)] * d[0] + v[3 * (1 + k) + 1] * d[1]) + v[3 * (1 + k) + 2] * d[2];
+            else gd = (v[3 + k] * d[0] + v[6 + k] * d[1]) + v[9 + k] * d[2];
+            ru[k] = u[1 + k] + gd;
+        }
+        re = u[4] + ((v[12] * d[0] + v[13] * d[1]) + v[14] * d[2]);
+    }
 }
 
 // Gradient (LSQ) + neighbour min/max + limiter of the five conservative components
@@ -279,7 +299,7 @@ __device__ inline void load_gl(const S *__restrict__ gl, uint32_t i, S g[15], S 
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<S> a, BcTable bc,
                                                          const uint32_t *__restrict__ tiles, uint32_t ntiles,
-                                                         int kind, const SolverCtl *ctl) {
+                                                         int kind, int corr, const SolverCtl *ctl) {
     if (ctl && ctl->stop) return;
     __shared__ uint32_t s_start[kTileCells + 1];
     __shared__ S s_buf[20][kTileSlots];
@@ -391,23 +411,26 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     }
     __syncthreads();
     // E: the tile's gradient + limiter records, contiguous in memory, written coalesced: in
-    // binary32 as 16-byte words (93.5 against 96 us on the 1.07 M-cell wing), in binary64 value
-    // by value (16-byte words measured 1% slower there)
+    // binary32 the limited rows as 16-byte words, in binary64 value by value (16-byte words
+    // measured 1% slower there)
     if constexpr (sizeof(S) == 4) {
-        constexpr int WG = kGL / 4;
+        constexpr int WG = kGL<S> / 4;
         using word = float __attribute__((ext_vector_type(4)));
-        word *dst = reinterpret_cast<word *>(a.gl + (size_t)c0 * kGL);
+        word *dst = reinterpret_cast<word *>(a.gl + (size_t)c0 * kGL<S>);
         for (int k = t; k < WG * ncell; k += blockDim.x) {
             const int cl = k / WG, r0 = (k - cl * WG) * 4, col = (int)s_start[cl];
             word v;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) v[e] = s_buf[r0 + e][col];
+            for (int e = 0; e < 4; ++e) {
+                const int r = r0 + e;
+                v[e] = r < 15 ? s_buf[r][col] * s_buf[15 + row_limiter(r, corr)][col] : S(0);
+            }
             dst[k] = v;
         }
     } else {
-        for (int k = t; k < kGL * ncell; k += blockDim.x) {
-            const int cl = k / kGL, r = k - cl * kGL;
-            a.gl[(size_t)c0 * kGL + k] = s_buf[r][s_start[cl]];
+        for (int k = t; k < kGL<S> * ncell; k += blockDim.x) {
+            const int cl = k / kGL<S>, r = k - cl * kGL<S>;
+            a.gl[(size_t)c0 * kGL<S> + k] = s_buf[r][s_start[cl]];
         }
     }
 }
@@ -444,16 +467,14 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
         const S ds = geo[3];
         const S d0[3] = {geo[4], geo[5], geo[6]};
         const S d1[3] = {geo[7], geo[8], geo[9]};
-        S ui[5], gi[15], li[5];
+        S ui[5];
         load_state<S>(a.u, i, ui);
-        load_gl<S>(a.gl, i, gi, li);
         S r0, ru0[3], e0, r1, ru1[3], e1;
-        reconstruct<S, CORR>(ui, gi, li, d0, r0, ru0, e0);
+        recon_at<S, CORR>(a.gl, i, ui, d0, r0, ru0, e0);
         if (j < m.N) {
-            S uj[5], gj[15], lj[5];
+            S uj[5];
             load_state<S>(a.u, j, uj);
-            load_gl<S>(a.gl, j, gj, lj);
-            reconstruct<S, CORR>(uj, gj, lj, d1, r1, ru1, e1);
+            recon_at<S, CORR>(a.gl, j, uj, d1, r1, ru1, e1);
         } else {   // ghost: its state from the BC, gradient and limiter of cell i
             S ug[5];
             const int r = bc_region(bc, j - m.N);
@@ -462,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
             } else {
                 ghost_state<S>(bc.r[r], n, ui, ug);
             }
-            reconstruct<S, CORR>(ug, gi, li, d1, r1, ru1, e1);
+            recon_at<S, CORR>(a.gl, i, ug, d1, r1, ru1, e1);   // cell i's gradient and limiter
         }
         const S p0 = S(0.4) * (e0 - fdiv(S(0.5), r0) * ((ru0[0] * ru0[0] + ru0[1] * ru0[1]) + ru0[2] * ru0[2]));
         const S p1 = S(0.4) * (e1 - fdiv(S(0.5), r1) * ((ru1[0] * ru1[0] + ru1[1] * ru1[1]) + ru1[2] * ru1[2]));
