@@ -709,13 +709,16 @@ class Mlp:
         return loss.value
 
 
-def warm_norm(x, t, origin=(0.0, 0.0, 0.0)) -> WarmNorm:
+def warm_norm(x, t, origin=(0.0, 0.0, 0.0), trust=(2.0, 4.0)) -> WarmNorm:
     """Input/output standardisation of a warm-start training set (mean, std per column); origin
-    is the point the rows' centre coordinates were compressed about."""
+    is the point the rows' centre coordinates were compressed about.  trust: wall distances
+    over which the prediction is blended out to the freestream (far-wake cells started from a
+    prediction made transonic C-grid flows diverge, those started from freestream do not)."""
     x, t = np.asarray(x, np.float64), np.asarray(t, np.float64)
     n = WarmNorm()
     for k in range(3):
         n.origin[k] = origin[k]
+    n.trust[0], n.trust[1] = trust
     xm, xs = x.mean(0), x.std(0)
     tm, ts = t.mean(0), t.std(0)
     for c in range(6):

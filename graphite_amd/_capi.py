@@ -57,7 +57,7 @@ class NsiConfig(C.Structure):
 
 class WarmNorm(C.Structure):
     _fields_ = [("origin", C.c_float * 3), ("in_mean", C.c_float * 6), ("in_std", C.c_float * 6),
-                ("out_mean", C.c_float * 5), ("out_std", C.c_float * 5)]
+                ("out_mean", C.c_float * 5), ("out_std", C.c_float * 5), ("trust", C.c_float * 2)]
 
 
 SIGNATURES = {

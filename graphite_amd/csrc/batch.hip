@@ -382,8 +382,8 @@ __global__ void k_warm_targets(const S *__restrict__ u, size_t stride, uint32_t 
 // Network outputs Y [rows][32] of rows row0.. -> conservative state in both state buffers.
 template <typename S>
 __global__ void k_warm_apply(const float *__restrict__ Y, size_t row0, int rows, uint32_t n_inst, gr_warm_norm nrm,
-                             const double *__restrict__ inst_mach, double p_inf, double T_inf, S *__restrict__ u0,
-                             S *__restrict__ u1, size_t stride) {
+                             const double *__restrict__ wd, const double *__restrict__ inst_mach, double p_inf,
+                             double T_inf, S *__restrict__ u0, S *__restrict__ u1, size_t stride) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= rows) return;
     const size_t r = row0 + q;
@@ -391,6 +391,15 @@ __global__ void k_warm_apply(const float *__restrict__ Y, size_t row0, int rows,
     double t[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) t[c] = (double)Y[(size_t)q * 32 + c] * nrm.out_std[c] + nrm.out_mean[c];
+    // trusted region: the deviation in full up to wall distance trust[0], blended out
+    // (smoothstep) by trust[1]
+    if (nrm.trust[1] > 0) {
+        const double d0 = nrm.trust[0], d1 = nrm.trust[1], d = wd[i];
+        const double s = d <= d0 ? 0.0 : d >= d1 ? 1.0 : (d - d0) / (d1 - d0);
+        const double wt = 1.0 - s * s * (3.0 - 2.0 * s);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) t[c] *= wt;
+    }
     const FarField ff = far_field(p_inf, T_inf);
     double vel[3], ke = 0;
 #pragma unroll
@@ -1024,6 +1033,7 @@ int gr_batch_warm_start(gr_batch *b, gr_mlp *n, const double *wall_distance, con
         require(gr::mlp_n_in(n) == 6 && gr::mlp_n_out(n) == 5, "the warm-start network must have n_in 6 and n_out 5");
         for (int c = 0; c < 6; ++c) require(norm->in_std[c] > 0, "input scales must be positive");
         for (int c = 0; c < 5; ++c) require(norm->out_std[c] > 0, "output scales must be positive");
+        require(norm->trust[1] <= 0 || norm->trust[1] > norm->trust[0], "trust radii must increase");
         require(max_ratio > 0, "max_ratio must be positive");
         require(!b->mach_h.empty(), "far field not set (gr_batch_set_freestream)");
         gr_mesh *m = b->mesh;
@@ -1046,11 +1056,11 @@ int gr_batch_warm_start(gr_batch *b, gr_mlp *n, const double *wall_distance, con
         const auto consume = [&](const float *Y, size_t row0, int rows) {
             if (b->cfg.dtype == GR_F64)
                 hipLaunchKernelGGL(gr::k_warm_apply<double>, dim3(gr::blocks_for(rows)), dim3(gr::kBlock), 0, m->stream,
-                                   Y, row0, rows, I, nrm, mach, b->p_inf, b->T_inf, (double *)b->ubuf[0].p,
+                                   Y, row0, rows, I, nrm, wd.p, mach, b->p_inf, b->T_inf, (double *)b->ubuf[0].p,
                                    (double *)b->ubuf[1].p, b->stride);
             else
                 hipLaunchKernelGGL(gr::k_warm_apply<float>, dim3(gr::blocks_for(rows)), dim3(gr::kBlock), 0, m->stream,
-                                   Y, row0, rows, I, nrm, mach, b->p_inf, b->T_inf, (float *)b->ubuf[0].p,
+                                   Y, row0, rows, I, nrm, wd.p, mach, b->p_inf, b->T_inf, (float *)b->ubuf[0].p,
                                    (float *)b->ubuf[1].p, b->stride);
         };
         gr::mlp_run_device(n, (size_t)N * I, m->stream, fill, consume);

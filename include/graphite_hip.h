@@ -451,6 +451,9 @@ typedef struct gr_warm_norm {
     float origin[3];                 /* origin of the compressed centre coordinates */
     float in_mean[6], in_std[6];     /* network input = (raw - mean) / std */
     float out_mean[5], out_std[5];   /* raw target = network output * std + mean */
+    float trust[2];                  /* the predicted deviation from freestream is applied in full
+                                        up to wall distance trust[0] and blended out by trust[1]
+                                        (trust[1] <= 0: everywhere) */
 } gr_warm_norm;
 
 /* All instances' states at once: rho/rhoe [n_inst][N+G], rhou [n_inst][N+G][3]. */

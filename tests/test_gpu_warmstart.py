@@ -81,6 +81,34 @@ def test_prediction_is_applied(gr, small):
         assert np.allclose(e[k, :N], E + RHO_INF * C_INF ** 2 * tt[4], rtol=1e-14)
 
 
+def test_prediction_blended_out_beyond_trust_radius(gr, small):
+    """With trust radii (d0, d1) the predicted deviation from freestream applies in full within
+    wall distance d0 and not at all beyond d1."""
+    from graphite_amd import warmstart as ws
+    m = small
+    M = machs(1, seed=7)
+    c = np.array([0.5, -0.25, 0.125, 0.0, 1.0], np.float32)
+    norm = gr.WarmNorm()
+    for j in range(6):
+        norm.in_mean[j], norm.in_std[j] = 0.0, 1.0
+    for j in range(5):
+        norm.out_mean[j], norm.out_std[j] = 0.0, 0.1
+    wd = ws.wall_distance(m)
+    d0, d1 = np.percentile(wd, 30), np.percentile(wd, 70)
+    norm.trust[0], norm.trust[1] = d0, d1
+    b = ws.make_batch(m, M)
+    b.warm_start(constant_net(gr, c), wd, norm, max_ratio=1e300)
+    r, u, e = b.get_states()
+    N = m.n_cells
+    full = RHO_INF * (1 + float(np.float32(c[0]) * np.float32(0.1)))
+    near, far = wd < np.float32(d0) * (1 - 1e-6), wd > np.float32(d1) * (1 + 1e-6)
+    assert near.any() and far.any()
+    assert np.allclose(r[0, :N][near], full, rtol=1e-14)
+    assert np.allclose(r[0, :N][far], RHO_INF, rtol=1e-14)
+    mid = ~(near | far)
+    assert np.all((r[0, :N][mid] >= RHO_INF * (1 - 1e-14)) & (r[0, :N][mid] <= full * (1 + 1e-14)))
+
+
 def test_bad_prediction_falls_back_to_freestream(gr, small):
     """A network that outputs NaN: the positivity guard puts every cell back at freestream, so
     the prediction's residual equals the freestream residual and a strict judge (max_ratio 1)
