@@ -189,7 +189,8 @@ def warm_start_leg(gr, args):
         return np.array([gr.lift_drag(F[k], test[k], 1.4, ws.P_INF, s_ref) for k in range(64)])
     c3, w3 = coeffs(cold), coeffs(warm)
     log(f"bench: warm-start leg (converged lift/drag: {args.warm_settle} + {args.warm_window} further iterations)")
-    agree = ws.force_agreement(cold, warm, test, "airfoil", s_ref, args.warm_settle, args.warm_window, 500, log=log)
+    agree = ws.force_agreement(cold, warm, test, "airfoil", s_ref, args.warm_settle, args.warm_window, args.warm_sample,
+                               log=log)
     rel = lambda a, b: (np.abs(a - b).max(axis=0) / np.abs(b[:, 0]).max()).tolist()
     it_w, it_c = wsol.iterations, cs.iterations
     return {
@@ -311,7 +312,8 @@ def main():
     ap.add_argument("--warm-steps", type=int, default=8000)
     ap.add_argument("--warm-settle", type=int, default=100000,
                     help="further iterations of both warm and cold flows before their forces are compared")
-    ap.add_argument("--warm-window", type=int, default=20000, help="iterations the compared forces are averaged over")
+    ap.add_argument("--warm-window", type=int, default=40000, help="iterations the compared forces are averaged over")
+    ap.add_argument("--warm-sample", type=int, default=100, help="iterations between force samples in the window")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
