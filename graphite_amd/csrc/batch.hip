@@ -68,7 +68,7 @@ __device__ inline BcRegion inst_region(const BcRegion &R, const double *inst_mac
 
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<S> a, BcTable bc,
-                                                          const double *__restrict__ inst_mach, int kind,
+                                                          const double *__restrict__ inst_mach, int kind, int corr,
                                                           const BatchCtl *ctl) {
     if (ctl->stop) return;
     const uint32_t i = batch_cell();
@@ -110,10 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<
         S out[3];
         lsq_apply<S>(m, i, &g[3 * c], out);
 #pragma unroll
-        for (int kk = 0; kk < 3; ++kk) {
-            g[3 * c + kk] = out[kk];
-            store_nt(&a.g[(3 * c + kk) * st + me], out[kk]);
-        }
+        for (int kk = 0; kk < 3; ++kk) g[3 * c + kk] = out[kk];
     }
     S lim[5] = {S(2), S(2), S(2), S(2), S(2)};
     for (uint32_t q = s; q < e; ++q) {
@@ -124,8 +121,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<
             lim[c] = fmin(lim[c], limiter_face<S>(ui[c], g[3 * c], g[3 * c + 1], g[3 * c + 2], umin[c], umax[c], dx,
                                                   dy, dz, kind));
     }
+    // binary64: the gradient and the limiters; binary32: the gradient rows premultiplied by the
+    // limiter the reconstruction applies to them (15 rows instead of 20, as the tile kernels)
+    if constexpr (sizeof(S) == 8) {
 #pragma unroll
-    for (int c = 0; c < 5; ++c) store_nt(&a.l[c * st + me], lim[c]);
+        for (int q = 0; q < 15; ++q) store_nt(&a.g[q * st + me], g[q]);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) store_nt(&a.l[c * st + me], lim[c]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 15; ++q) store_nt(&a.g[q * st + me], g[q] * lim[row_limiter(q, corr)]);
+    }
 }
 
 // At least 3 waves per SIMD: left to itself the compiler gives the binary64 kernel 174 VGPRs
@@ -142,11 +148,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
     double e_loc = 0;
     if (i < m.N) {
         const size_t st = a.stride, me = (size_t)i * kBatchLanes + k;
-        S ui[5], gi[15], li[5];
+        S ui[5], gi[15], li[5] = {S(0), S(0), S(0), S(0), S(0)};
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             ui[c] = a.u[c * st + me];
-            li[c] = a.l[c * st + me];
+            if constexpr (sizeof(S) == 8) li[c] = a.l[c * st + me];
         }
         if (ctl->done[k]) {   // frozen instance: carry its state over
 #pragma unroll
@@ -167,18 +173,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                 const S d0[3] = {(S)(fcx - xi), (S)(fcy - yi), (S)(fcz - zi)};
                 const S d1[3] = {(S)(fcx - m.cx[j]), (S)(fcy - m.cy[j]), (S)(fcz - m.cz[j])};
                 S r0, ru0[3], e0, r1, ru1[3], e1;
-                reconstruct<S, CORR>(ui, gi, li, d0, r0, ru0, e0);
+                if constexpr (sizeof(S) == 8) reconstruct<S, CORR>(ui, gi, li, d0, r0, ru0, e0);
+                else reconstruct_pre<S, CORR>(ui, gi, d0, r0, ru0, e0);
                 if (j < m.N) {
                     const size_t nj = (size_t)j * kBatchLanes + k;
                     S uj[5], gj[15], lj[5];
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
                         uj[c] = a.u[c * st + nj];
-                        lj[c] = a.l[c * st + nj];
+                        if constexpr (sizeof(S) == 8) lj[c] = a.l[c * st + nj];
                     }
 #pragma unroll
                     for (int q2 = 0; q2 < 15; ++q2) gj[q2] = a.g[q2 * st + nj];
-                    reconstruct<S, CORR>(uj, gj, lj, d1, r1, ru1, e1);
+                    if constexpr (sizeof(S) == 8) reconstruct<S, CORR>(uj, gj, lj, d1, r1, ru1, e1);
+                    else reconstruct_pre<S, CORR>(uj, gj, d1, r1, ru1, e1);
                 } else {
                     S ug[5];
                     const int r = bc_region(bc, j - m.N);
@@ -188,7 +196,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                     } else {
                         ghost_state<S>(inst_region(bc.r[r], inst_mach, k), n, ui, ug);
                     }
-                    reconstruct<S, CORR>(ug, gi, li, d1, r1, ru1, e1);
+                    if constexpr (sizeof(S) == 8) reconstruct<S, CORR>(ug, gi, li, d1, r1, ru1, e1);
+                    else reconstruct_pre<S, CORR>(ug, gi, d1, r1, ru1, e1);
                 }
                 const S p0 = S(0.4) * (e0 - fdiv(S(0.5), r0) * ((ru0[0] * ru0[0] + ru0[1] * ru0[1]) + ru0[2] * ru0[2]));
                 const S p1 = S(0.4) * (e1 - fdiv(S(0.5), r1) * ((ru1[0] * ru1[0] + ru1[1] * ru1[1]) + ru1[2] * ru1[2]));
@@ -512,7 +521,7 @@ struct gr_batch {
         const dim3 grid(nblocks()), blk(gr::kBlock);
         if (stage == 0) {
             hipLaunchKernelGGL(gr::k_batch_gradlim<S>, grid, blk, 0, m->stream, m->view(), args<S>(b), table(),
-                               inst_mach.p, cfg.limiter, (const gr::BatchCtl *)c);
+                               inst_mach.p, cfg.limiter, cfg.corrected, (const gr::BatchCtl *)c);
         } else if (stage == 1) {
             if (cfg.corrected)
                 hipLaunchKernelGGL((gr::k_batch_flux<S, true>), grid, blk, 0, m->stream, m->view(), args<S>(b),
@@ -681,19 +690,8 @@ struct gr_batch {
         std::memset(&c0, 0, sizeof c0);
         for (int k = 0; k < gr::kBatchLanes; ++k) c0.done[k] = k < (int)n_inst ? 0 : 1;
         GR_HIP(hipMemcpyAsync(ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
-        const gr::BcTable t = table();
-        const dim3 grid(nblocks()), blk(gr::kBlock);
-        if (cfg.dtype == GR_F64) {
-            hipLaunchKernelGGL(gr::k_batch_gradlim<double>, grid, blk, 0, m->stream, m->view(), args<double>(cur), t,
-                               (const double *)inst_mach.p, cfg.limiter, (const gr::BatchCtl *)ctl.p);
-            hipLaunchKernelGGL((gr::k_batch_flux<double, false>), grid, blk, 0, m->stream, m->view(), args<double>(cur),
-                               t, (const double *)inst_mach.p, cfg.cfl, partials.p, (const gr::BatchCtl *)ctl.p);
-        } else {
-            hipLaunchKernelGGL(gr::k_batch_gradlim<float>, grid, blk, 0, m->stream, m->view(), args<float>(cur), t,
-                               (const double *)inst_mach.p, cfg.limiter, (const gr::BatchCtl *)ctl.p);
-            hipLaunchKernelGGL((gr::k_batch_flux<float, false>), grid, blk, 0, m->stream, m->view(), args<float>(cur),
-                               t, (const double *)inst_mach.p, cfg.cfl, partials.p, (const gr::BatchCtl *)ctl.p);
-        }
+        // the iteration's gradient and flux launches (stages 0 and 1) on the current state
+        for (int stage = 0; stage < 2; ++stage) enqueue_stage(stage, cur, ctl.p);
         hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream, partials.p,
                            nblocks(), (const gr::BatchCtl *)nullptr, partials2.p);
         hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,

@@ -273,25 +273,29 @@ struct SolverArgs {
     size_t lsq_stride;
 };
 
+// The reconstruction u + (g.d) l with the gradient rows premultiplied by their limiters
+// (G = g l, row_limiter): u + G.d.
+template <typename S, bool CORR>
+__device__ inline void reconstruct_pre(const S u[5], const S G[15], const S d[3], S &r, S ru[3], S &re) {
+    r = u[0] + ((G[0] * d[0] + G[1] * d[1]) + G[2] * d[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        S gd;
+        if (CORR) gd = (G[3 * (1 + k)] * d[0] + G[3 * (1 + k) + 1] * d[1]) + G[3 * (1 + k) + 2] * d[2];
+        else gd = (G[3 + k] * d[0] + G[6 + k] * d[1]) + G[9 + k] * d[2];
+        ru[k] = u[1 + k] + gd;
+    }
+    re = u[4] + ((G[12] * d[0] + G[13] * d[1]) + G[14] * d[2]);
+}
+
 // Reconstructed state at offset d of cell c from its state u and its gradient+limiter record.
 template <typename S, bool CORR>
 __device__ inline void recon_at(const S *__restrict__ gl, uint32_t c, const S u[5], const S d[3], S &r, S ru[3],
                                 S &re) {
     S v[kGL<S>];
     load_vec<S, kGL<S>>(gl + (size_t)c * kGL<S>, v);
-    if constexpr (sizeof(S) == 8) {
-        reconstruct<S, CORR>(u, v, v + 15, d, r, ru, re);
-    } else {   // rows premultiplied by their limiters
-        r = u[0] + ((v[0] * d[0] + v[1] * d[1]) + v[2] * d[2]);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            S gd;
-            if (CORR) gd = (v[3 * (1 + k)] * d[0] + v[3 * (1 + k) + 1] * d[1]) + v[3 * (1 + k) + 2] * d[2];
-            else gd = (v[3 + k] * d[0] + v[6 + k] * d[1]) + v[9 + k] * d[2];
-            ru[k] = u[1 + k] + gd;
-        }
-        re = u[4] + ((v[12] * d[0] + v[13] * d[1]) + v[14] * d[2]);
-    }
+    if constexpr (sizeof(S) == 8) reconstruct<S, CORR>(u, v, v + 15, d, r, ru, re);
+    else reconstruct_pre<S, CORR>(u, v, d, r, ru, re);   // rows premultiplied by their limiters
 }
 
 // Gradient (LSQ) + neighbour min/max + limiter of the five conservative components

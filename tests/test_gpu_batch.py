@@ -127,3 +127,19 @@ def test_batch_cgrid_64_instances(gr):
             assert np.array_equal(x, y), k
     r = b.residual_norms()
     assert np.all(np.isfinite(r)) and r.shape == (64,)
+
+
+def test_batch_binary32_equals_single_binary32(gr, box):
+    """The batched kernels and the single-flow tile kernels evaluate the same binary32
+    arithmetic (gradient rows premultiplied by their limiters in both): bitwise equal states."""
+    m = box
+    M = machs(3, seed=11)
+    b = gr.Batch(m, 3, gr.GR_F32, cfl=0.5)
+    for name, kind in BOX_BCS:
+        b.set_bc(name, kind)
+    b.set_freestream(M, P_INF, T_INF)
+    b.iterate(12)
+    for k in range(3):
+        sv, res = single(gr, m, M[k], 12, gr.GR_F32)
+        for x, y in zip(b.get_state(k), sv.get_state()):
+            assert np.array_equal(x[:m.n_cells], y[:m.n_cells]), k
