@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -63,3 +64,28 @@ def test_shim_header_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", "-"], input=src, text=True,
                        capture_output=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_the_header(gr, tmp_path):
+    """The ctypes mirrors of the header's structs have the C layout (size and field offsets)."""
+    from graphite_amd import _capi
+    header = os.path.join(ROOT, "include", "graphite_hip.h")
+    src = f"""#include <stddef.h>
+#include <stdio.h>
+#include "{header}"
+int main(void) {{
+    printf("%zu %zu %zu\\n", sizeof(gr_warm_norm), offsetof(gr_warm_norm, out_std), offsetof(gr_warm_norm, trust));
+    printf("%zu\\n", sizeof(gr_wing_params));
+    return 0;
+}}
+"""
+    c = tmp_path / "layout.c"
+    c.write_text(src)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", str(c), "-o", str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    W = _capi.WarmNorm
+    assert [int(v) for v in out[:3]] == [ctypes.sizeof(W), W.out_std.offset, W.trust.offset]
+    assert int(out[3]) == ctypes.sizeof(_capi.WingParams)
+    n = gr.warm_norm(np.zeros((4, 6)), np.zeros((4, 5)))
+    assert list(n.trust) == [2.0, 4.0]
