@@ -560,7 +560,7 @@ struct gr_solver {
     gr::DevArray<char> ubuf[2], grad;   // grad: gradient + limiter records (kGL<S> per cell)
     gr::DevArray<char> slot_geo;        // flux, gradient and ghost geometry records
     gr::DevArray<float> lsq32;          // binary32 LSQ matrices (binary64 solvers use the mesh's)
-    gr::DevArray<uint32_t> tiles;       // first cell of each tile, ntiles+1 entries
+    gr::DevArray<uint32_t> tiles;       // tile records, kTileRec words each
     uint32_t ntiles = 0, tile_cells = 0;
     gr::DevArray<double> partials;      // one V*|R|^2 partial per tile
     int cur = 0;         // buffer holding the current interior state
@@ -628,12 +628,22 @@ struct gr_solver {
             }
         }
         if (t.back() != m->h.n_cells) t.push_back(m->h.n_cells);
+        std::vector<uint32_t> rec((t.size() - 1) * gr::kTileRec, 0u);
+        for (size_t k = 0; k + 1 < t.size(); ++k) {
+            uint32_t *r = &rec[k * gr::kTileRec];
+            const uint32_t c0 = t[k], c1 = t[k + 1], q0 = st[c0];
+            r[0] = c0;
+            r[1] = q0;
+            r[2] = (c1 - c0) | (st[c1] - q0) << 16;
+            uint8_t *b = reinterpret_cast<uint8_t *>(r + 3);
+            for (uint32_t c = c0; c <= c1; ++c) b[c - c0] = (uint8_t)(st[c] - q0);
+        }
         GR_HIP(hipStreamSynchronize(m->stream));   // no launch may still read the old tiles
         drop_graphs();
         ntiles = (uint32_t)t.size() - 1;
         tile_cells = max_cells;
-        tiles.alloc(t.size());
-        tiles.upload(t.data(), t.size(), m->stream);
+        tiles.alloc(std::max<size_t>(1, rec.size()));
+        tiles.upload(rec.data(), rec.size(), m->stream);
         partials.alloc(ntiles + 64);
         GR_HIP(hipStreamSynchronize(m->stream));
     }

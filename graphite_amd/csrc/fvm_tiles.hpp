@@ -24,6 +24,12 @@ constexpr int kTileSlots = 248;   // 20 binary64 rows x 248 slots = 39.7 KB LDS:
 constexpr int kTileCells = 51;    // one thread per (component, cell) in the per-cell phases
 static_assert(5 * kTileCells <= kBlock, "per-cell phases need 5 threads per cell");
 
+// A tile as one 64-byte record (16 words): its first cell c0, its first face slot q0, ncell |
+// nslot << 16, then the ncell+1 slot offsets of its cells from q0 as bytes (at most 52, each at
+// most kTileSlots).  A block reads everything it needs to start from this one record.
+constexpr int kTileRec = 16;
+static_assert(kTileCells + 1 <= 4 * (kTileRec - 3) && kTileSlots < 256, "tile record layout");
+
 // Bijection workgroup id -> tile such that XCD x (= w mod 8) gets a contiguous tile range.
 __device__ inline uint32_t xcd_tile(uint32_t w, uint32_t T) {
     const uint32_t x = w & 7u, k = w >> 3;
@@ -307,11 +313,10 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     if (ctl && ctl->stop) return;
     __shared__ uint32_t s_start[kTileCells + 1];
     __shared__ S s_buf[20][kTileSlots];
-    const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
-    const uint32_t c0 = tiles[tile], c1 = tiles[tile + 1];
-    const int ncell = (int)(c1 - c0);
-    const uint32_t q0 = m.cfs[c0];
-    const int nslot = (int)(m.cfs[c1] - q0);
+    const uint32_t *tr = tiles + (size_t)xcd_tile(blockIdx.x, ntiles) * kTileRec;
+    const uint32_t c0 = tr[0], q0 = tr[1];
+    const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
+    const uint8_t *starts = reinterpret_cast<const uint8_t *>(tr + 3);
     const int t = threadIdx.x;
     // loads of the per-(component, cell) phase B (the cell's LSQ matrix and value), issued first
     // so that their latency hides under the slot gathers of phase A
@@ -326,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         for (int k = 0; k < 6; ++k) bmat[k] = a.lsq[k * a.lsq_stride + ic];
         bu = a.u[(size_t)ic * kUR + bc_c];
     }
-    for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = m.cfs[c0 + k] - q0;
+    for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = starts[k];
     __syncthreads();
 
     // A: per-slot gradient terms d*(u_j-u_i)/|d|^2 (rows 0..14) and u_j (rows 15..19)
@@ -442,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
 // Flux + local-time-step update (fvmEulerSteadyOp + euler_steady_step,
 // src/models/euler.cu:236-381), Jacobi, with V*|R|^2 block partials.
 template <typename S, bool CORR>
-__global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 4 : 8, 8))) void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
                                                       const uint32_t *__restrict__ tiles, uint32_t ntiles, double cfl,
                                                       double *partials, const SolverCtl *ctl) {
     if (ctl && ctl->stop) return;
@@ -450,15 +455,14 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView m, SolverArgs<S> 
     __shared__ S s_c[5][kTileSlots];   // central flux term ((F0+F1)*ds)*0.5
     __shared__ S s_d[5][kTileSlots];   // dissipation term ((eig*(U1-U0))*0.5)*ds
     __shared__ S s_e[kTileSlots];      // eig*ds
-    const uint32_t tile = xcd_tile(blockIdx.x, ntiles);
-    const uint32_t c0 = tiles[tile], c1 = tiles[tile + 1];
-    const int ncell = (int)(c1 - c0);
-    const uint32_t q0 = m.cfs[c0];
-    const int nslot = (int)(m.cfs[c1] - q0);
+    const uint32_t *tr = tiles + (size_t)xcd_tile(blockIdx.x, ntiles) * kTileRec;
+    const uint32_t c0 = tr[0], q0 = tr[1];
+    const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
+    const uint8_t *starts = reinterpret_cast<const uint8_t *>(tr + 3);
     const int t = threadIdx.x;
     const bool in_b = t < 5 * ncell;   // one thread per (component, cell) in the update
     const int cc_ = in_b ? t / ncell : 0, cl = in_b ? t - cc_ * ncell : 0;
-    for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = m.cfs[c0 + k] - q0;
+    for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = starts[k];
     __syncthreads();
 
     if (t < nslot) {
