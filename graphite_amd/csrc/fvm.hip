@@ -622,6 +622,8 @@ struct gr_solver {
         uint32_t first = 0;
         for (uint32_t c = 0; c < m->h.n_cells; ++c) {
             require(st[c + 1] - st[c] <= (uint32_t)gr::kTileSlots, "cell with more face slots than a tile holds");
+            // the gradient kernel keeps a cell's results in its first three slot columns
+            require(st[c + 1] - st[c] >= 3, "cell with fewer than three faces");
             if (st[c + 1] - st[first] > (uint32_t)gr::kTileSlots || c + 1 - first > max_cells) {
                 t.push_back(c);
                 first = c;

@@ -311,8 +311,23 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
                                                          const uint32_t *__restrict__ tiles, uint32_t ntiles,
                                                          int kind, int corr, const SolverCtl *ctl) {
     if (ctl && ctl->stop) return;
+    // LDS rows per slot column.  Binary32 (kTerms): 0..14 the slot's gradient terms
+    // d*(u_j-u_i)/|d|^2, component c direction k in row 3c+k, and 15..19 u_j.  Binary64: 0..4
+    // u_j - u_i, 5..7 d = x_j - x_i, 8 1/|d|^2, 9..13 u_j -- 27.8 KB instead of 39.7 KB, five
+    // blocks per CU instead of four; phase B forms the terms itself.  After phase B the cell's
+    // first slot columns sb.. hold its gradient (grad_at), minimum and maximum (min_at, max_at);
+    // after phase D its limiters sit in row kUj+c of column sb.
+    constexpr bool kTerms = sizeof(S) == 4;
+    constexpr int kUj = kTerms ? 15 : 9;
     __shared__ uint32_t s_start[kTileCells + 1];
-    __shared__ S s_buf[20][kTileSlots];
+    __shared__ S s_buf[kUj + 5][kTileSlots];
+    auto grad_at = [&](int c, int k, int sb) -> S & { return kTerms ? s_buf[3 * c + k][sb] : s_buf[c][sb + k]; };
+    auto min_at = [&](int c, int sb) -> S & {
+        return kTerms ? s_buf[c][sb + 1] : s_buf[5 + (2 * c) % 4][sb + (2 * c) / 4];
+    };
+    auto max_at = [&](int c, int sb) -> S & {
+        return kTerms ? s_buf[5 + c][sb + 1] : s_buf[5 + (2 * c + 1) % 4][sb + (2 * c + 1) / 4];
+    };
     const uint32_t *tr = tiles + (size_t)xcd_tile(blockIdx.x, ntiles) * kTileRec;
     const uint32_t c0 = tr[0], q0 = tr[1];
     const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
@@ -334,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = starts[k];
     __syncthreads();
 
-    // A: per-slot gradient terms d*(u_j-u_i)/|d|^2 (rows 0..14) and u_j (rows 15..19)
+    // A: per slot u_j and either its gradient terms or u_j - u_i with the slot's offset and weight
     int lc = 0;
     S ui[5], d0[3] = {S(0), S(0), S(0)};
     if (t < nslot) {
@@ -359,17 +374,24 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const S du = uj[c] - ui[c];
+            if constexpr (kTerms) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) s_buf[3 * c + k][t] = (dw[k] * du) * dw[3];
-            s_buf[15 + c][t] = uj[c];
+                for (int k = 0; k < 3; ++k) s_buf[3 * c + k][t] = (dw[k] * du) * dw[3];
+            } else {
+                s_buf[c][t] = du;
+            }
+            s_buf[kUj + c][t] = uj[c];
+        }
+        if constexpr (!kTerms) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s_buf[5 + k][t] = dw[k];
         }
     }
     __syncthreads();
 
-    // B: one thread per (component, cell): sum in slot order, LSQ apply, min/max.  The
-    // gradient goes to the cell's first slot column (rows 3c..3c+2), min/max to its second
-    // (rows c and 5+c) -- rows other threads of the cell still read, hence the barrier between
-    // the reads and the writes.
+    // B: one thread per (component, cell): the terms d*(u_j-u_i)/|d|^2 summed in slot order, LSQ
+    // apply, min/max.  The results go to rows other threads of the cell still read, hence the
+    // barrier between the reads and the writes.
     S bg[3] = {S(0), S(0), S(0)}, bmin = S(0), bmax = S(0);
     int bsb = 0;
     if (in_b) {
@@ -378,10 +400,16 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         S g3[3] = {S(0), S(0), S(0)};
         bmin = bmax = bu;
         for (int s = bsb; s < se; ++s) {
+            if constexpr (kTerms) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) g3[k] += s_buf[3 * bc_c + k][s];
-            bmin = fmin(bmin, s_buf[15 + bc_c][s]);
-            bmax = fmax(bmax, s_buf[15 + bc_c][s]);
+                for (int k = 0; k < 3; ++k) g3[k] += s_buf[3 * bc_c + k][s];
+            } else {
+                const S du = s_buf[bc_c][s], w2 = s_buf[8][s];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) g3[k] += (s_buf[5 + k][s] * du) * w2;
+            }
+            bmin = fmin(bmin, s_buf[kUj + bc_c][s]);
+            bmax = fmax(bmax, s_buf[kUj + bc_c][s]);
         }
         // M * g with M symmetric (xx,yy,zz,xy,xz,yz), as lsq_apply
         bg[0] = (bmat[0] * g3[0] + bmat[3] * g3[1]) + bmat[4] * g3[2];
@@ -391,32 +419,32 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     __syncthreads();
     if (in_b) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) s_buf[3 * bc_c + k][bsb] = bg[k];
-        s_buf[bc_c][bsb + 1] = bmin;
-        s_buf[5 + bc_c][bsb + 1] = bmax;
+        for (int k = 0; k < 3; ++k) grad_at(bc_c, k, bsb) = bg[k];
+        min_at(bc_c, bsb) = bmin;
+        max_at(bc_c, bsb) = bmax;
     }
     __syncthreads();
 
-    // C: per-slot limiter of each component at this slot's face centre.  Reads rows 0..14 of
-    // the cell's first two slot columns, writes rows 15..19 of its own column (u_j, consumed
-    // in B): disjoint, so no barrier is needed between the reads and the writes.
+    // C: per-slot limiter of each component at this slot's face centre.  Reads rows below kUj
+    // of the cell's first slot columns, writes rows kUj.. of its own column (u_j, consumed in
+    // B): disjoint, so no barrier is needed between the reads and the writes.
     if (t < nslot) {
         const int sb = (int)s_start[lc];
 #pragma unroll
         for (int c = 0; c < 5; ++c)
-            s_buf[15 + c][t] = limiter_face<S>(ui[c], s_buf[3 * c][sb], s_buf[3 * c + 1][sb], s_buf[3 * c + 2][sb],
-                                               s_buf[c][sb + 1], s_buf[5 + c][sb + 1], d0[0], d0[1], d0[2], kind);
+            s_buf[kUj + c][t] = limiter_face<S>(ui[c], grad_at(c, 0, sb), grad_at(c, 1, sb), grad_at(c, 2, sb),
+                                                min_at(c, sb), max_at(c, sb), d0[0], d0[1], d0[2], kind);
     }
     __syncthreads();
 
     // D: one thread per (component, cell): minimum over the cell's faces, starting from 2.  Row
-    // 15+c of the cell's slots is read by this thread alone, so the limiter can go straight to
-    // the cell's first slot column, under its gradient rows.
+    // kUj+c of the cell's slots is read by this thread alone, so the limiter can go straight to
+    // the cell's first slot column.
     if (in_b) {
         const int se = (int)s_start[bc_l + 1];
         S l = S(2);
-        for (int s = bsb; s < se; ++s) l = fmin(l, s_buf[15 + bc_c][s]);
-        s_buf[15 + bc_c][bsb] = l;
+        for (int s = bsb; s < se; ++s) l = fmin(l, s_buf[kUj + bc_c][s]);
+        s_buf[kUj + bc_c][bsb] = l;
     }
     __syncthreads();
     // E: the tile's gradient + limiter records, contiguous in memory, written coalesced: in
@@ -431,15 +459,15 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
             word v;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int r = r0 + e;
-                v[e] = r < 15 ? s_buf[r][col] * s_buf[15 + row_limiter(r, corr)][col] : S(0);
+                const int r = r0 + e;   // gradient row r: component r/3, direction r%3
+                v[e] = r < 15 ? grad_at(r / 3, r % 3, col) * s_buf[kUj + row_limiter(r, corr)][col] : S(0);
             }
             dst[k] = v;
         }
     } else {
         for (int k = t; k < kGL<S> * ncell; k += blockDim.x) {
-            const int cl = k / kGL<S>, r = k - cl * kGL<S>;
-            a.gl[(size_t)c0 * kGL<S> + k] = s_buf[r][s_start[cl]];
+            const int cl = k / kGL<S>, r = k - cl * kGL<S>, col = (int)s_start[cl];
+            a.gl[(size_t)c0 * kGL<S> + k] = r < 15 ? grad_at(r / 3, r % 3, col) : s_buf[kUj + (r - 15)][col];
         }
     }
 }
