@@ -84,9 +84,11 @@ __device__ inline S limiter_face(S u, S gx, S gy, S gz, S umin, S umax, S dx, S 
     } else {
         sig = S(0);
     }
+    // the reference's two branches (delta > tol: d_u / delta, delta < -tol: d_d / delta) as one
+    // division of the selected numerator: same values, and a wave whose lanes take both
+    // branches divides and evaluates the limiter function once instead of twice
     S lim = S(1);
-    if (delta > Tol<S>::v) lim = limiter_func<S>(fdiv(d_u, delta), kind);
-    else if (delta < -Tol<S>::v) lim = limiter_func<S>(fdiv(d_d, delta), kind);
+    if (delta > Tol<S>::v || delta < -Tol<S>::v) lim = limiter_func<S>(fdiv(delta > Tol<S>::v ? d_u : d_d, delta), kind);
     lim *= S(0.8);
     return sig + (S(1) - sig) * lim;
 }
