@@ -149,8 +149,14 @@ constexpr int kGL = sizeof(S) == 8 ? 20 : 16;
 // The limiter (component) that multiplies gradient row r in the reconstruction: rho and rho E
 // their own; momentum direction k of every component the limiter of component 1 + k (the
 // reference's tensor * vector), or the row's own component with the corrected form.
-__device__ inline int row_limiter(int r, int corr) {
+__host__ __device__ constexpr int row_limiter(int r, int corr) {
     return r < 3 ? 0 : r >= 12 ? 4 : corr ? r / 3 : 1 + r % 3;
+}
+// row_limiter(r, corr) of rows 0..14 packed 3 bits per row (bits 3r..3r+2), for a lookup by shift.
+__host__ __device__ constexpr uint64_t row_limiters_packed(int corr) {
+    uint64_t v = 0;
+    for (int r = 0; r < 15; ++r) v |= (uint64_t)row_limiter(r, corr) << (3 * r);
+    return v;
 }
 
 // Per-slot geometry of the iteration, one record per face slot q of an interior cell i
@@ -454,13 +460,15 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         constexpr int WG = kGL<S> / 4;
         using word = float __attribute__((ext_vector_type(4)));
         word *dst = reinterpret_cast<word *>(a.gl + (size_t)c0 * kGL<S>);
+        const uint64_t lrows = corr ? row_limiters_packed(1) : row_limiters_packed(0);
         for (int k = t; k < WG * ncell; k += blockDim.x) {
             const int cl = k / WG, r0 = (k - cl * WG) * 4, col = (int)s_start[cl];
             word v;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int r = r0 + e;   // gradient row r: component r/3, direction r%3
-                v[e] = r < 15 ? grad_at(r / 3, r % 3, col) * s_buf[kUj + row_limiter(r, corr)][col] : S(0);
+                const int r = r0 + e;   // gradient row r (binary32 layout: LDS row r), padding at 15
+                const int lr = (int)((lrows >> (3 * r)) & 7u);
+                v[e] = r < 15 ? s_buf[r][col] * s_buf[kUj + lr][col] : S(0);
             }
             dst[k] = v;
         }
