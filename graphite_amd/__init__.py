@@ -20,6 +20,8 @@ from ._capi import (GR_OP_BOUND_SCAL_GRAD_C, GR_OP_BOUND_VEC_GRAD_C, GR_OP_CONV_
 from ._capi import (GR_BC_FARFIELD, GR_BC_SLIP, GR_F32, GR_F64, GR_HEX, GR_LIM_BARTH_JESPERSEN,
                     GR_LIM_MICHALAK, GR_LIM_VENKATAKRISHNAN, GR_OK, GR_PYRAMID, GR_QUAD, GR_TET,
                     GR_TRI, GR_WEDGE, ElementsInfo, MeshSizes, SolveResult, SolverConfig, WarmNorm, WingParams, NsiConfig, load)
+from ._capi import (GR_ERR_DEVICE, GR_ERR_DIVERGED, GR_ERR_INVALID, GR_ERR_IO, GR_ERR_NOT_FOUND,  # noqa: F401
+                    GR_ERR_TOPOLOGY)
 
 __all__ = ["GraphiteError", "Elements", "Mesh", "Field", "Solver", "device_count", "load",
            "euler_init", "euler_bc_slip", "euler_bc_farfield", "euler_step_prepare", "euler_step",

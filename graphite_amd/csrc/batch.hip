@@ -642,6 +642,10 @@ struct gr_batch {
             if (launched) *launched = 0;
             return;
         }
+        if (cfg.use_graph) {
+            graph(0);   // both parities captured before the first launch
+            graph(1);
+        }
         GR_HIP(hipMemcpyAsync(ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
         const int K = graph_iters;
         const int launches = (max_iter + K - 1) / K;
@@ -938,6 +942,11 @@ int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]) {
             }
         } guard{ev};
         for (auto &e : ev) GR_HIP(hipEventCreate(&e));
+        // the flux launches write buffer 1-cur, which holds the state the ghost values of a
+        // download are built from: saved here and restored after the runs
+        gr::DevArray<char> saved;
+        saved.alloc(b->ubuf[1 - b->cur].n);
+        GR_HIP(hipMemcpyAsync(saved.p, b->ubuf[1 - b->cur].p, saved.n, hipMemcpyDeviceToDevice, m->stream));
         gr::BatchCtl &c0 = b->ctl_host[0];
         std::memset(&c0, 0, sizeof c0);
         c0.max_iter = n_iter + 1;
@@ -958,6 +967,8 @@ int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]) {
             ms[j] = t / n_iter;
         }
         b->last_iters = 0;
+        GR_HIP(hipMemcpyAsync(b->ubuf[1 - b->cur].p, saved.p, saved.n, hipMemcpyDeviceToDevice, m->stream));
+        GR_HIP(hipStreamSynchronize(m->stream));
     });
 }
 

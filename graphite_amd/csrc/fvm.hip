@@ -664,6 +664,11 @@ struct gr_solver {
         } guard{ev};
         for (auto &e : ev) GR_HIP(hipEventCreate(&e));
         const int b = cur;
+        // the flux launches write the interior of buffer 1-b, which holds the state the ghost
+        // values of a download are built from: saved here and restored after the runs
+        gr::DevArray<char> saved;
+        saved.alloc(ubuf[1 - b].n);
+        GR_HIP(hipMemcpyAsync(saved.p, ubuf[1 - b].p, saved.n, hipMemcpyDeviceToDevice, m->stream));
         GR_HIP(hipEventRecord(ev[0], m->stream));
         for (int k = 0; k < n; ++k) {
             if (cfg.dtype == GR_F64) launch_gradlim<double>(b, nullptr);
@@ -686,6 +691,8 @@ struct gr_solver {
             GR_HIP(hipEventElapsedTime(&t, ev[j], ev[j + 1]));
             ms[j] = t / n;
         }
+        GR_HIP(hipMemcpyAsync(ubuf[1 - b].p, saved.p, saved.n, hipMemcpyDeviceToDevice, m->stream));
+        GR_HIP(hipStreamSynchronize(m->stream));
     }
     // Per-slot geometry records of the tile kernels, from the mesh metrics.
     template <typename S> void build_slot_geometry() {
@@ -951,6 +958,12 @@ int gr_solver_iterate(gr_solver *s, int max_iter, double tol, double reference, 
         c0.tol = tol;
         c0.max_iter = max_iter;
         c0.hist_cap = s->hist_cap;
+        if (s->cfg.use_graph) {
+            // both buffer parities captured before the first launch, so that no capture or
+            // instantiation ever falls between the launches of a run
+            s->graph(0);
+            s->graph(1);
+        }
         GR_HIP(hipMemcpyAsync(s->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
         const int K = s->graph_iters;
         const int launches = (max_iter + K - 1) / K;

@@ -173,6 +173,59 @@ __global__ void k_lsq_matrices(uint32_t N, const uint32_t *__restrict__ cfs, con
     mat[5 * ms + i] = m12 / det;
 }
 
+// Mesh validity after the metric kernels (counters: [0] interior cells with a non-positive or
+// non-finite volume, [1] face slots of interior cells whose face does not separate the two
+// cell centres (n.(x_f - x_i) and n.(x_f - x_j) not of strictly opposite signs: a face with
+// n.d = 0, where the reference's binary64 sign() returns 0 and drops the face from the flux,
+// or a tangled cell), [2] interior cells with a non-finite least-squares matrix, [3] first bad
+// cell; [4] faces with a non-positive or non-finite area or a non-finite normal, [5] first bad
+// face).
+__global__ void k_mesh_check_cells(uint32_t N, const uint32_t *__restrict__ cfs, const uint32_t *__restrict__ cf,
+                                   const uint32_t *__restrict__ cc, const double *__restrict__ fgeo, size_t fs,
+                                   const double *__restrict__ cgeo, size_t cs, const double *__restrict__ mat,
+                                   size_t ms, int check_mat, uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    bool bad = false;
+    const double v = cgeo[3 * cs + i];
+    if (!(v > 0.0) || !isfinite(v)) {
+        atomicAdd(&out[0], 1u);
+        bad = true;
+    }
+    const double xi[3] = {cgeo[i], cgeo[cs + i], cgeo[2 * cs + i]};
+    for (uint32_t q = cfs[i]; q < cfs[i + 1]; ++q) {
+        const uint32_t f = cf[q], j = cc[q];
+        const double fc[3] = {fgeo[f], fgeo[fs + f], fgeo[2 * fs + f]};
+        const double n[3] = {fgeo[3 * fs + f], fgeo[4 * fs + f], fgeo[5 * fs + f]};
+        const double si = (n[0] * (fc[0] - xi[0]) + n[1] * (fc[1] - xi[1])) + n[2] * (fc[2] - xi[2]);
+        const double sj = (n[0] * (fc[0] - cgeo[j]) + n[1] * (fc[1] - cgeo[cs + j])) + n[2] * (fc[2] - cgeo[2 * cs + j]);
+        if (!((si > 0.0 && sj < 0.0) || (si < 0.0 && sj > 0.0))) {
+            atomicAdd(&out[1], 1u);
+            bad = true;
+        }
+    }
+    if (check_mat) {
+        bool fin = true;
+        for (int k = 0; k < 6; ++k) fin = fin && isfinite(mat[k * ms + i]);
+        if (!fin) {
+            atomicAdd(&out[2], 1u);
+            bad = true;
+        }
+    }
+    if (bad) atomicMin(&out[3], i);
+}
+
+__global__ void k_mesh_check_faces(uint32_t F, const double *__restrict__ fgeo, size_t fs, uint32_t *__restrict__ out) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const double a = fgeo[6 * fs + f];
+    const bool fin = isfinite(fgeo[3 * fs + f]) && isfinite(fgeo[4 * fs + f]) && isfinite(fgeo[5 * fs + f]);
+    if (!(a > 0.0) || !isfinite(a) || !fin) {
+        atomicAdd(&out[4], 1u);
+        atomicMin(&out[5], f);
+    }
+}
+
 }  // namespace gr
 
 // ---- gr_mesh ------------------------------------------------------------------------------
@@ -413,8 +466,32 @@ int gr_mesh_compute_metrics(gr_mesh *m) {
                                h.n_cells, m->cfs.p, m->cc.p, m->cell_geo.p, cs, m->mat.p, ms);
         else
             GR_HIP(hipMemsetAsync(m->mat.p, 0, m->mat.n * sizeof(double), m->stream));
+        // reject meshes the scheme cannot run on (the reference runs them silently)
+        gr::DevArray<uint32_t> chk;
+        chk.alloc(6);
+        const uint32_t init[6] = {0u, 0u, 0u, 0xFFFFFFFFu, 0u, 0xFFFFFFFFu};
+        chk.upload(init, 6, m->stream);
+        if (h.n_cells)
+            hipLaunchKernelGGL(gr::k_mesh_check_cells, dim3(gr::blocks_for(h.n_cells)), dim3(gr::kBlock), 0, m->stream,
+                               h.n_cells, m->cfs.p, m->cf.p, m->cc.p, m->face_geo.p, fs, m->cell_geo.p, cs, m->mat.p, ms,
+                               h.one_dimensional ? 0 : 1, chk.p);
+        if (h.n_faces)
+            hipLaunchKernelGGL(gr::k_mesh_check_faces, dim3(gr::blocks_for(h.n_faces)), dim3(gr::kBlock), 0, m->stream,
+                               h.n_faces, m->face_geo.p, fs, chk.p);
         GR_HIP(hipGetLastError());
+        uint32_t c[6];
+        chk.download(c, 6, m->stream);
         GR_HIP(hipStreamSynchronize(m->stream));
+        m->metrics_ready = false;
+        if (c[4])
+            throw gr::Error(GR_ERR_TOPOLOGY, std::to_string(c[4]) + " face(s) with a non-positive or non-finite area "
+                                             "(first: face " + std::to_string(c[5]) + ")");
+        if (c[0] || c[1] || c[2])
+            throw gr::Error(GR_ERR_TOPOLOGY,
+                            "invalid mesh: " + std::to_string(c[0]) + " cell(s) with a non-positive volume, " +
+                                std::to_string(c[1]) + " face slot(s) whose face does not separate the two cell "
+                                "centres (n.d = 0 or a tangled cell), " + std::to_string(c[2]) +
+                                " singular least-squares matrix(es); first: cell " + std::to_string(c[3]));
         m->metrics_ready = true;
     });
 }

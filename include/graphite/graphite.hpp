@@ -121,30 +121,60 @@ struct FvmFieldBase {
     void *value = nullptr, *gradient = nullptr, *limiter = nullptr;
 };
 struct FvmField {
-    gr_field *h = nullptr;
+    gr_field *h = nullptr;        // created when the field is bound to a mesh
     cuMesh *mesh = nullptr;
     FvmFieldBase *_cpu = nullptr;
     std::vector<double> v, g, l;
     std::string name;
     unsigned ncomp = 1;
+    size_t size = 0;
+    int alloc_grads = 0;
 };
 
-// FvmFieldNew (src/fvmfields.cu:7-46): the field is sized for the mesh with size cells+ghosts.
+namespace graphite_detail {
+// The reference's fields are bare arrays sized cells+ghosts; the mesh comes with each call.  A
+// field here is bound to the mesh of its first call that names one (and must be used with that
+// mesh afterwards).  Calls without a mesh (VarGpuToCpu, VarCpuToGpu) on an unbound field bind it
+// to the one registered mesh of its size, and refuse when several meshes have that size.
+inline gr_field *bind(FvmField *f, cuMesh *m) {
+    if (!f) throw graphite_error(GR_ERR_INVALID, "null field");
+    if (f->mesh) {
+        if (m && m != f->mesh)
+            throw graphite_error(GR_ERR_INVALID, "field '" + f->name + "' is bound to another mesh");
+        return f->h;
+    }
+    if (!m) {
+        int count = 0;
+        for (cuMesh *c : registry().meshes)
+            if (c->h && (size_t)c->sizes.n_cells + c->sizes.n_ghosts == f->size) {
+                m = c;
+                ++count;
+            }
+        if (count == 0)
+            throw graphite_error(GR_ERR_INVALID, "field '" + f->name + "': no mesh with this many cells and ghosts");
+        if (count > 1)
+            throw graphite_error(GR_ERR_INVALID, "field '" + f->name + "': " + std::to_string(count) +
+                                                     " meshes have this many cells and ghosts; pass the field to a "
+                                                     "call that names its mesh first");
+    }
+    if (!m->h) throw graphite_error(GR_ERR_INVALID, "mesh not loaded");
+    if ((size_t)m->sizes.n_cells + m->sizes.n_ghosts != f->size)
+        throw graphite_error(GR_ERR_INVALID, "field '" + f->name + "' is not sized for this mesh");
+    check(gr_field_create(m->h, f->ncomp, GR_F64, f->name.c_str(), f->alloc_grads, &f->h));
+    f->mesh = m;
+    return f->h;
+}
+}  // namespace graphite_detail
+
+// FvmFieldNew (src/fvmfields.cu:7-46): a field of `size` elements (cells + ghosts of the mesh it
+// will be used with), zero-initialised on the device when it is bound.
 inline FvmField *FvmFieldNew(size_t elem_size, size_t size, const char *name, int alloc_grads) {
-    cuMesh *mesh = nullptr;
-    for (cuMesh *m : graphite_detail::registry().meshes)
-        if (m->h && cuMeshGetVarSize(m) == size) mesh = m;
-    if (!mesh) throw graphite_error(GR_ERR_INVALID, "FvmFieldNew: no mesh with this many cells and ghosts");
     const unsigned nc = elem_size == sizeof(vector) ? 3 : 1;
     auto *f = new FvmField();
-    f->mesh = mesh;
     f->ncomp = nc;
     f->name = name ? name : "";
-    int rc = gr_field_create(mesh->h, nc, GR_F64, name, alloc_grads, &f->h);
-    if (rc != GR_OK) {
-        delete f;
-        graphite_detail::check(rc);
-    }
+    f->size = size;
+    f->alloc_grads = alloc_grads;
     f->v.assign(size * nc, 0.0);
     if (alloc_grads) {
         f->g.assign(size * nc * 3, 0.0);
@@ -156,44 +186,54 @@ inline FvmField *FvmFieldNew(size_t elem_size, size_t size, const char *name, in
 }
 inline void FvmFieldFree(FvmField *f) {
     if (!f) return;
-    gr_field_free(f->h);
+    if (f->h) gr_field_free(f->h);
     delete f->_cpu;
     delete f;
 }
 inline FvmFieldType FvmFieldGetType(const FvmField *f) { return f->ncomp == 3 ? VECTOR : SCALAR; }
 inline void *FvmFieldValueIndex(FvmField *f, const size_t &i) { return f->v.data() + f->ncomp * i; }
-inline void VarGpuToCpu(FvmField *f) { graphite_detail::check(gr_field_download(f->h, f->v.data(), nullptr, nullptr)); }
+inline void VarGpuToCpu(FvmField *f) {
+    graphite_detail::check(gr_field_download(graphite_detail::bind(f, nullptr), f->v.data(), nullptr, nullptr));
+}
 inline void VarAllGpuToCpu(FvmField *f) {
-    graphite_detail::check(gr_field_download(f->h, f->v.data(), f->g.empty() ? nullptr : f->g.data(),
-                                             f->l.empty() ? nullptr : f->l.data()));
+    graphite_detail::check(gr_field_download(graphite_detail::bind(f, nullptr), f->v.data(),
+                                             f->g.empty() ? nullptr : f->g.data(), f->l.empty() ? nullptr : f->l.data()));
 }
 // Host -> device of the host mirror (the reference writes _hgpu arrays with cudaMemcpy).
-inline void VarCpuToGpu(FvmField *f) { graphite_detail::check(gr_field_upload(f->h, f->v.data())); }
+inline void VarCpuToGpu(FvmField *f) {
+    graphite_detail::check(gr_field_upload(graphite_detail::bind(f, nullptr), f->v.data()));
+}
 
-// fvmops (src/fvmops.cu) as host calls
-inline void ScalarVarComputeGradient(cuMesh *, FvmField *f) { graphite_detail::check(gr_field_compute_gradient(f->h)); }
-inline void VectorVarComputeGradient(cuMesh *, FvmField *f) { graphite_detail::check(gr_field_compute_gradient(f->h)); }
-inline void ScalarVarComputeLimiters(cuMesh *, FvmField *f) { graphite_detail::check(gr_field_compute_limiter(f->h, GR_LIM_MICHALAK)); }
-inline void VectorVarComputeLimiters(cuMesh *, FvmField *f) { graphite_detail::check(gr_field_compute_limiter(f->h, GR_LIM_MICHALAK)); }
-inline void fvmScalGradLimBc(cuMesh *, FvmField *f) { graphite_detail::check(gr_field_gradlim_bc(f->h)); }
-inline void fvmVecGradLimBc(cuMesh *, FvmField *f) { graphite_detail::check(gr_field_gradlim_bc(f->h)); }
-inline void ScalarVarInit(cuMesh *, FvmField *f, scalar value) { graphite_detail::check(gr_field_init(f->h, &value)); }
-inline void VectorVarInit(cuMesh *, FvmField *f, vector value) {
+// fvmops (src/fvmops.cu) as host calls; the field is bound to (or must belong to) the mesh given
+namespace graphite_detail {
+inline gr_field *on(cuMesh *m, FvmField *f) {
+    if (!m) throw graphite_error(GR_ERR_INVALID, "null mesh");
+    return bind(f, m);
+}
+}  // namespace graphite_detail
+inline void ScalarVarComputeGradient(cuMesh *m, FvmField *f) { graphite_detail::check(gr_field_compute_gradient(graphite_detail::on(m, f))); }
+inline void VectorVarComputeGradient(cuMesh *m, FvmField *f) { graphite_detail::check(gr_field_compute_gradient(graphite_detail::on(m, f))); }
+inline void ScalarVarComputeLimiters(cuMesh *m, FvmField *f) { graphite_detail::check(gr_field_compute_limiter(graphite_detail::on(m, f), GR_LIM_MICHALAK)); }
+inline void VectorVarComputeLimiters(cuMesh *m, FvmField *f) { graphite_detail::check(gr_field_compute_limiter(graphite_detail::on(m, f), GR_LIM_MICHALAK)); }
+inline void fvmScalGradLimBc(cuMesh *m, FvmField *f) { graphite_detail::check(gr_field_gradlim_bc(graphite_detail::on(m, f))); }
+inline void fvmVecGradLimBc(cuMesh *m, FvmField *f) { graphite_detail::check(gr_field_gradlim_bc(graphite_detail::on(m, f))); }
+inline void ScalarVarInit(cuMesh *m, FvmField *f, scalar value) { graphite_detail::check(gr_field_init(graphite_detail::on(m, f), &value)); }
+inline void VectorVarInit(cuMesh *m, FvmField *f, vector value) {
     double v[3] = {value.x, value.y, value.z};
-    graphite_detail::check(gr_field_init(f->h, v));
+    graphite_detail::check(gr_field_init(graphite_detail::on(m, f), v));
 }
-inline void fvmScalarBcFixed(cuMesh *, FvmField *f, scalar bc_value, unsigned region_id) {
-    graphite_detail::check(gr_field_bc_fixed(f->h, region_id, &bc_value));
+inline void fvmScalarBcFixed(cuMesh *m, FvmField *f, scalar bc_value, unsigned region_id) {
+    graphite_detail::check(gr_field_bc_fixed(graphite_detail::on(m, f), region_id, &bc_value));
 }
-inline void fvmScalarBcZeroGrad(cuMesh *, FvmField *f, unsigned region_id) {
-    graphite_detail::check(gr_field_bc_zero_grad(f->h, region_id));
+inline void fvmScalarBcZeroGrad(cuMesh *m, FvmField *f, unsigned region_id) {
+    graphite_detail::check(gr_field_bc_zero_grad(graphite_detail::on(m, f), region_id));
 }
-inline void fvmVectorBcZeroGrad(cuMesh *, FvmField *f, unsigned region_id) {
-    graphite_detail::check(gr_field_bc_zero_grad(f->h, region_id));
+inline void fvmVectorBcZeroGrad(cuMesh *m, FvmField *f, unsigned region_id) {
+    graphite_detail::check(gr_field_bc_zero_grad(graphite_detail::on(m, f), region_id));
 }
-inline scalar FvmFieldReduceSum(cuMesh *, FvmField *f) {
+inline scalar FvmFieldReduceSum(cuMesh *m, FvmField *f) {
     double s = 0;
-    graphite_detail::check(gr_field_reduce_sum(f->h, &s));
+    graphite_detail::check(gr_field_reduce_sum(graphite_detail::on(m, f), &s));
     return s;
 }
 
@@ -201,26 +241,32 @@ inline scalar FvmFieldReduceSum(cuMesh *, FvmField *f) {
 inline int FvmEulerInit(cuMesh *m, FvmField *rho, FvmField *rhou, FvmField *rhoe, const vector &mach,
                         const scalar &pressure, const scalar &temperature) {
     double mv[3] = {mach.x, mach.y, mach.z};
-    graphite_detail::check(gr_euler_init(m->h, rho->h, rhou->h, rhoe->h, mv, pressure, temperature));
+    graphite_detail::check(gr_euler_init(m->h, graphite_detail::on(m, rho), graphite_detail::on(m, rhou),
+                                         graphite_detail::on(m, rhoe), mv, pressure, temperature));
     return 0;
 }
 inline int FvmEulerSlipBc(cuMesh *m, FvmField *rho, FvmField *rhou, FvmField *rhoe, const char *region) {
-    graphite_detail::check(gr_euler_bc_slip(m->h, rho->h, rhou->h, rhoe->h, region));
+    graphite_detail::check(gr_euler_bc_slip(m->h, graphite_detail::on(m, rho), graphite_detail::on(m, rhou),
+                                            graphite_detail::on(m, rhoe), region));
     return 0;
 }
 inline int FvmEulerInletOutletBc(cuMesh *m, FvmField *rho, FvmField *rhou, FvmField *rhoe, const vector &mach,
                                  const scalar &pressure, const scalar &temperature, const char *region) {
     double mv[3] = {mach.x, mach.y, mach.z};
-    graphite_detail::check(gr_euler_bc_farfield(m->h, rho->h, rhou->h, rhoe->h, mv, pressure, temperature, region));
+    graphite_detail::check(gr_euler_bc_farfield(m->h, graphite_detail::on(m, rho), graphite_detail::on(m, rhou),
+                                                graphite_detail::on(m, rhoe), mv, pressure, temperature, region));
     return 0;
 }
 inline int FvmEulerStepPrepare(cuMesh *m, FvmField *rho, FvmField *rhou, FvmField *rhoe) {
-    graphite_detail::check(gr_euler_step_prepare(m->h, rho->h, rhou->h, rhoe->h, GR_LIM_MICHALAK));
+    graphite_detail::check(gr_euler_step_prepare(m->h, graphite_detail::on(m, rho), graphite_detail::on(m, rhou),
+                                                 graphite_detail::on(m, rhoe), GR_LIM_MICHALAK));
     return 0;
 }
 inline int FvmEulerStep(cuMesh *m, FvmField *rho, FvmField *rhou, FvmField *rhoe, const scalar &cfl,
                         const char *time_scheme, FvmField *dt, FvmField *error) {
-    graphite_detail::check(gr_euler_step(m->h, rho->h, rhou->h, rhoe->h, cfl, time_scheme, dt ? dt->h : nullptr,
-                                         error ? error->h : nullptr));
+    graphite_detail::check(gr_euler_step(m->h, graphite_detail::on(m, rho), graphite_detail::on(m, rhou),
+                                         graphite_detail::on(m, rhoe), cfl, time_scheme,
+                                         dt ? graphite_detail::on(m, dt) : nullptr,
+                                         error ? graphite_detail::on(m, error) : nullptr));
     return 0;
 }

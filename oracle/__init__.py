@@ -83,6 +83,8 @@ def lib() -> C.CDLL:
                                      C.POINTER(C.POINTER(_OrcMesh))]
         L.orc_mesh_free.argtypes = [C.POINTER(_OrcMesh)]
         L.orc_mesh_metrics.argtypes = [C.POINTER(_OrcMesh)]
+        L.orc_mesh_check.restype = None
+        L.orc_mesh_check.argtypes = [C.POINTER(_OrcMesh), C.POINTER(C.c_uint32)]
         L.orc_limiter_face.restype = C.c_double
         L.orc_limiter_face.argtypes = [C.c_double, _f64p, C.c_double, C.c_double, _f64p, C.c_int]
         mp = C.POINTER(_OrcMesh)
@@ -140,9 +142,10 @@ def _arr(ptr, n, dtype):
 class OracleMesh:
     """Mesh built and measured by the oracle; arrays are numpy copies."""
 
-    def __init__(self, elements):
+    def __init__(self, elements, validate: bool = True):
         """elements: object with nodes (n,3) f64, elem_types, elem_starts, elem_nodes (u32) and
-        markers = list of (name, types, starts, nodes)."""
+        markers = list of (name, types, starts, nodes).  With validate, a mesh the scheme cannot
+        run on (see orc_mesh_check) raises ValueError; `invalid` holds the four counts."""
         L = lib()
         nodes = np.ascontiguousarray(elements.nodes, dtype=np.float64).reshape(-1)
         et = np.ascontiguousarray(elements.elem_types, dtype=np.uint32)
@@ -166,6 +169,12 @@ class OracleMesh:
         self.region_names = [m[0] for m in elements.markers]
         L.orc_mesh_metrics(out)
         self._copy()
+        counts = (C.c_uint32 * 4)()
+        L.orc_mesh_check(out, counts)
+        self.invalid = tuple(int(c) for c in counts)
+        if validate and any(self.invalid):
+            raise ValueError("oracle: invalid mesh (non-positive volumes, faces not separating their cells, "
+                             f"singular LSQ matrices, degenerate faces) = {self.invalid}")
 
     def _copy(self):
         s = self._p.contents

@@ -83,6 +83,9 @@ void orc_mesh_free(orc_mesh *m);
 
 /* All four metric kernels in order: faces, cells, ghost cells, LSQ matrices. */
 void orc_mesh_metrics(orc_mesh *m);
+/* Validity counts after orc_mesh_metrics (non-positive volumes, faces that do not separate
+ * their two cell centres, singular least-squares matrices, degenerate faces). */
+void orc_mesh_check(const orc_mesh *m, uint32_t counts[4]);
 
 /* ---- FV operators (src/fvmops.cu) -------------------------------------------------------- */
 

@@ -456,3 +456,36 @@ void orc_mesh_metrics(orc_mesh *m)
         for (int k = 0; k < 9; ++k) m->cell_matrices[9 * i + k] = M[k] / det;
     }
 }
+
+/* Mesh validity, the rule the product applies after its metric kernels (the reference runs such
+ * meshes silently).  counts[0]: interior cells with a volume that is not positive and finite;
+ * [1]: face slots of interior cells whose face does not separate the two cell centres, i.e.
+ * n.(x_f - x_i) and n.(x_f - x_j) are not of strictly opposite signs (a face with n.d = 0, where
+ * the binary64 sign() of include/datatypes.cuh:209-211 returns 0 and outer_normal drops the face,
+ * or a tangled cell); [2]: interior cells with a non-finite least-squares matrix; [3]: faces
+ * whose area is not positive and finite or whose normal is not finite. */
+void orc_mesh_check(const orc_mesh *m, uint32_t counts[4])
+{
+    const uint32_t N = m->n_cells;
+    counts[0] = counts[1] = counts[2] = counts[3] = 0;
+    for (uint32_t i = 0; i < N; ++i) {
+        const double v = m->cell_volumes[i];
+        if (!(v > 0.0) || !isfinite(v)) counts[0]++;
+        const double *xi = &m->cell_centers[3 * i];
+        for (uint32_t q = m->cell_faces_starts[i]; q < m->cell_faces_starts[i + 1]; ++q) {
+            const uint32_t f = m->cell_faces[q], j = m->cell_connects[q];
+            const double *fc = &m->face_centers[3 * f], *n = &m->face_normals[3 * f];
+            const double *xj = &m->cell_centers[3 * j];
+            const double si = (n[0] * (fc[0] - xi[0]) + n[1] * (fc[1] - xi[1])) + n[2] * (fc[2] - xi[2]);
+            const double sj = (n[0] * (fc[0] - xj[0]) + n[1] * (fc[1] - xj[1])) + n[2] * (fc[2] - xj[2]);
+            if (!((si > 0.0 && sj < 0.0) || (si < 0.0 && sj > 0.0))) counts[1]++;
+        }
+        int fin = 1;
+        for (int k = 0; k < 9; ++k) fin = fin && isfinite(m->cell_matrices[9 * (size_t)i + k]);
+        if (!fin) counts[2]++;
+    }
+    for (uint32_t f = 0; f < m->n_faces; ++f) {
+        const double a = m->face_areas[f], *n = &m->face_normals[3 * f];
+        if (!(a > 0.0) || !isfinite(a) || !isfinite(n[0]) || !isfinite(n[1]) || !isfinite(n[2])) counts[3]++;
+    }
+}

@@ -29,8 +29,8 @@ def machs(n, seed=0):
 BOX_BCS = [("xmin", 1), ("xmax", 1), ("ymin", 0), ("ymax", 0), ("zmin", 1), ("zmax", 0)]
 
 
-def single(gr, m, mach, n_iter, dtype, state=None, tol=0.0, ref=0.0):
-    sv = gr.Solver(m, dtype=dtype, cfl=0.5)
+def single(gr, m, mach, n_iter, dtype, state=None, tol=0.0, ref=0.0, corrected=False):
+    sv = gr.Solver(m, dtype=dtype, cfl=0.5, corrected=corrected)
     for name, kind in BOX_BCS:
         sv.set_bc(name, kind, mach, P_INF, T_INF)
     if state is None:
@@ -41,10 +41,11 @@ def single(gr, m, mach, n_iter, dtype, state=None, tol=0.0, ref=0.0):
     return sv, res
 
 
-def test_batch_instances_equal_single_solves(gr, box):
+@pytest.mark.parametrize("corrected", [False, True])
+def test_batch_instances_equal_single_solves(gr, box, corrected):
     m = box
     M = machs(5, seed=1)
-    b = gr.Batch(m, 5, gr.GR_F64, cfl=0.5)
+    b = gr.Batch(m, 5, gr.GR_F64, cfl=0.5, corrected=corrected)
     for name, kind in BOX_BCS:
         b.set_bc(name, kind)
     b.set_freestream(M, P_INF, T_INF)
@@ -56,7 +57,7 @@ def test_batch_instances_equal_single_solves(gr, box):
     its, conv, ran = b.iterate(15)
     assert list(its) == [15] * 5 and ran == 15
     for k in range(5):
-        sv, res = single(gr, m, M[k], 15, gr.GR_F64, state=states.get(k))
+        sv, res = single(gr, m, M[k], 15, gr.GR_F64, state=states.get(k), corrected=corrected)
         for x, y in zip(b.get_state(k), sv.get_state()):
             assert np.array_equal(x, y), k
         # states are bitwise equal; the residual sums add the same cell terms in another order
@@ -129,17 +130,41 @@ def test_batch_cgrid_64_instances(gr):
     assert np.all(np.isfinite(r)) and r.shape == (64,)
 
 
-def test_batch_binary32_equals_single_binary32(gr, box):
+@pytest.mark.parametrize("corrected", [False, True])
+def test_batch_binary32_equals_single_binary32(gr, box, corrected):
     """The batched kernels and the single-flow tile kernels evaluate the same binary32
-    arithmetic (gradient rows premultiplied by their limiters in both): bitwise equal states."""
+    arithmetic (gradient rows premultiplied by their limiters in both): bitwise equal states,
+    with the reference's transposed momentum reconstruction and with the corrected one."""
     m = box
     M = machs(3, seed=11)
-    b = gr.Batch(m, 3, gr.GR_F32, cfl=0.5)
+    b = gr.Batch(m, 3, gr.GR_F32, cfl=0.5, corrected=corrected)
     for name, kind in BOX_BCS:
         b.set_bc(name, kind)
     b.set_freestream(M, P_INF, T_INF)
     b.iterate(12)
     for k in range(3):
-        sv, res = single(gr, m, M[k], 12, gr.GR_F32)
+        sv, res = single(gr, m, M[k], 12, gr.GR_F32, corrected=corrected)
         for x, y in zip(b.get_state(k), sv.get_state()):
             assert np.array_equal(x[:m.n_cells], y[:m.n_cells]), k
+
+
+def test_batch_profile_leaves_downloads_unchanged(gr, box):
+    """The batch's timing runs write its second state buffer: iterate -> profile -> get_state
+    must return bitwise what iterate -> get_state returns, ghosts included."""
+    m = box
+    M = machs(3, seed=4)
+
+    def run(profile):
+        b = gr.Batch(m, 3, gr.GR_F64, cfl=0.5)
+        for name, kind in BOX_BCS:
+            b.set_bc(name, kind)
+        b.set_freestream(M, P_INF, T_INF)
+        b.set_state(1, *perturbed_state(m.metrics()["cell_centers"], mach=M[1]))
+        b.iterate(11)
+        if profile:
+            b.profile(3)
+        return [b.get_state(k) for k in range(3)]
+
+    for sa, sb in zip(run(False), run(True)):
+        for x, y in zip(sa, sb):
+            assert np.array_equal(x, y)
