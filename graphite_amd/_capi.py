@@ -92,12 +92,14 @@ SIGNATURES = {
     "gr_mesh_bandwidth": (_i, [_vp, _u32p]),
     "gr_mesh_compute_metrics": (_i, [_vp]),
     "gr_mesh_get_metrics": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "gr_mesh_get_view": (_i, [_vp, _vp]),
     "gr_field_create": (_i, [_vp, _u32, _i, _s, _i, C.POINTER(_vp)]),
     "gr_field_free": (None, [_vp]),
     "gr_field_upload": (_i, [_vp, _dp]),
     "gr_field_download": (_i, [_vp, _dp, _dp, _dp]),
     "gr_field_init": (_i, [_vp, _dp]),
     "gr_field_operator": (_i, [_vp, _i, _dp, _vp, _dp]),
+    "gr_field_get_view": (_i, [_vp, _vp]),
     "gr_field_reduce_sum": (_i, [_vp, _dp]),
     "gr_field_compute_gradient": (_i, [_vp]),
     "gr_field_compute_limiter": (_i, [_vp, _i]),

@@ -496,6 +496,35 @@ int gr_mesh_compute_metrics(gr_mesh *m) {
     });
 }
 
+int gr_mesh_get_view(const gr_mesh *m, gr_mesh_view *out) {
+    return guarded([&] {
+        require(m && out, "null argument");
+        require(m->metrics_ready, "metrics not computed (gr_mesh_compute_metrics)");
+        const gr::MeshView v = m->view();
+        gr_mesh_view o{};
+        o.n_cells = v.N;
+        o.n_ghosts = v.G;
+        o.n_faces = v.F;
+        o.cell_faces_starts = v.cfs;
+        o.cell_faces = v.cf;
+        o.cell_connects = v.cc;
+        o.face_center_x = v.fcx;
+        o.face_center_y = v.fcy;
+        o.face_center_z = v.fcz;
+        o.face_normal_x = v.fnx;
+        o.face_normal_y = v.fny;
+        o.face_normal_z = v.fnz;
+        o.face_area = v.fa;
+        o.cell_center_x = v.cx;
+        o.cell_center_y = v.cy;
+        o.cell_center_z = v.cz;
+        o.cell_volume = v.cv;
+        o.lsq_matrix = v.mat;
+        o.lsq_stride = v.mat_stride;
+        *out = o;
+    });
+}
+
 int gr_mesh_get_metrics(const gr_mesh *m, double *face_centers, double *face_normals, double *face_areas,
                         double *cell_centers, double *cell_volumes, double *cell_matrices) {
     return guarded([&] {

@@ -168,6 +168,24 @@ int gr_mesh_compute_metrics(gr_mesh *m);
 int gr_mesh_get_metrics(const gr_mesh *m, double *face_centers, double *face_normals,
                         double *face_areas, double *cell_centers, double *cell_volumes,
                         double *cell_matrices);
+/* The mesh as a device kernel reads it (the reference's device struct cuMeshBase* that its
+ * kernels take, include/mesh.cuh:18-52, kernelInput(x) = x->_gpu, include/datatypes.cuh:76):
+ * device pointers to the CSR arrays and the binary64 metric blocks (centres, unit normals,
+ * areas; cell centres and volumes over cells and ghosts; LSQ matrices as six blocks xx, yy, zz,
+ * xy, xz, yz of stride lsq_stride).  Valid until gr_mesh_free, and until gr_mesh_renumber_rcm
+ * or gr_mesh_compute_metrics run again.  include/graphite/fvm_ops.hpp builds on it. */
+typedef struct gr_mesh_view {
+    uint32_t n_cells, n_ghosts, n_faces;
+    const uint32_t *cell_faces_starts, *cell_faces, *cell_connects;
+    const double *face_center_x, *face_center_y, *face_center_z;
+    const double *face_normal_x, *face_normal_y, *face_normal_z;
+    const double *face_area;
+    const double *cell_center_x, *cell_center_y, *cell_center_z;
+    const double *cell_volume;
+    const double *lsq_matrix;
+    size_t lsq_stride;
+} gr_mesh_view;
+int gr_mesh_get_view(const gr_mesh *m, gr_mesh_view *out);
 
 /* ---- fields (FvmField, include/fvmfields.cuh:16-43) ---------------------------------------- */
 typedef struct gr_field gr_field;
@@ -203,6 +221,19 @@ enum gr_fvm_op {
     GR_OP_CONV_VEC_U = 10
 };
 int gr_field_operator(const gr_field *u, int op, const double phi_const[3], const gr_field *phi_field, double *out);
+/* A field as a device kernel reads it (FvmFieldBase's device arrays, include/fvmfields.cuh:16-30):
+ * per component a block of n = cells + ghosts values, gradients (component c, direction k at
+ * gradient[3c+k]) and limiters (NULL without alloc_grads), of dtype GR_F64 or GR_F32.  Valid
+ * until the field is freed or gr_euler_step writes it (that call swaps its value buffer). */
+typedef struct gr_field_view {
+    int dtype;
+    uint32_t n_comp;
+    size_t n;
+    void *value[3];
+    void *gradient[9];
+    void *limiter[3];
+} gr_field_view;
+int gr_field_get_view(const gr_field *f, gr_field_view *out);
 /* ScalarVarInit / VectorVarInit (src/fvmops.cu:8-43): value = v, gradient 0, limiter 1. */
 int gr_field_init(gr_field *f, const double *v);
 /* FvmFieldReduceSum (src/fvmops.cu:369-383): sum over interior cells, device reduction in
