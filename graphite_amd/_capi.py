@@ -5,7 +5,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgraphite_hip.so")
+# GRAPHITE_HIP_LIB selects another build of the same library (the sanitizer build of the host code)
+LIB_PATH = os.environ.get("GRAPHITE_HIP_LIB") or os.path.join(HERE, "libgraphite_hip.so")
 
 GR_OK, GR_ERR_INVALID, GR_ERR_TOPOLOGY, GR_ERR_IO, GR_ERR_NOT_FOUND, GR_ERR_DEVICE, GR_ERR_DIVERGED = range(7)
 GR_F32, GR_F64 = 4, 8

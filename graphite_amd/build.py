@@ -1,6 +1,6 @@
 """Builds libgraphite_hip.so in-tree: HIP sources with hipcc for gfx950, host C++ with g++.
 
-Usage: python -m graphite_amd.build [-v]
+Usage: python -m graphite_amd.build [-v] [--sanitize]
 """
 from __future__ import annotations
 
@@ -64,5 +64,32 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+SAN_DIR = os.path.join(OBJ, "san")
+SAN_LIB = os.path.join(SAN_DIR, "libgraphite_hip.so")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(verbose: bool = False) -> str:
+    """The library with its host C++ sources (connectivity, generators, file I/O, case driver)
+    compiled by g++ under AddressSanitizer and UndefinedBehaviorSanitizer, linked with the
+    regular HIP objects; for CPU test runs with libasan preloaded.  Returns its path."""
+    build(verbose)
+    os.makedirs(SAN_DIR, exist_ok=True)
+    hip, cpp = _sources()
+    objs = [os.path.join(OBJ, f + ".o") for f in hip]
+    for f in cpp:
+        o = os.path.join(SAN_DIR, f + ".o")
+        _run(["g++", "-c", os.path.join(CSRC, f), "-o", o] + CXX_FLAGS + SAN_FLAGS, verbose)
+        objs.append(o)
+    rocm_lib = os.path.join(ROCM, "lib")
+    _run(["g++", "-shared", "-o", SAN_LIB + ".tmp"] + objs + SAN_FLAGS +
+         [f"-L{rocm_lib}", "-lamdhip64", f"-Wl,-rpath,{rocm_lib}"], verbose)
+    os.replace(SAN_LIB + ".tmp", SAN_LIB)
+    return SAN_LIB
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--sanitize" in sys.argv:
+        print(build_sanitized(verbose="-v" in sys.argv))
+    else:
+        print(build(verbose="-v" in sys.argv))

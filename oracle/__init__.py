@@ -18,7 +18,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# GRAPHITE_ORACLE_LIB selects another build of the oracle (its sanitizer build)
+_LIB_PATH = os.environ.get("GRAPHITE_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 LIM_MICHALAK, LIM_BARTH_JESPERSEN, LIM_VENKATAKRISHNAN = 0, 1, 2
 BC_SLIP, BC_FARFIELD = 0, 1
