@@ -113,6 +113,7 @@ SIGNATURES = {
     "gr_euler_step_prepare": (_i, [_vp, _vp, _vp, _vp, _i]),
     "gr_euler_step": (_i, [_vp, _vp, _vp, _vp, _d, _s, _vp, _vp]),
     "gr_euler_residual": (_i, [_vp, _vp, _vp, _vp, _dp]),
+    "gr_euler_residual_noalloc": (_i, [_vp, _vp, _vp, _vp, _dp]),
     "gr_euler_set_corrected_reconstruction": (_i, [_vp, _i]),
     "gr_solver_create": (_i, [_vp, C.POINTER(SolverConfig), C.POINTER(_vp)]),
     "gr_solver_free": (None, [_vp]),

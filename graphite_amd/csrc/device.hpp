@@ -78,6 +78,7 @@ struct gr_mesh {
     bool on_device = false;
     bool metrics_ready = false;
     int corrected = 0;   // momentum reconstruction of gr_euler_step
+    int device = -1;     // HIP device current when the mesh went to the device
     gr::DevArray<uint32_t> fns, fn, cfs, cf, cc;
     gr::DevArray<double> nodes;
     gr::DevArray<double> face_geo;   // 7 blocks of pad64(F): cx,cy,cz,nx,ny,nz,a

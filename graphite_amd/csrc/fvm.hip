@@ -5,6 +5,8 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <utility>
 
 #include "fvm_tiles.hpp"
@@ -538,6 +540,66 @@ int gr_euler_residual(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe,
                                nullptr, 0, nullptr);
         GR_HIP(hipGetLastError());
         r.download(resid, 5 * (size_t)m->h.n_cells, m->stream);
+        GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+}  // extern "C"
+
+// ---- residual without allocation -------------------------------------------------------------
+// The staging buffer of gr_euler_residual_noalloc: a file-scope device array each GPU holds once
+// for the life of the process (32 MiB), used through its device address, which is looked up once
+// per device (hipGetSymbolAddress, that device current) and kept in g_scratch_ptr.  Calls that
+// overlap on one GPU share it and race.
+namespace {
+constexpr size_t kScratchDoubles = size_t(1) << 22;
+constexpr int kMaxDevices = 64;
+static_assert(kScratchDoubles / 5 == GR_RESIDUAL_NOALLOC_MAX_CELLS, "header and scratch size disagree");
+__device__ __align__(256) double scratch[kScratchDoubles];
+double *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+double *scratch_ptr(int device) {
+    require(device >= 0 && device < kMaxDevices, "device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        GR_HIP(hipGetDevice(&cur));
+        require(cur == device, "the mesh's device must be the current device");
+        void *p = nullptr;
+        GR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "no device address for the residual scratch array");
+        g_scratch_ptr[device] = (double *)p;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+extern "C" {
+
+int gr_euler_residual_noalloc(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, double *resid) {
+    return guarded([&] {
+        check_euler_fields(m, rho, rhou, rhoe, true);
+        require(resid != nullptr, "null output");
+        const size_t N = m->h.n_cells;
+        if (N > kScratchDoubles / 5)
+            throw gr::Error(GR_ERR_INVALID, "gr_euler_residual_noalloc: " + std::to_string(N) +
+                                                " cells exceed its maximum of " + std::to_string(kScratchDoubles / 5) +
+                                                " (use gr_euler_residual)");
+        double *r = scratch_ptr(m->device);
+        // the kernel writes all 5 entries of every interior cell (nothing carries over between calls),
+        // and no new state (un = null)
+        if (rho->dtype == GR_F64) {
+            auto a = euler_args<double>(rho, rhou, rhoe);
+            for (auto &p : a.un) p = nullptr;
+            launch_flux<double>(m, m->corrected, a, 1.0, nullptr, nullptr, r, nullptr, 0, nullptr);
+        } else {
+            auto a = euler_args<float>(rho, rhou, rhoe);
+            for (auto &p : a.un) p = nullptr;
+            launch_flux<float>(m, m->corrected, a, 1.0, nullptr, nullptr, r, nullptr, 0, nullptr);
+        }
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipMemcpyAsync(resid, r, 5 * N * sizeof(double), hipMemcpyDeviceToHost, m->stream));
         GR_HIP(hipStreamSynchronize(m->stream));
     });
 }

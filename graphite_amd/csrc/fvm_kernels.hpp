@@ -392,10 +392,12 @@ __global__ __launch_bounds__(kBlock) void k_euler_flux(MeshView m, EulerArgs<S> 
             dts += eig * ds;
         }
         const S dt = (S)cfl / dts;
-        a.un[0][i] = ui[0] + (-fr) * dt;
+        if (a.un[0]) {   // null when only the residual is wanted
+            a.un[0][i] = ui[0] + (-fr) * dt;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.un[1 + k][i] = ui[1 + k] + (-fu[k]) * dt;
-        a.un[4][i] = ui[4] + (-fe) * dt;
+            for (int k = 0; k < 3; ++k) a.un[1 + k][i] = ui[1 + k] + (-fu[k]) * dt;
+            a.un[4][i] = ui[4] + (-fe) * dt;
+        }
         const double dfr = fr, dfe = fe, dfu[3] = {fu[0], fu[1], fu[2]};
         e_loc = ((dfr * dfr + ((dfu[0] * dfu[0] + dfu[1] * dfu[1]) + dfu[2] * dfu[2])) + dfe * dfe) * V;
         if (err) err[i] = (S)e_loc;

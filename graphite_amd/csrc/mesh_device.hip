@@ -240,6 +240,7 @@ void gr_mesh::ensure_device() {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         throw gr::Error(GR_ERR_DEVICE, "no HIP device available");
+    GR_HIP(hipGetDevice(&device));
     if (!stream) GR_HIP(hipStreamCreate(&stream));
     if (!host_scalar) GR_HIP(hipHostMalloc((void **)&host_scalar, sizeof(double)));
     const size_t NG = (size_t)h.n_cells + h.n_ghosts;

@@ -276,6 +276,16 @@ int gr_euler_step(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, dou
 /* Per-cell flux residual R (5 per interior cell: rho, rhou xyz, rhoe) of the current state,
  * without updating it (the sums fvmEulerSteadyOp forms, src/models/euler.cu:236-339). */
 int gr_euler_residual(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, double *resid);
+/* gr_euler_residual without any device allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It has no
+ * counterpart in the reference.  The residual is staged in a 32 MiB device array that the
+ * library holds on each GPU it runs on until the process ends, so meshes of at most
+ * GR_RESIDUAL_NOALLOC_MAX_CELLS interior cells are accepted (larger: GR_ERR_INVALID).  Calls
+ * that overlap on one GPU (several host threads) share that array and race; calls from one
+ * thread are ordered (each synchronises the library's stream before returning).  The current
+ * device must be the mesh's. */
+#define GR_RESIDUAL_NOALLOC_MAX_CELLS 838860
+int gr_euler_residual_noalloc(gr_mesh *m, gr_field *rho, gr_field *rhou, gr_field *rhoe, double *resid);
 /* Momentum reconstruction: 0 = the reference's transposed product (default), 1 = grad(u).d
  * per component. */
 int gr_euler_set_corrected_reconstruction(gr_mesh *m, int on);
