@@ -17,7 +17,7 @@ precision error at the 1e-3 point, the batched C-grid (64 flows per launch), the
 start against cold batched and cold single-flow solves, and the incompressible 64^3 cavity
 solved to its continuity criterion.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--only-headline] [--no-cpu] [--no-converge]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--only-headline] [--only-nsi] [--no-cpu] [--no-converge]
                   [--no-ogrid] [--no-batch] [--no-warm] [--no-nsi]
 """
 from __future__ import annotations
@@ -206,23 +206,71 @@ def warm_start_leg(gr, args):
     }
 
 
+def warm_forces(gr, args, m, net, norm, test, s_ref):
+    """Time to settled forces: cold (freestream) and warm-started batches of the 64 test flows
+    run args.force_horizon iterations with lift and drag sampled every 100; the converged value
+    of each flow is the cold run's mean over its last args.force_window iterations (the scheme
+    ends in a limit cycle); for each flow and start, the iterations until the trailing-window
+    mean of both coefficients is within 1e-4 (and 1e-3) of it and stays there."""
+    from graphite_amd import warmstart as ws
+    every, H, W = 100, args.force_horizon, args.force_window
+    res = {"horizon_iterations": H, "window_iterations": W, "sample_every": every,
+           "criterion": "trailing-window mean of C_L and C_D within tol of the cold run's mean over its last window, "
+                        "relative to max(|C|, 0.05 for lift, 0.005 for drag), from then on"}
+    cold = ws.make_batch(m, test)
+    warm = ws.make_batch(m, test)
+    t0 = time.perf_counter()
+    warm.warm_start(net, ws.wall_distance(m), norm)
+    t_pred = time.perf_counter() - t0
+    sc, secs_c, rc = ws.force_history(cold, test, "airfoil", s_ref, H, every, log=log, tag=" cold")
+    sw, secs_w, rw = ws.force_history(warm, test, "airfoil", s_ref, H, every, log=log, tag=" warm")
+    ref = sc[-(W // every):].mean(0)
+    per_iter_c, per_iter_w = secs_c / H, secs_w / H
+    for tol in (1e-3, 1e-4):
+        tc = ws.time_to_forces(sc, ref, every, W // 4, tol)
+        tw = ws.time_to_forces(sw, ref, every, W // 4, tol)
+        both = (tc > 0) & (tw > 0)
+        key = f"tol_{tol:.0e}"
+        res[key] = {"flows_met_cold": int((tc > 0).sum()), "flows_met_warm": int((tw > 0).sum()),
+                    "iterations_cold_p50_p99": [float(np.percentile(tc[tc > 0], q)) if (tc > 0).any() else None for q in (50, 99)],
+                    "iterations_warm_p50_p99": [float(np.percentile(tw[tw > 0], q)) if (tw > 0).any() else None for q in (50, 99)],
+                    "speedup_batch_wall": (float(tc[both].max() * per_iter_c) / float(tw[both].max() * per_iter_w + t_pred))
+                    if both.any() else None,
+                    "speedup_per_flow_p50": float(np.median(tc[both] / tw[both])) if both.any() else None}
+        log(f"bench: time to forces within {tol:.0e}: {res[key]}")
+    res["restarted"] = {"cold": int(rc.sum()), "warm": int(rw.sum())}
+    res["batch_seconds_per_iteration"] = {"cold": per_iter_c, "warm": per_iter_w}
+    return res
+
+
 def nsi_leg(gr, args):
-    """Lid-driven cavity 64^3 (262,144 hexes), Re 400, binary32 bulk with binary64 reductions."""
+    """Lid-driven cavity 64^3 (262,144 hexes), Re 400, lid velocity 1: steps and seconds to a
+    continuity ratio of 1e-4 (continuity residual over the largest of the first five steps), in
+    mixed precision (binary64 velocity, pressure and face fluxes; binary32 gradients, increments
+    and CG vectors) and in binary64."""
     log("bench: incompressible cavity leg")
     n = 64
     m = gr.Mesh.build(gr.Elements.box(n, n, n))
     m.renumber_rcm()
     m.compute_metrics()
-    ns = gr.Nsi(m, nu=1.0 / 400.0, dt=0.005, dtype=gr.GR_F32, cg_max=400, cg_tol=1e-4)
-    ns.set_wall("ymax", (1.0, 0.0, 0.0))
-    ns.step(5)
-    t0 = time.perf_counter()
-    c, its = ns.step(args.nsi_steps)
-    dt = time.perf_counter() - t0
-    return {"workload": "lid-driven cavity 64^3 hexes, Re 400, dt 0.005, Jacobi-PCG to 1e-4 relative",
-            "cells": m.n_cells, "steps": args.nsi_steps, "seconds": dt,
-            "steps_per_second": args.nsi_steps / dt, "cell_updates_per_second": m.n_cells * args.nsi_steps / dt,
-            "cg_iterations_mean": float(np.mean(its)), "continuity_last": float(c[-1]), "dtype": "f32 bulk, f64 dots"}
+    res = {"workload": "lid-driven cavity 64^3 hexes (262,144 cells), Re 400, lid velocity 1, dt 0.005, "
+                       "Jacobi-PCG to 1e-4 of |b| per step, to a continuity ratio of 1e-4",
+           "cells": m.n_cells,
+           "ratio_definition": "sum over cells of |net outflow of the predicted face fluxes| of a step over the "
+                               "largest of the first five steps"}
+    for name, dtype in (("mixed", gr.GR_F32), ("f64", gr.GR_F64)):
+        ns = gr.Nsi(m, nu=1.0 / 400.0, dt=0.005, dtype=dtype, cg_max=500, cg_tol=1e-4)
+        ns.set_wall("ymax", (1.0, 0.0, 0.0))
+        t0 = time.perf_counter()
+        steps, ratio = ns.iterate(args.nsi_steps, 1e-4)
+        dt = time.perf_counter() - t0
+        _, its = ns.history(steps)
+        res[name] = {"steps": steps, "seconds": dt, "ratio": ratio, "converged": bool(ratio < 1e-4),
+                     "steps_per_second": steps / dt, "cell_updates_per_second": m.n_cells * steps / dt,
+                     "cg_iterations_mean": float(np.mean(its)), "cg_iterations_total": int(np.sum(its)),
+                     "us_per_cg_iteration": dt / max(1, int(np.sum(its))) * 1e6}
+        log(f"bench: cavity {name}: {steps} steps to ratio {ratio:.2e} in {dt:.2f} s")
+    return res
 
 
 def cpu_model():
@@ -360,14 +408,19 @@ def main():
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-nsi", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the swept-wing headline alone (profiling)")
+    ap.add_argument("--only-nsi", action="store_true", help="the cavity leg alone (profiling)")
     ap.add_argument("--wing-iters", type=int, default=6000)
-    ap.add_argument("--nsi-steps", type=int, default=20000, help="step limit of the cavity solve to its criterion")
+    ap.add_argument("--nsi-steps", type=int, default=12000, help="step limit of the cavity solve to its criterion")
     ap.add_argument("--warm-flows", type=int, default=512)
     ap.add_argument("--warm-steps", type=int, default=8000)
     ap.add_argument("--warm-settle", type=int, default=20000,
                     help="further iterations of both warm and cold flows before their forces are compared")
     ap.add_argument("--warm-window", type=int, default=20000, help="iterations the compared forces are averaged over")
     ap.add_argument("--warm-sample", type=int, default=500, help="iterations between force samples in the window")
+    ap.add_argument("--warm-forces", action="store_true",
+                    help="also measure the iterations to settled lift/drag, warm vs cold (long)")
+    ap.add_argument("--force-horizon", type=int, default=140000)
+    ap.add_argument("--force-window", type=int, default=40000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -377,11 +430,14 @@ def main():
     import graphite_amd as gr
     if gr.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
+    if args.only_nsi:
+        print(json.dumps({"nsi_cavity": nsi_leg(gr, args)}), flush=True)
+        return
 
     log("bench: building the swept-wing mesh")
     e = gr.Elements.wing(scale=WING_SCALE)
     m = gr.Mesh.build(e)
-    m.renumber_rcm()
+    m.renumber_hilbert()
     m.compute_metrics()
     s = m.sizes()
     mach = wing_mach()

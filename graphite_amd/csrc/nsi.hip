@@ -1,21 +1,29 @@
 // Incompressible pressure-correction solver (no arithmetic in the reference: src/models/nsi.cu is
 // a stub that does not compile).  Collocated, cell-centred, constant density 1; per step:
 //   1. ghosts (no-slip walls with the region's velocity, zero normal pressure gradient) and the
-//      least-squares gradients of u and p (the Euler path's k_grad_lim);
+//      least-squares gradients of u and p (the Euler path's weighted LSQ, k_nsi_grad);
 //   2. predictor u* = u + dt/V (-sum F_out u_up + nu sum a (u_j - u_i)) - dt grad p, with u_up
 //      the second-order upwind value;
 //   3. face fluxes F* = A (avg(u*).n + dt (avg(grad p).n - (p1 - p0)/D)) from the lower- to the
 //      higher-numbered cell (momentum interpolation against pressure checkerboarding);
 //   4. Poisson sum_f a (p'_i - p'_j) = -(1/dt) sum F*_out by Jacobi-preconditioned conjugate
-//      gradients (binary64 dot products, fixed-order block reductions: deterministic);
+//      gradients (binary64 dot products, fixed-order block reductions: deterministic), four
+//      kernels per iteration (spmv with the direction update, alpha, vector update, beta),
+//      replayed from a HIP graph of 16 iterations until the device reports convergence;
 //   5. corrections F = F* - dt a (p'_1 - p'_0), u = u* - dt grad p', p += p'.
-// State, fluxes and CG vectors are S (binary64 or binary32); geometry, dot products and the
-// continuity residual are binary64.
+// Two precisions: all binary64 (S = T = double), or mixed (S = float, T = double): velocity,
+// pressure and face fluxes (the carriers, T) stay binary64 and the predictor stores the increment
+// du* = u* - u instead of u*, while gradients, increments and CG vectors (the bulk, S) are
+// binary32.  A binary32 velocity alone holds the continuity residual of the 64^3 cavity at about
+// 2e-4 of its start (the rounding of u, about 6e-8 |u|, enters every face flux); the increment
+// form carries only the rounding of small increments.  Geometry, dot products and the continuity
+// residual are binary64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fvm_kernels.hpp"
@@ -40,6 +48,12 @@ struct NsiFaces {
     const double *D, *a;           // (x1 - x0).n > 0 and A / D
 };
 
+// K blocks of cell values passed by value (no pointer table in memory).
+template <typename V, int K>
+struct NsiVals {
+    V *p[K];
+};
+
 // Ghost values: u_g = 2 u_wall - u_i, p_g = p_i (p may be null).
 template <typename S>
 __global__ void k_nsi_bc(MeshView m, S *ux, S *uy, S *uz, S *p, NsiWalls w) {
@@ -55,21 +69,53 @@ __global__ void k_nsi_bc(MeshView m, S *ux, S *uy, S *uz, S *p, NsiWalls w) {
     if (p) p[c] = p[i];
 }
 
-// Predictor for interior cell i.
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_nsi_predict(MeshView m, const S *__restrict__ ux, const S *__restrict__ uy,
-                                                        const S *__restrict__ uz, const S *const *gu, const S *const *gp,
-                                                        const S *__restrict__ F, NsiWalls w, double nu, double dt,
-                                                        NsiFaces fc, S *__restrict__ sx, S *__restrict__ sy,
-                                                        S *__restrict__ sz) {
+// Weighted least-squares gradients of NC components of values T into S (the gradient part of the
+// Euler path's k_grad_lim, src/fvmops.cu:51-111): differences u_j - u_i are formed in T before
+// the cast, so binary64 carriers keep their accuracy in binary32 gradients.
+template <typename S, typename T, int NC>
+__global__ __launch_bounds__(kBlock) void k_nsi_grad(MeshView m, NsiVals<T, NC> u, NsiVals<S, 3 * NC> g) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.N) return;
-    const S *u[3] = {ux, uy, uz};
+    const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+    T ui[NC];
+    S acc[NC][3];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        ui[c] = u.p[c][i];
+        acc[c][0] = acc[c][1] = acc[c][2] = S(0);
+    }
+    for (uint32_t q = m.cfs[i]; q < m.cfs[i + 1]; ++q) {
+        const uint32_t j = m.cc[q];
+        const S d[3] = {(S)(m.cx[j] - xi), (S)(m.cy[j] - yi), (S)(m.cz[j] - zi)};
+        const S w2 = S(1) / ((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const S du = (S)(u.p[c][j] - ui[c]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[c][k] += (d[k] * du) * w2;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        S out[3];
+        lsq_apply<S>(m, i, acc[c], out);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g.p[3 * c + k][i] = out[k];
+    }
+}
+
+// Predictor for interior cell i: u* (all binary64) or the increment du* = u* - u (mixed).
+template <typename S, typename T>
+__global__ __launch_bounds__(kBlock) void k_nsi_predict(MeshView m, NsiVals<T, 3> u, const S *const *gu,
+                                                        const S *const *gp, const T *__restrict__ F, NsiWalls w,
+                                                        double nu, double dt, NsiFaces fc, NsiVals<S, 3> o) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.N) return;
     S conv[3] = {S(0), S(0), S(0)}, diff[3] = {S(0), S(0), S(0)};
-    const S ui[3] = {ux[i], uy[i], uz[i]};
+    const T ui[3] = {u.p[0][i], u.p[1][i], u.p[2][i]};
     for (uint32_t q = m.cfs[i]; q < m.cfs[i + 1]; ++q) {
         const uint32_t f = m.cf[q], j = m.cc[q];
-        const S out = i < j ? F[f] : -F[f];
+        const S out = (S)(i < j ? F[f] : -F[f]);
         S uf[3];
         if (j >= m.N) {
             const double *uw = nsi_wall(w, j - m.N);
@@ -80,62 +126,70 @@ __global__ __launch_bounds__(kBlock) void k_nsi_predict(MeshView m, const S *__r
             const S dd[3] = {(S)(m.fcx[f] - m.cx[up]), (S)(m.fcy[f] - m.cy[up]), (S)(m.fcz[f] - m.cz[up])};
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                uf[k] = u[k][up] + ((gu[3 * k][up] * dd[0] + gu[3 * k + 1][up] * dd[1]) + gu[3 * k + 2][up] * dd[2]);
+                uf[k] = (S)u.p[k][up] + ((gu[3 * k][up] * dd[0] + gu[3 * k + 1][up] * dd[1]) + gu[3 * k + 2][up] * dd[2]);
         }
         const S a = (S)fc.a[f];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             conv[k] += out * uf[k];
-            diff[k] += a * (u[k][j] - ui[k]);
+            diff[k] += a * (S)(u.p[k][j] - ui[k]);
         }
     }
     const S s = (S)(dt / m.cv[i]), sdt = (S)dt, snu = (S)nu;
-    S *o[3] = {sx, sy, sz};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) o[k][i] = ui[k] + s * (snu * diff[k] - conv[k]) - sdt * gp[k][i];
+    for (int k = 0; k < 3; ++k) {
+        if constexpr (std::is_same<S, T>::value) o.p[k][i] = ui[k] + s * (snu * diff[k] - conv[k]) - sdt * gp[k][i];
+        else o.p[k][i] = s * (snu * diff[k] - conv[k]) - sdt * gp[k][i];
+    }
 }
 
-// Predicted face fluxes, face-parallel.
-template <typename S>
-__global__ void k_nsi_flux(MeshView m, NsiFaces fc, const S *__restrict__ sx, const S *__restrict__ sy,
-                           const S *__restrict__ sz, const S *const *gp, const S *__restrict__ p, NsiWalls w,
-                           double dt, S *__restrict__ Fs) {
+// Component k of u* at interior cell c: stored (all binary64) or u + du* (mixed), in T.
+template <typename S, typename T>
+__device__ inline T nsi_ustar(const NsiVals<T, 3> &u, const NsiVals<S, 3> &us, int k, uint32_t c) {
+    if constexpr (std::is_same<S, T>::value) return us.p[k][c];
+    else return u.p[k][c] + (T)us.p[k][c];
+}
+
+// Predicted face fluxes, face-parallel, in T.
+template <typename S, typename T>
+__global__ void k_nsi_flux(MeshView m, NsiFaces fc, NsiVals<T, 3> u, NsiVals<S, 3> us, const S *const *gp,
+                           const T *__restrict__ p, NsiWalls w, double dt, T *__restrict__ Fs) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const uint32_t c0 = fc.c0[f], c1 = fc.c1[f];
-    const S n[3] = {(S)fc.nx[f], (S)fc.ny[f], (S)fc.nz[f]};
-    const S A = (S)m.fa[f];
+    const T n[3] = {(T)fc.nx[f], (T)fc.ny[f], (T)fc.nz[f]};
+    const T A = (T)m.fa[f];
     if (c1 >= m.N) {
         const double *uw = nsi_wall(w, c1 - m.N);
-        Fs[f] = A * (((S)uw[0] * n[0] + (S)uw[1] * n[1]) + (S)uw[2] * n[2]);
+        Fs[f] = A * (((T)uw[0] * n[0] + (T)uw[1] * n[1]) + (T)uw[2] * n[2]);
         return;
     }
-    const S *s[3] = {sx, sy, sz};
-    S un = S(0), gn = S(0);
+    T un = T(0), gn = T(0);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        un += S(0.5) * (s[k][c0] + s[k][c1]) * n[k];
-        gn += S(0.5) * (gp[k][c0] + gp[k][c1]) * n[k];
+        un += T(0.5) * (nsi_ustar<S, T>(u, us, k, c0) + nsi_ustar<S, T>(u, us, k, c1)) * n[k];
+        gn += T(0.5) * ((T)gp[k][c0] + (T)gp[k][c1]) * n[k];
     }
-    Fs[f] = A * (un + (S)dt * (gn - (p[c1] - p[c0]) / (S)fc.D[f]));
+    Fs[f] = A * (un + (T)dt * (gn - (p[c1] - p[c0]) / (T)fc.D[f]));
 }
 
-// Net outflow of the predicted fluxes: b_i = -(1/dt) sum F*_out; block partials of |sum| and of b.
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_nsi_div(MeshView m, const S *__restrict__ Fs, double dt, S *__restrict__ b,
+// Net outflow of the predicted fluxes: b_i = -(1/dt) sum F*_out (in T, stored as S); block
+// partials of |sum| (the continuity residual) and of b.
+template <typename S, typename T>
+__global__ __launch_bounds__(kBlock) void k_nsi_div(MeshView m, const T *__restrict__ Fs, double dt, S *__restrict__ b,
                                                     double *__restrict__ part_abs, double *__restrict__ part_sum) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double ab = 0, sm = 0;
     if (i < m.N) {
-        S s = S(0);
+        T s = T(0);
         for (uint32_t q = m.cfs[i]; q < m.cfs[i + 1]; ++q) {
             const uint32_t f = m.cf[q];
             s += i < m.cc[q] ? Fs[f] : -Fs[f];
         }
-        const S bi = -s / (S)dt;
-        b[i] = bi;
+        const T bi = -s / (T)dt;
+        b[i] = (S)bi;
         ab = fabs((double)s);
-        sm = (double)bi;
+        sm = (double)(S)bi;
     }
     const double t1 = block_sum(ab);
     __syncthreads();
@@ -162,11 +216,12 @@ __global__ __launch_bounds__(kBlock) void k_nsi_sum(const double *__restrict__ p
     if (threadIdx.x == 0) *out = t;
 }
 
-// r = b - mean(b), x = 0, z = r / diag, d = z; partials of r.z and r.r.
+// r = b - mean(b), x = 0, z = r / diag, d = z (the first iteration's direction: beta = 0 makes
+// the spmv form z + 0 d); partials of r.z and r.r.
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_cg_init(uint32_t N, const S *__restrict__ b, const double *__restrict__ diag,
                                                     CgScalars *cg, S *__restrict__ x, S *__restrict__ r,
-                                                    S *__restrict__ d, double *__restrict__ p_rz,
+                                                    S *__restrict__ z, S *__restrict__ d, double *__restrict__ p_rz,
                                                     double *__restrict__ p_rr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0, rr = 0;
@@ -175,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(uint32_t N, const S *__restr
         const S zi = diag[i] > 0 ? (S)((double)ri / diag[i]) : S(0);
         x[i] = S(0);
         r[i] = ri;
+        z[i] = zi;
         d[i] = zi;
         rz = (double)ri * (double)zi;
         rr = (double)ri * (double)ri;
@@ -188,25 +244,46 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(uint32_t N, const S *__restr
     }
 }
 
-// q = A d over interior faces; partials of d.q.
+// One CG iteration, first kernel: d = z + beta d_old (beta = 0 at the first iteration) for the
+// cell and, on the fly, for its neighbours; q = A d; partials of d.q.  d is double-buffered
+// (read d_old, write d_new) so that no block reads a direction another block has already
+// overwritten.  (Reducing the partials in the last block to finish, inside this kernel and the
+// update, needs a device-scope fence in every block, which writes back the XCD's L2: 66 us per
+// iteration on the 64^3 cavity, against 25 us with separate one-block kernels.)
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(MeshView m, const double *__restrict__ a, const S *__restrict__ d,
-                                                    S *__restrict__ q, double *__restrict__ p_dq, const CgScalars *cg) {
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(MeshView m, const double *__restrict__ a, const S *__restrict__ z,
+                                                    const S *__restrict__ d_old, S *__restrict__ d_new,
+                                                    S *__restrict__ q, double *__restrict__ p_dq, CgScalars *cg) {
     if (cg->done) return;
+    const S beta = (S)cg->beta;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double dq = 0;
     if (i < m.N) {
-        const S di = d[i];
+        const S di = z[i] + beta * d_old[i];
         S s = S(0);
         for (uint32_t k = m.cfs[i]; k < m.cfs[i + 1]; ++k) {
             const uint32_t j = m.cc[k];
-            if (j < m.N) s += (S)a[m.cf[k]] * (di - d[j]);
+            if (j < m.N) s += (S)a[m.cf[k]] * (di - (z[j] + beta * d_old[j]));
         }
+        d_new[i] = di;
         q[i] = s;
         dq = (double)di * (double)s;
     }
     const double t = block_sum(dq);
     if (threadIdx.x == 0) p_dq[blockIdx.x] = t;
+}
+
+// alpha = r.z / d.q from the spmv's block partials (one block; fixed-order sum).
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(const double *__restrict__ p_dq, int nb, CgScalars *cg) {
+    if (cg->done) return;
+    double s = 0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) s += p_dq[k];
+    const double dq = block_sum(s);
+    if (threadIdx.x == 0) {
+        cg->dq = dq;
+        if (dq == 0.0) cg->done = 1;
+        else cg->alpha = cg->rz / dq;
+    }
 }
 
 // x += alpha d, r -= alpha q, z = r / diag; partials of r.z and r.r.
@@ -237,28 +314,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(uint32_t N, const double *
     }
 }
 
-// d = z + beta d
-template <typename S>
-__global__ void k_cg_dir(uint32_t N, const CgScalars *cg, const S *__restrict__ z, S *__restrict__ d) {
-    if (cg->done) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) d[i] = z[i] + (S)cg->beta * d[i];
-}
-
-// alpha = r.z / d.q from the spmv's block partials (one block; fixed-order sum).
-__global__ __launch_bounds__(kBlock) void k_cg_alpha(const double *__restrict__ p_dq, int nb, CgScalars *cg) {
-    if (cg->done) return;
-    double s = 0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) s += p_dq[k];
-    const double dq = block_sum(s);
-    if (threadIdx.x == 0) {
-        cg->dq = dq;
-        if (dq == 0.0) cg->done = 1;
-        else cg->alpha = cg->rz / dq;
-    }
-}
-
-// New r.z and r.r from the update's partials; iteration count, convergence test and beta.
+// New r.z and r.r from the update's partials; iteration count, convergence test and beta (one
+// block).
 __global__ __launch_bounds__(kBlock) void k_cg_beta(const double *__restrict__ p_rz, const double *__restrict__ p_rr,
                                                     int nb, CgScalars *cg, double tol, int cg_max) {
     if (cg->done) return;
@@ -296,6 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(const double *__restrict__ 
         cg->rr = rr;
         cg->bnorm = sqrt(rr);
         cg->iters = 0;
+        cg->beta = 0.0;
         cg->done = (cg->bnorm == 0.0 || cg_max == 0) ? 1 : 0;
     }
 }
@@ -316,26 +374,27 @@ __global__ __launch_bounds__(kBlock) void k_nsi_partial_sum(uint32_t N, const S 
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-template <typename S>
-__global__ void k_nsi_correct_faces(MeshView m, NsiFaces fc, const S *__restrict__ Fs, const S *__restrict__ pp,
-                                    double dt, S *__restrict__ F) {
+template <typename S, typename T>
+__global__ void k_nsi_correct_faces(MeshView m, NsiFaces fc, const T *__restrict__ Fs, const S *__restrict__ pp,
+                                    double dt, T *__restrict__ F) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const uint32_t c0 = fc.c0[f], c1 = fc.c1[f];
-    F[f] = c1 >= m.N ? Fs[f] : Fs[f] - (S)dt * (S)fc.a[f] * (pp[c1] - pp[c0]);
+    F[f] = c1 >= m.N ? Fs[f] : Fs[f] - (T)dt * (T)fc.a[f] * (T)(pp[c1] - pp[c0]);
 }
 
-template <typename S>
-__global__ void k_nsi_correct_cells(uint32_t N, const S *__restrict__ sx, const S *__restrict__ sy,
-                                    const S *__restrict__ sz, const S *const *gpp, const S *__restrict__ pp, double dt,
-                                    S *__restrict__ ux, S *__restrict__ uy, S *__restrict__ uz, S *__restrict__ p) {
+template <typename S, typename T>
+__global__ void k_nsi_correct_cells(uint32_t N, NsiVals<T, 3> u, NsiVals<S, 3> us, const S *const *gpp,
+                                    const S *__restrict__ pp, double dt, T *__restrict__ p) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const S sdt = (S)dt;
-    ux[i] = sx[i] - sdt * gpp[0][i];
-    uy[i] = sy[i] - sdt * gpp[1][i];
-    uz[i] = sz[i] - sdt * gpp[2][i];
-    p[i] += pp[i];
+    const T sdt = (T)dt;
+    T un[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) un[k] = nsi_ustar<S, T>(u, us, k, i) - sdt * (T)gpp[k][i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) u.p[k][i] = un[k];
+    p[i] += (T)pp[i];
 }
 
 // Face tables: lower/higher cell from the CSR (the lower cell lists the face first).
@@ -370,31 +429,50 @@ using gr::guarded;
 using gr::require;
 
 struct gr_nsi {
+    static constexpr int kCgChunk = 16;   // CG iterations per graph replay (even: d buffers return)
     gr_mesh *mesh = nullptr;
     gr_nsi_config cfg{};
     size_t cs = 0;                        // padded cell stride
     gr::NsiWalls walls{};
-    gr::DevArray<char> st;                // u (3 blocks), p, u* (3), p', and gradient blocks
-    gr::DevArray<char> flux;              // F, F*
-    gr::DevArray<char> cgv;               // b, x (= p'), r, z, d, q
+    gr::DevArray<char> st;                // bulk S: u* or du* (3 blocks), p', grad u (9), grad p (3), grad p' (3)
+    gr::DevArray<double> carrier;         // T = binary64: u (3 blocks), p
+    gr::DevArray<double> flux;            // T: F, F*
+    gr::DevArray<char> cgv;               // b, d (second buffer), r, z, d, q; x = p' (state block 7)
     gr::DevArray<uint32_t> c0, c1;
     gr::DevArray<double> fgeo;            // nx, ny, nz, D, a (face blocks)
     gr::DevArray<double> diag, part, part2, cont;
     gr::DevArray<gr::CgScalars> cg;
-    gr::CgScalars *cg_host = nullptr;     // pinned
+    gr::CgScalars *cg_host = nullptr;     // pinned: [0], [1] CG polls, [2] end of step
+    double *cont_host = nullptr;          // pinned
     gr::DevArray<void *> ptrs;            // gradient pointer tables
-    int hist_n = 0;
+    hipGraphExec_t g_pre = nullptr, g_cg = nullptr, g_post = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<double> history;          // continuity residual per step
     std::vector<int> cg_iters;
+    size_t history_start = 0;             // first step since creation or the last set_state
 
     ~gr_nsi() {
+        drop_graphs();
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
         if (cg_host) (void)hipHostFree(cg_host);
+        if (cont_host) (void)hipHostFree(cont_host);
+    }
+    void drop_graphs() {
+        for (hipGraphExec_t *g : {&g_pre, &g_cg, &g_post})
+            if (*g) {
+                (void)hipGraphExecDestroy(*g);
+                *g = nullptr;
+            }
     }
     size_t es() const { return (size_t)cfg.dtype; }
+    static constexpr int kBulkBlocks = 19;
     template <typename S> S *blk(int b) const { return (S *)st.p + (size_t)b * cs; }
-    // blocks: 0-2 u, 3 p, 4-6 u*, 7 p', 8-16 grad u, 17-19 grad p, 20-22 grad p', 23-25 unused limiter
-    template <typename S> S *F() const { return (S *)flux.p; }
-    template <typename S> S *Fs() const { return (S *)flux.p + gr::pad64(mesh->h.n_faces); }
+    // bulk blocks: 0-2 u* (binary64) or du* = u* - u (mixed), 3 p', 4-12 grad u, 13-15 grad p,
+    // 16-18 grad p'
+    double *cu(int k) const { return carrier.p + (size_t)k * cs; }   // carriers: 0-2 u, 3 p
+    double *F() const { return flux.p; }
+    double *Fs() const { return flux.p + gr::pad64(mesh->h.n_faces); }
     template <typename S> S *cgb(int k) const { return (S *)cgv.p + (size_t)k * cs; }
     gr::NsiFaces faces() const {
         const size_t fs = gr::pad64(mesh->h.n_faces);
@@ -402,170 +480,198 @@ struct gr_nsi {
     }
     unsigned nb() const { return gr::blocks_for(mesh->h.n_cells); }
 
-    template <typename S> void gradients(S *const *vals, int nc, S *const *grads, S *lim_scratch) {
-        gr_mesh *m = mesh;
-        if (nc == 3) {
-            gr::GradLimArgs<S, 3> a{};
-            for (int c = 0; c < 3; ++c) {
-                a.u[c] = vals[c];
-                a.l[c] = lim_scratch;
-                for (int k = 0; k < 3; ++k) a.g[3 * c + k] = grads[3 * c + k];
-            }
-            hipLaunchKernelGGL((gr::k_grad_lim<S, 3, true, false>), dim3(nb()), dim3(gr::kBlock), 0, m->stream,
-                               m->view(), a, 0, (const gr::SolverCtl *)nullptr);
-        } else {
-            gr::GradLimArgs<S, 1> a{};
-            a.u[0] = vals[0];
-            a.l[0] = lim_scratch;
-            for (int k = 0; k < 3; ++k) a.g[k] = grads[k];
-            hipLaunchKernelGGL((gr::k_grad_lim<S, 1, true, false>), dim3(nb()), dim3(gr::kBlock), 0, m->stream,
-                               m->view(), a, 0, (const gr::SolverCtl *)nullptr);
-        }
+    template <typename S, typename T, int NC> void gradient(T *const *vals, S *const *grads) {
+        gr::NsiVals<T, NC> u{};
+        gr::NsiVals<S, 3 * NC> g{};
+        for (int c = 0; c < NC; ++c) u.p[c] = vals[c];
+        for (int k = 0; k < 3 * NC; ++k) g.p[k] = grads[k];
+        hipLaunchKernelGGL((gr::k_nsi_grad<S, T, NC>), dim3(nb()), dim3(gr::kBlock), 0, mesh->stream, mesh->view(), u, g);
     }
+    gr::NsiVals<double, 3> uvals() const { return gr::NsiVals<double, 3>{{cu(0), cu(1), cu(2)}}; }
+    template <typename S> gr::NsiVals<S, 3> usvals() const { return gr::NsiVals<S, 3>{{blk<S>(0), blk<S>(1), blk<S>(2)}}; }
 
-    // CG on A x = b (x = block 7 of the state, p'); returns nothing, iterations in cg->iters.
-    template <typename S> void solve_poisson() {
-        gr_mesh *m = mesh;
-        const uint32_t N = m->h.n_cells;
-        const unsigned B = nb();
-        S *b = cgb<S>(0), *r = cgb<S>(2), *z = cgb<S>(3), *d = cgb<S>(4), *q = cgb<S>(5);
-        S *x = blk<S>(7);
-        gr::CgScalars *c = cg.p;
-        double *pa = part.p, *pb = part2.p;
-        // mean of b, then init
-        hipLaunchKernelGGL(gr::k_nsi_partial_sum<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)b, pa);
-        hipLaunchKernelGGL(gr::k_nsi_sum, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)pa, (int)B,
-                           &c->bsum, 0, (const gr::CgScalars *)c);
-        hipLaunchKernelGGL(gr::k_cg_init<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)b,
-                           (const double *)diag.p, c, x, r, d, pa, pb);
-        hipLaunchKernelGGL(gr::k_cg_start, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)pa,
-                           (const double *)pb, (int)B, c, cfg.cg_max);
-        const double *a = fgeo.p + 4 * gr::pad64(m->h.n_faces);
-        for (int it = 0; it < cfg.cg_max; ++it) {
-            hipLaunchKernelGGL(gr::k_cg_spmv<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, m->view(), a, (const S *)d,
-                               q, pa, (const gr::CgScalars *)c);
-            hipLaunchKernelGGL(gr::k_cg_alpha, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)pa, (int)B, c);
-            hipLaunchKernelGGL(gr::k_cg_update<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const double *)diag.p,
-                               c, x, r, z, (const S *)d, (const S *)q, pa, pb);
-            hipLaunchKernelGGL(gr::k_cg_beta, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)pa,
-                               (const double *)pb, (int)B, c, cfg.cg_tol, cfg.cg_max);
-            hipLaunchKernelGGL(gr::k_cg_dir<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const gr::CgScalars *)c,
-                               (const S *)z, d);
-            if ((it & 15) == 15) {
-                // stop launching once the device reports convergence
-                GR_HIP(hipMemcpyAsync(cg_host, c, sizeof *cg_host, hipMemcpyDeviceToHost, m->stream));
-                GR_HIP(hipStreamSynchronize(m->stream));
-                if (cg_host->done) break;
-            }
-        }
-        // zero mean of x
-        hipLaunchKernelGGL(gr::k_nsi_partial_sum<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)x, pa);
-        hipLaunchKernelGGL(gr::k_nsi_sum, dim3(1), dim3(gr::kBlock), 0, m->stream, pa, (int)B, &c->xsum, 0,
-                           (const gr::CgScalars *)c);
-        hipLaunchKernelGGL(gr::k_nsi_shift<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const gr::CgScalars *)c, x);
-    }
-
-    template <typename S> void step(double *cont_out, int *its_out) {
+    // Steps 1-3 and the CG set-up: ghosts, gradients, predictor, predicted fluxes, their
+    // divergence (b and the continuity residual), r = b - mean(b), z, d, r.z, |b|.
+    template <typename S> void enqueue_pre() {
+        using T = double;
         gr_mesh *m = mesh;
         const gr::MeshView v = m->view();
         const uint32_t N = m->h.n_cells, G = m->h.n_ghosts, F = m->h.n_faces;
         const unsigned B = nb(), BG = gr::blocks_for(std::max<uint32_t>(G, 1)), BF = gr::blocks_for(F);
-        S *ux = blk<S>(0), *uy = blk<S>(1), *uz = blk<S>(2), *p = blk<S>(3);
-        S *sx = blk<S>(4), *sy = blk<S>(5), *sz = blk<S>(6), *pp = blk<S>(7);
-        S *lim = blk<S>(23);
+        T *p = cu(3);
         S **tab = (S **)ptrs.p;   // [0..8] grad u, [9..11] grad p, [12..14] grad p'
-        S *gu[9], *gp[3], *gpp[3];
-        for (int k = 0; k < 9; ++k) gu[k] = blk<S>(8 + k);
-        for (int k = 0; k < 3; ++k) {
-            gp[k] = blk<S>(17 + k);
-            gpp[k] = blk<S>(20 + k);
-        }
-        // 1. ghosts and gradients
-        if (G) hipLaunchKernelGGL(gr::k_nsi_bc<S>, dim3(BG), dim3(gr::kBlock), 0, m->stream, v, ux, uy, uz, p, walls);
-        S *uv[3] = {ux, uy, uz};
-        gradients<S>(uv, 3, gu, lim);
-        gradients<S>(&p, 1, gp, lim);
-        // 2. predictor
-        hipLaunchKernelGGL(gr::k_nsi_predict<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, v, (const S *)ux,
-                           (const S *)uy, (const S *)uz, (const S *const *)tab, (const S *const *)(tab + 9),
-                           (const S *)this->F<S>(), walls, cfg.nu, cfg.dt, faces(), sx, sy, sz);
-        if (G)
-            hipLaunchKernelGGL(gr::k_nsi_bc<S>, dim3(BG), dim3(gr::kBlock), 0, m->stream, v, sx, sy, sz, (S *)nullptr,
-                               walls);
-        // 3. face fluxes and divergence
-        hipLaunchKernelGGL(gr::k_nsi_flux<S>, dim3(BF), dim3(gr::kBlock), 0, m->stream, v, faces(), (const S *)sx,
-                           (const S *)sy, (const S *)sz, (const S *const *)(tab + 9), (const S *)p, walls, cfg.dt,
-                           Fs<S>());
-        hipLaunchKernelGGL(gr::k_nsi_div<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, v, (const S *)Fs<S>(), cfg.dt,
+        S *gu[9], *gp[3];
+        for (int k = 0; k < 9; ++k) gu[k] = blk<S>(4 + k);
+        for (int k = 0; k < 3; ++k) gp[k] = blk<S>(13 + k);
+        if (G) hipLaunchKernelGGL(gr::k_nsi_bc<T>, dim3(BG), dim3(gr::kBlock), 0, m->stream, v, cu(0), cu(1), cu(2), p, walls);
+        T *uv[3] = {cu(0), cu(1), cu(2)};
+        gradient<S, T, 3>(uv, gu);
+        gradient<S, T, 1>(&p, gp);
+        hipLaunchKernelGGL((gr::k_nsi_predict<S, T>), dim3(B), dim3(gr::kBlock), 0, m->stream, v, uvals(),
+                           (const S *const *)tab, (const S *const *)(tab + 9), (const T *)this->F(), walls, cfg.nu,
+                           cfg.dt, faces(), usvals<S>());
+        hipLaunchKernelGGL((gr::k_nsi_flux<S, T>), dim3(BF), dim3(gr::kBlock), 0, m->stream, v, faces(), uvals(),
+                           usvals<S>(), (const S *const *)(tab + 9), (const T *)p, walls, cfg.dt, Fs());
+        hipLaunchKernelGGL((gr::k_nsi_div<S, T>), dim3(B), dim3(gr::kBlock), 0, m->stream, v, (const T *)Fs(), cfg.dt,
                            cgb<S>(0), part.p, part2.p);
         hipLaunchKernelGGL(gr::k_nsi_sum, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)part.p, (int)B,
                            cont.p, 0, (const gr::CgScalars *)cg.p);
-        // 4. pressure correction
-        solve_poisson<S>();
+        // CG set-up: mean of b, r, z, d = z, then r.z and |b|
+        S *b = cgb<S>(0), *r = cgb<S>(2), *z = cgb<S>(3), *d = cgb<S>(4);
+        S *x = blk<S>(3);
+        gr::CgScalars *c = cg.p;
+        double *pa = part.p, *pb = part2.p;
+        hipLaunchKernelGGL(gr::k_nsi_partial_sum<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)b, pa);
+        hipLaunchKernelGGL(gr::k_nsi_sum, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)pa, (int)B,
+                           &c->bsum, 0, (const gr::CgScalars *)c);
+        hipLaunchKernelGGL(gr::k_cg_init<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)b,
+                           (const double *)diag.p, c, x, r, z, d, pa, pb);
+        hipLaunchKernelGGL(gr::k_cg_start, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)pa,
+                           (const double *)pb, (int)B, c, cfg.cg_max);
+    }
+
+    // kCgChunk CG iterations; every kernel returns at once after convergence.  Iteration k reads
+    // direction buffer k % 2 (block 4, then block 1) and writes the other.
+    template <typename S> void enqueue_cg() {
+        gr_mesh *m = mesh;
+        const uint32_t N = m->h.n_cells;
+        const unsigned B = nb();
+        S *r = cgb<S>(2), *z = cgb<S>(3), *q = cgb<S>(5);
+        S *dbuf[2] = {cgb<S>(4), cgb<S>(1)};
+        S *x = blk<S>(3);
+        gr::CgScalars *c = cg.p;
+        const double *a = fgeo.p + 4 * gr::pad64(m->h.n_faces);
+        for (int it = 0; it < kCgChunk; ++it) {
+            S *d_old = dbuf[it & 1], *d_new = dbuf[(it + 1) & 1];
+            hipLaunchKernelGGL(gr::k_cg_spmv<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, m->view(), a, (const S *)z,
+                               (const S *)d_old, d_new, q, part.p, c);
+            hipLaunchKernelGGL(gr::k_cg_alpha, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)part.p, (int)B, c);
+            hipLaunchKernelGGL(gr::k_cg_update<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const double *)diag.p,
+                               c, x, r, z, (const S *)d_new, (const S *)q, part.p, part2.p);
+            hipLaunchKernelGGL(gr::k_cg_beta, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)part.p,
+                               (const double *)part2.p, (int)B, c, cfg.cg_tol, cfg.cg_max);
+        }
+    }
+
+    // Zero mean of p', its ghosts and gradient, and step 5 (corrections of F, u, p; ghosts).
+    template <typename S> void enqueue_post() {
+        using T = double;
+        gr_mesh *m = mesh;
+        const gr::MeshView v = m->view();
+        const uint32_t N = m->h.n_cells, G = m->h.n_ghosts, F = m->h.n_faces;
+        const unsigned B = nb(), BG = gr::blocks_for(std::max<uint32_t>(G, 1)), BF = gr::blocks_for(F);
+        S *pp = blk<S>(3);
+        S **tab = (S **)ptrs.p;
+        S *gpp[3];
+        for (int k = 0; k < 3; ++k) gpp[k] = blk<S>(16 + k);
+        gr::CgScalars *c = cg.p;
+        double *pa = part.p;
+        hipLaunchKernelGGL(gr::k_nsi_partial_sum<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)pp, pa);
+        hipLaunchKernelGGL(gr::k_nsi_sum, dim3(1), dim3(gr::kBlock), 0, m->stream, pa, (int)B, &c->xsum, 0,
+                           (const gr::CgScalars *)c);
+        hipLaunchKernelGGL(gr::k_nsi_shift<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const gr::CgScalars *)c, pp);
         if (G)
             hipLaunchKernelGGL(gr::k_nsi_bc<S>, dim3(BG), dim3(gr::kBlock), 0, m->stream, v, (S *)nullptr,
                                (S *)nullptr, (S *)nullptr, pp, walls);
-        gradients<S>(&pp, 1, gpp, lim);
-        // 5. corrections
-        hipLaunchKernelGGL(gr::k_nsi_correct_faces<S>, dim3(BF), dim3(gr::kBlock), 0, m->stream, v, faces(),
-                           (const S *)Fs<S>(), (const S *)pp, cfg.dt, this->F<S>());
-        hipLaunchKernelGGL(gr::k_nsi_correct_cells<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)sx,
-                           (const S *)sy, (const S *)sz, (const S *const *)(tab + 12), (const S *)pp, cfg.dt, ux, uy,
-                           uz, p);
-        if (G) hipLaunchKernelGGL(gr::k_nsi_bc<S>, dim3(BG), dim3(gr::kBlock), 0, m->stream, v, ux, uy, uz, p, walls);
-        GR_HIP(hipGetLastError());
-        double c = 0;
-        gr::CgScalars h;
-        GR_HIP(hipMemcpyAsync(&c, cont.p, sizeof c, hipMemcpyDeviceToHost, m->stream));
-        GR_HIP(hipMemcpyAsync(&h, cg.p, sizeof h, hipMemcpyDeviceToHost, m->stream));
+        gradient<S, S, 1>(&pp, gpp);
+        hipLaunchKernelGGL((gr::k_nsi_correct_faces<S, T>), dim3(BF), dim3(gr::kBlock), 0, m->stream, v, faces(),
+                           (const T *)Fs(), (const S *)pp, cfg.dt, this->F());
+        hipLaunchKernelGGL((gr::k_nsi_correct_cells<S, T>), dim3(B), dim3(gr::kBlock), 0, m->stream, N, uvals(),
+                           usvals<S>(), (const S *const *)(tab + 12), (const S *)pp, cfg.dt, cu(3));
+        if (G)
+            hipLaunchKernelGGL(gr::k_nsi_bc<T>, dim3(BG), dim3(gr::kBlock), 0, m->stream, v, cu(0), cu(1), cu(2), cu(3),
+                               walls);
+    }
+
+    template <typename Fn> hipGraphExec_t capture(Fn &&fn) {
+        hipGraph_t g = nullptr;
+        GR_HIP(hipStreamBeginCapture(mesh->stream, hipStreamCaptureModeRelaxed));
+        try {
+            fn();
+        } catch (...) {
+            (void)hipStreamEndCapture(mesh->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        GR_HIP(hipStreamEndCapture(mesh->stream, &g));
+        hipGraphExec_t e = nullptr;
+        const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        GR_HIP(r);
+        return e;
+    }
+    template <typename S> void ensure_graphs() {
+        if (!g_pre) g_pre = capture([&] { enqueue_pre<S>(); });
+        if (!g_cg) g_cg = capture([&] { enqueue_cg<S>(); });
+        if (!g_post) g_post = capture([&] { enqueue_post<S>(); });
+    }
+
+    // One time step: the pre graph, CG graphs until the device reports convergence (two in
+    // flight: the host waits for the previous replay's copy of the CG scalars while the next one
+    // runs; a replay after convergence does no work), the post graph.
+    template <typename S> void step(double *cont_out, int *its_out) {
+        gr_mesh *m = mesh;
+        ensure_graphs<S>();
+        GR_HIP(hipGraphLaunch(g_pre, m->stream));
+        const int chunks = (cfg.cg_max + kCgChunk - 1) / kCgChunk;
+        for (int L = 0; L < chunks; ++L) {
+            GR_HIP(hipGraphLaunch(g_cg, m->stream));
+            GR_HIP(hipMemcpyAsync(&cg_host[L & 1], cg.p, sizeof *cg_host, hipMemcpyDeviceToHost, m->stream));
+            GR_HIP(hipEventRecord(ev[L & 1], m->stream));
+            if (L >= 1) {
+                GR_HIP(hipEventSynchronize(ev[(L - 1) & 1]));
+                if (cg_host[(L - 1) & 1].done) break;
+            }
+        }
+        GR_HIP(hipGraphLaunch(g_post, m->stream));
+        GR_HIP(hipMemcpyAsync(cont_host, cont.p, sizeof *cont_host, hipMemcpyDeviceToHost, m->stream));
+        GR_HIP(hipMemcpyAsync(&cg_host[2], cg.p, sizeof *cg_host, hipMemcpyDeviceToHost, m->stream));
         GR_HIP(hipStreamSynchronize(m->stream));
-        if (cont_out) *cont_out = c;
-        if (its_out) *its_out = h.iters;
+        if (cont_out) *cont_out = *cont_host;
+        if (its_out) *its_out = cg_host[2].iters;
     }
 };
 
 
 namespace {
 
-template <typename S>
 void upload_state(gr_nsi *s, const double *u, const double *p, const double *F) {
     gr_mesh *m = s->mesh;
     const size_t NG = (size_t)m->h.n_cells + m->h.n_ghosts, NF = m->h.n_faces;
-    std::vector<S> blk(4 * s->cs, S(0)), fl(gr::pad64(NF), S(0));
+    std::vector<double> blk(4 * s->cs, 0.0), fl(gr::pad64(NF), 0.0);
     for (size_t i = 0; i < NG; ++i) {
-        for (int k = 0; k < 3; ++k) blk[k * s->cs + i] = u ? (S)u[3 * i + k] : S(0);
-        blk[3 * s->cs + i] = p ? (S)p[i] : S(0);
+        for (int k = 0; k < 3; ++k) blk[k * s->cs + i] = u ? u[3 * i + k] : 0.0;
+        blk[3 * s->cs + i] = p ? p[i] : 0.0;
     }
-    for (size_t f = 0; f < NF; ++f) fl[f] = F ? (S)F[f] : S(0);
-    GR_HIP(hipMemcpyAsync(s->st.p, blk.data(), blk.size() * sizeof(S), hipMemcpyHostToDevice, m->stream));
-    GR_HIP(hipMemcpyAsync(s->flux.p, fl.data(), fl.size() * sizeof(S), hipMemcpyHostToDevice, m->stream));
+    for (size_t f = 0; f < NF; ++f) fl[f] = F ? F[f] : 0.0;
+    GR_HIP(hipMemcpyAsync(s->carrier.p, blk.data(), blk.size() * sizeof(double), hipMemcpyHostToDevice, m->stream));
+    GR_HIP(hipMemcpyAsync(s->flux.p, fl.data(), fl.size() * sizeof(double), hipMemcpyHostToDevice, m->stream));
     GR_HIP(hipStreamSynchronize(m->stream));
 }
 
-template <typename S>
 void download_state(gr_nsi *s, double *u, double *p, double *F) {
     gr_mesh *m = s->mesh;
     const size_t NG = (size_t)m->h.n_cells + m->h.n_ghosts, NF = m->h.n_faces;
-    std::vector<S> blk(4 * s->cs), fl(NF ? NF : 1);
-    GR_HIP(hipMemcpyAsync(blk.data(), s->st.p, blk.size() * sizeof(S), hipMemcpyDeviceToHost, m->stream));
-    if (NF) GR_HIP(hipMemcpyAsync(fl.data(), s->flux.p, NF * sizeof(S), hipMemcpyDeviceToHost, m->stream));
+    std::vector<double> blk(4 * s->cs), fl(NF ? NF : 1);
+    GR_HIP(hipMemcpyAsync(blk.data(), s->carrier.p, blk.size() * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    if (NF) GR_HIP(hipMemcpyAsync(fl.data(), s->flux.p, NF * sizeof(double), hipMemcpyDeviceToHost, m->stream));
     GR_HIP(hipStreamSynchronize(m->stream));
     for (size_t i = 0; i < NG; ++i) {
         if (u)
-            for (int k = 0; k < 3; ++k) u[3 * i + k] = (double)blk[k * s->cs + i];
-        if (p) p[i] = (double)blk[3 * s->cs + i];
+            for (int k = 0; k < 3; ++k) u[3 * i + k] = blk[k * s->cs + i];
+        if (p) p[i] = blk[3 * s->cs + i];
     }
     if (F)
-        for (size_t f = 0; f < NF; ++f) F[f] = (double)fl[f];
+        for (size_t f = 0; f < NF; ++f) F[f] = fl[f];
 }
 
 template <typename S>
 void fill_pointer_table(gr_nsi *s) {
     std::vector<void *> t(15);
-    for (int k = 0; k < 9; ++k) t[k] = s->blk<S>(8 + k);
+    for (int k = 0; k < 9; ++k) t[k] = s->blk<S>(4 + k);
     for (int k = 0; k < 3; ++k) {
-        t[9 + k] = s->blk<S>(17 + k);
-        t[12 + k] = s->blk<S>(20 + k);
+        t[9 + k] = s->blk<S>(13 + k);
+        t[12 + k] = s->blk<S>(16 + k);
     }
     s->ptrs.upload(t.data(), t.size(), s->mesh->stream);
 }
@@ -589,10 +695,12 @@ int gr_nsi_create(gr_mesh *m, const gr_nsi_config *cfg, gr_nsi **out) {
             const gr::HostMesh &h = m->h;
             const uint32_t N = h.n_cells, G = h.n_ghosts, F = h.n_faces;
             s->cs = gr::pad64((size_t)N + G);
-            s->st.alloc(26 * s->cs * s->es());
+            s->st.alloc(gr_nsi::kBulkBlocks * s->cs * s->es());
             GR_HIP(hipMemsetAsync(s->st.p, 0, s->st.n, m->stream));
-            s->flux.alloc(2 * gr::pad64(F) * s->es());
-            GR_HIP(hipMemsetAsync(s->flux.p, 0, s->flux.n, m->stream));
+            s->carrier.alloc(4 * s->cs);
+            GR_HIP(hipMemsetAsync(s->carrier.p, 0, s->carrier.n * sizeof(double), m->stream));
+            s->flux.alloc(2 * gr::pad64(F));
+            GR_HIP(hipMemsetAsync(s->flux.p, 0, s->flux.n * sizeof(double), m->stream));
             s->cgv.alloc(6 * s->cs * s->es());
             GR_HIP(hipMemsetAsync(s->cgv.p, 0, s->cgv.n, m->stream));
             // face tables: the lower-numbered cell lists the face first
@@ -623,7 +731,9 @@ int gr_nsi_create(gr_mesh *m, const gr_nsi_config *cfg, gr_nsi **out) {
             s->cont.alloc(1);
             s->cg.alloc(1);
             GR_HIP(hipMemsetAsync(s->cg.p, 0, sizeof(gr::CgScalars), m->stream));
-            GR_HIP(hipHostMalloc((void **)&s->cg_host, sizeof(gr::CgScalars)));
+            GR_HIP(hipHostMalloc((void **)&s->cg_host, 3 * sizeof(gr::CgScalars)));
+            GR_HIP(hipHostMalloc((void **)&s->cont_host, sizeof(double)));
+            for (auto &e : s->ev) GR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             s->ptrs.alloc(15);
             if (cfg->dtype == GR_F64) fill_pointer_table<double>(s);
             else fill_pointer_table<float>(s);
@@ -653,22 +763,22 @@ int gr_nsi_set_wall(gr_nsi *s, const char *region, const double u[3]) {
             if (h.region_names[r] == region) break;
         if (r == h.n_regions) throw gr::Error(GR_ERR_NOT_FOUND, std::string("region '") + region + "' not found in mesh");
         for (int k = 0; k < 3; ++k) s->walls.u[r][k] = u[k];
+        s->drop_graphs();   // the wall velocities are kernel arguments of the captured graphs
     });
 }
 
 int gr_nsi_set_state(gr_nsi *s, const double *u, const double *p, const double *F) {
     return guarded([&] {
         require(s != nullptr, "null argument");
-        if (s->cfg.dtype == GR_F64) upload_state<double>(s, u, p, F);
-        else upload_state<float>(s, u, p, F);
+        upload_state(s, u, p, F);
+        s->history_start = s->history.size();
     });
 }
 
 int gr_nsi_get_state(gr_nsi *s, double *u, double *p, double *F) {
     return guarded([&] {
         require(s != nullptr, "null argument");
-        if (s->cfg.dtype == GR_F64) download_state<double>(s, u, p, F);
-        else download_state<float>(s, u, p, F);
+        download_state(s, u, p, F);
     });
 }
 
@@ -694,10 +804,9 @@ int gr_nsi_iterate(gr_nsi *s, int max_steps, double tol, int *steps, double *rat
     return guarded([&] {
         require(s != nullptr, "null argument");
         require(max_steps >= 1 && tol >= 0, "bad max_steps or tol");
-        const size_t base = s->history.size();
-        double ref = 0, r = 1;
+        double r = 1;
         int k = 0;
-        for (; k < max_steps; ++k) {
+        while (k < max_steps) {
             double c = 0;
             int it = 0;
             if (s->cfg.dtype == GR_F64) s->step<double>(&c, &it);
@@ -705,14 +814,14 @@ int gr_nsi_iterate(gr_nsi *s, int max_steps, double tol, int *steps, double *rat
             if (!std::isfinite(c)) throw gr::Error(GR_ERR_DIVERGED, "continuity residual is not finite (diverged)");
             s->history.push_back(c);
             s->cg_iters.push_back(it);
-            if (k == 0) ref = c > 0 ? c : 1.0;
-            r = c / ref;
-            if (tol > 0 && k > 0 && r < tol) {
-                ++k;
-                break;
-            }
+            ++k;
+            const size_t done = s->history.size() - s->history_start;
+            if (done < 5) continue;
+            double ref = 0;
+            for (size_t j = 0; j < 5; ++j) ref = std::max(ref, s->history[s->history_start + j]);
+            r = ref > 0 ? c / ref : 0.0;
+            if (tol > 0 && r < tol) break;
         }
-        (void)base;
         if (steps) *steps = k;
         if (ratio) *ratio = r;
     });

@@ -431,7 +431,9 @@ int gr_case_run(const char *case_path, int verbose, gr_solve_result *result);
  * N+G, face fluxes n_faces (volume flux from the face's lower- to its higher-numbered cell). */
 typedef struct gr_nsi gr_nsi;
 typedef struct gr_nsi_config {
-    int dtype;       /* GR_F64, or GR_F32: binary32 state and CG vectors, binary64 dot products */
+    int dtype;       /* GR_F64 (all binary64), or GR_F32 (mixed): binary64 velocity, pressure and
+                        face fluxes, binary32 gradients, predictor increments u* - u and CG
+                        vectors; binary64 dot products and continuity residual in both */
     double nu, dt;   /* kinematic viscosity, time step */
     int cg_max;      /* CG iterations per step at most */
     double cg_tol;   /* stop CG when |r| < cg_tol |b| (0: always cg_max iterations) */
@@ -447,8 +449,10 @@ int gr_nsi_get_state(gr_nsi *s, double *u, double *p, double *F);
 /* n_steps steps; cont / cg_iters (n_steps entries, may be NULL) receive each step's continuity
  * residual (sum over cells of |net outflow of the predicted fluxes|) and CG iterations. */
 int gr_nsi_step(gr_nsi *s, int n_steps, double *cont, int *cg_iters);
-/* Steps until the continuity residual falls below tol times that of the first step of this
- * call (or max_steps). */
+/* Steps until the continuity ratio falls below tol (or max_steps): the continuity residual of
+ * the step over the largest of the solver's first five steps since creation or the last
+ * gr_nsi_set_state (the normalisation is fixed once five steps exist; before that the ratio is
+ * not tested).  steps: steps run by this call; ratio: the last step's ratio. */
 int gr_nsi_iterate(gr_nsi *s, int max_steps, double tol, int *steps, double *ratio);
 /* The last n steps' continuity residuals and CG iterations. */
 int gr_nsi_history(const gr_nsi *s, double *cont, int *cg_iters, int n);
