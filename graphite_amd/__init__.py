@@ -562,6 +562,24 @@ class Batch:
     def set_graph_iters(self, iters: int) -> None:
         _check(load().gr_batch_set_graph_iters(self._h, iters))
 
+    def autotune(self, n_iter: int = 10) -> dict:
+        """Chooses the launch schedule (cells per block, flux occupancy cap, graph length) by
+        timed runs; the state is left unchanged.  Returns the choice."""
+        sch = (C.c_int * 3)()
+        _check(load().gr_batch_autotune(self._h, n_iter, sch))
+        return {"cells_per_block": sch[0], "flux_waves_cap": bool(sch[1]), "graph_iters": sch[2]}
+
+    @property
+    def schedule(self) -> dict:
+        sch = (C.c_int * 3)()
+        _check(load().gr_batch_get_schedule(self._h, sch))
+        return {"cells_per_block": sch[0], "flux_waves_cap": bool(sch[1]), "graph_iters": sch[2]}
+
+    @schedule.setter
+    def schedule(self, s: dict) -> None:
+        sch = (C.c_int * 3)(int(s["cells_per_block"]), int(bool(s["flux_waves_cap"])), int(s["graph_iters"]))
+        _check(load().gr_batch_set_schedule(self._h, sch))
+
     def set_states(self, rho, rhou, rhoe) -> None:
         """All instances: rho/rhoe (n_inst, N+G), rhou (n_inst, N+G, 3)."""
         r, u, e = _f64(rho).reshape(-1), _f64(rhou).reshape(-1), _f64(rhoe).reshape(-1)

@@ -143,6 +143,9 @@ SIGNATURES = {
     "gr_batch_profile": (_i, [_vp, _i, _dp]),
     "gr_batch_kernel_bytes": (_i, [_vp, _dp]),
     "gr_batch_set_graph_iters": (_i, [_vp, _i]),
+    "gr_batch_get_schedule": (_i, [_vp, C.POINTER(C.c_int)]),
+    "gr_batch_set_schedule": (_i, [_vp, C.POINTER(C.c_int)]),
+    "gr_batch_autotune": (_i, [_vp, _i, C.POINTER(C.c_int)]),
     "gr_solver_force": (_i, [_vp, _s, _dp]),
     "gr_batch_force": (_i, [_vp, _s, _dp]),
     "gr_mesh_wall_distance": (_i, [_vp, C.POINTER(C.c_char_p), _u32, _dp]),

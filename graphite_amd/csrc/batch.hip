@@ -20,7 +20,7 @@
 namespace gr {
 
 constexpr int kBatchLanes = 64;
-constexpr int kCellsPerBlock = kBlock / kBatchLanes;   // 4
+constexpr int kCellsPerBlock = kBlock / kBatchLanes;   // 4 (ghost kernels; default schedule)
 constexpr int kFinalParts = 16;                       // partial-sum groups of the batch reduction
 constexpr uint32_t kReduceBlocks = 256;                // first-stage blocks of the batch reduction
 
@@ -51,8 +51,9 @@ struct BatchArgs {
 // Cell of this wave (wave-uniform, in a scalar register).  Blocks are mapped XCD-aware: the
 // blocks one XCD runs (block index mod 8) take a contiguous range of cells, so the neighbour
 // rows a cell re-reads are in that XCD's L2.
+template <int CPB>
 __device__ inline uint32_t batch_cell() {
-    return __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * kCellsPerBlock + (threadIdx.x >> 6));
+    return __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * CPB + (threadIdx.x >> 6));
 }
 
 // Far-field region uses the instance's Mach vector: copy the region with it substituted.
@@ -66,12 +67,13 @@ __device__ inline BcRegion inst_region(const BcRegion &R, const double *inst_mac
     return r;
 }
 
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<S> a, BcTable bc,
-                                                          const double *__restrict__ inst_mach, int kind, int corr,
-                                                          const BatchCtl *ctl) {
+// CPB cells (waves) per block: one of the schedules the batch's autotuner chooses among.
+template <typename S, int CPB>
+__global__ __launch_bounds__(CPB *kBatchLanes) void k_batch_gradlim(MeshView m, BatchArgs<S> a, BcTable bc,
+                                                                    const double *__restrict__ inst_mach, int kind,
+                                                                    int corr, const BatchCtl *ctl) {
     if (ctl->stop) return;
-    const uint32_t i = batch_cell();
+    const uint32_t i = batch_cell<CPB>();
     if (i >= m.N) return;
     const uint32_t k = threadIdx.x & 63;
     const size_t st = a.stride, me = (size_t)i * kBatchLanes + k;
@@ -134,16 +136,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_gradlim(MeshView m, BatchArgs<
     }
 }
 
-// At least 3 waves per SIMD: left to itself the compiler gives the binary64 kernel 174 VGPRs
-// (2 waves); capping it at 168 spills 6 registers and shortened the launch from 360 to 325 us
-// on the 64-flow C-grid (4 waves spill 45 and take 580 us).
-template <typename S, bool CORR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_batch_flux(MeshView m, BatchArgs<S> a, BcTable bc,
-                                                       const double *__restrict__ inst_mach, double cfl,
-                                                       double *__restrict__ partials, const BatchCtl *ctl) {
+// The flux + update of one cell per wave, CPB cells per block; block partials of the
+// per-instance residual.
+template <typename S, bool CORR, int CPB>
+__device__ __forceinline__ void batch_flux(const MeshView &m, const BatchArgs<S> &a, const BcTable &bc,
+                                           const double *__restrict__ inst_mach, double cfl,
+                                           double *__restrict__ partials, const BatchCtl *ctl) {
     if (ctl->stop) return;
-    __shared__ double s_err[kCellsPerBlock][kBatchLanes];
-    const uint32_t i = batch_cell();
+    __shared__ double s_err[CPB][kBatchLanes];
+    const uint32_t i = batch_cell<CPB>();
     const uint32_t k = threadIdx.x & 63, w = threadIdx.x >> 6;
     double e_loc = 0;
     if (i < m.N) {
@@ -235,9 +236,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
     if (threadIdx.x < kBatchLanes) {
         double t = 0;
 #pragma unroll
-        for (int ww = 0; ww < kCellsPerBlock; ++ww) t += s_err[ww][threadIdx.x];
+        for (int ww = 0; ww < CPB; ++ww) t += s_err[ww][threadIdx.x];
         partials[(size_t)blockIdx.x * kBatchLanes + threadIdx.x] = t;
     }
+}
+
+// Two occupancies of the same flux: left to the compiler (the binary64 kernel takes 174 VGPRs,
+// 2 waves per SIMD) and capped at 3 waves per SIMD (168 VGPRs, 6 spilled: 360 -> 325 us on the
+// 64-flow C-grid; 4 waves spill 45 and take 580 us).  The autotuner times both.
+template <typename S, bool CORR, int CPB>
+__global__ __launch_bounds__(CPB *kBatchLanes) void k_batch_flux(MeshView m, BatchArgs<S> a, BcTable bc,
+                                                                 const double *__restrict__ inst_mach, double cfl,
+                                                                 double *__restrict__ partials, const BatchCtl *ctl) {
+    batch_flux<S, CORR, CPB>(m, a, bc, inst_mach, cfl, partials, ctl);
+}
+template <typename S, bool CORR, int CPB>
+__global__ __launch_bounds__(CPB *kBatchLanes) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_batch_flux_w3(
+    MeshView m, BatchArgs<S> a, BcTable bc, const double *__restrict__ inst_mach, double cfl,
+    double *__restrict__ partials, const BatchCtl *ctl) {
+    batch_flux<S, CORR, CPB>(m, a, bc, inst_mach, cfl, partials, ctl);
 }
 
 // First stage of the per-instance residual sums: block r adds the flux blocks' partial rows
@@ -483,6 +500,11 @@ struct gr_batch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int graph_iters = 10;
     int last_iters = 0;
+    // launch schedule (the autotuner's choices): cells (waves) per block of the gradient and
+    // flux kernels, and whether the flux is capped at 3 waves per SIMD
+    int cpb = 4;
+    int flux_w3 = 1;
+    static constexpr int kMinCpb = 2;   // partial rows are allocated for the smallest choice
 
     ~gr_batch() {
         if (mesh && mesh->stream) (void)hipStreamSynchronize(mesh->stream);
@@ -497,7 +519,7 @@ struct gr_batch {
             g = nullptr;
         }
     }
-    uint32_t nblocks() const { return (mesh->h.n_cells + gr::kCellsPerBlock - 1) / gr::kCellsPerBlock; }
+    uint32_t nblocks() const { return (mesh->h.n_cells + cpb - 1) / cpb; }
     template <typename S> gr::BatchArgs<S> args(int b) const {
         return gr::BatchArgs<S>{(const S *)ubuf[b].p, (S *)ubuf[1 - b].p, (S *)grad.p, (S *)lim.p, stride};
     }
@@ -516,19 +538,35 @@ struct gr_batch {
     }
     // Stage 0: gradient+limiter, 1: flux+update, 2: per-instance reduction, of an iteration
     // reading buffer b.
+    template <typename S, int CPB> void launch_cpb(int stage, int b, gr::BatchCtl *c) {
+        gr_mesh *m = mesh;
+        const dim3 grid(nblocks()), blk(CPB * gr::kBatchLanes);
+        const gr::BatchCtl *cc = c;
+        if (stage == 0) {
+            hipLaunchKernelGGL((gr::k_batch_gradlim<S, CPB>), grid, blk, 0, m->stream, m->view(), args<S>(b), table(),
+                               inst_mach.p, cfg.limiter, cfg.corrected, cc);
+            return;
+        }
+#define GR_BATCH_FLUX(KERNEL, CORR)                                                                                   \
+    hipLaunchKernelGGL((gr::KERNEL<S, CORR, CPB>), grid, blk, 0, m->stream, m->view(), args<S>(b), table(), inst_mach.p, \
+                       cfg.cfl, partials.p, cc)
+        if (flux_w3) {
+            if (cfg.corrected) GR_BATCH_FLUX(k_batch_flux_w3, true);
+            else GR_BATCH_FLUX(k_batch_flux_w3, false);
+        } else {
+            if (cfg.corrected) GR_BATCH_FLUX(k_batch_flux, true);
+            else GR_BATCH_FLUX(k_batch_flux, false);
+        }
+#undef GR_BATCH_FLUX
+    }
+    // Stage 0: gradient+limiter, 1: flux+update, 2: per-instance reduction, of an iteration
+    // reading buffer b.
     template <typename S> void launch_stage(int stage, int b, gr::BatchCtl *c) {
         gr_mesh *m = mesh;
-        const dim3 grid(nblocks()), blk(gr::kBlock);
-        if (stage == 0) {
-            hipLaunchKernelGGL(gr::k_batch_gradlim<S>, grid, blk, 0, m->stream, m->view(), args<S>(b), table(),
-                               inst_mach.p, cfg.limiter, cfg.corrected, (const gr::BatchCtl *)c);
-        } else if (stage == 1) {
-            if (cfg.corrected)
-                hipLaunchKernelGGL((gr::k_batch_flux<S, true>), grid, blk, 0, m->stream, m->view(), args<S>(b),
-                                   table(), inst_mach.p, cfg.cfl, partials.p, (const gr::BatchCtl *)c);
-            else
-                hipLaunchKernelGGL((gr::k_batch_flux<S, false>), grid, blk, 0, m->stream, m->view(), args<S>(b),
-                                   table(), inst_mach.p, cfg.cfl, partials.p, (const gr::BatchCtl *)c);
+        if (stage < 2) {
+            if (cpb == 2) launch_cpb<S, 2>(stage, b, c);
+            else if (cpb == 8) launch_cpb<S, 8>(stage, b, c);
+            else launch_cpb<S, 4>(stage, b, c);
         } else {
             hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream, partials.p,
                                nblocks(), (const gr::BatchCtl *)c, partials2.p);
@@ -803,7 +841,7 @@ int gr_batch_create(gr_mesh *m, uint32_t n_inst, const gr_solver_config *cfg, gr
             b->mach_h.assign(3 * n_inst, 0.0);
             b->region_kind.assign(m->h.n_regions, -1);
             b->ctl.alloc(1);
-            b->partials.alloc((size_t)b->nblocks() * gr::kBatchLanes);
+            b->partials.alloc(((size_t)m->h.n_cells + gr_batch::kMinCpb - 1) / gr_batch::kMinCpb * gr::kBatchLanes);
             b->partials2.alloc((size_t)gr::kReduceBlocks * gr::kBatchLanes);
             b->out.alloc(gr::kBatchLanes);
             b->lane_flags.alloc(gr::kBatchLanes);
@@ -969,6 +1007,126 @@ int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]) {
         b->last_iters = 0;
         GR_HIP(hipMemcpyAsync(b->ubuf[1 - b->cur].p, saved.p, saved.n, hipMemcpyDeviceToDevice, m->stream));
         GR_HIP(hipStreamSynchronize(m->stream));
+    });
+}
+
+int gr_batch_autotune(gr_batch *b, int n_iter, int schedule[3]) {
+    return guarded([&] {
+        require(b && schedule, "null argument");
+        require(n_iter > 0 && n_iter <= 1000, "n_iter must be in 1..1000");
+        gr_mesh *m = b->mesh;
+        // The timed runs write both state buffers (and the gradient and limiter rows): both are
+        // saved here and restored at the end, so the batch's state is unchanged.
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        struct Guard {
+            hipEvent_t *e;
+            ~Guard() {
+                for (int k = 0; k < 2; ++k)
+                    if (e[k]) (void)hipEventDestroy(e[k]);
+            }
+        } guard{ev};
+        for (auto &e : ev) GR_HIP(hipEventCreate(&e));
+        gr::DevArray<char> saved[2];
+        for (int k = 0; k < 2; ++k) {
+            saved[k].alloc(b->ubuf[k].n);
+            GR_HIP(hipMemcpyAsync(saved[k].p, b->ubuf[k].p, saved[k].n, hipMemcpyDeviceToDevice, m->stream));
+        }
+        gr::BatchCtl &c0 = b->ctl_host[0];
+        auto reset_ctl = [&](int max_iter) {
+            std::memset(&c0, 0, sizeof c0);
+            c0.max_iter = max_iter;
+            c0.n_inst = (int)b->n_inst;
+            c0.hist_cap = b->hist_cap;
+            for (int k = 0; k < gr::kBatchLanes; ++k) c0.done[k] = k < (int)b->n_inst ? 0 : 1;
+            GR_HIP(hipMemcpyAsync(b->ctl.p, &c0, sizeof c0, hipMemcpyHostToDevice, m->stream));
+        };
+        auto elapsed = [&]() {
+            GR_HIP(hipGetLastError());
+            GR_HIP(hipEventSynchronize(ev[1]));
+            float t = 0;
+            GR_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
+            return (double)t;
+        };
+        // 1. cells per block x flux occupancy: gradient + flux launches back to back
+        reset_ctl(1 << 30);
+        double best = 1e300;
+        int best_cpb = b->cpb, best_w3 = b->flux_w3;
+        for (int cpb : {2, 4, 8}) {
+            for (int w3 : {1, 0}) {
+                b->cpb = cpb;
+                b->flux_w3 = w3;
+                b->enqueue_stage(0, b->cur, b->ctl.p);   // first use of the variant, untimed
+                b->enqueue_stage(1, b->cur, b->ctl.p);
+                GR_HIP(hipEventRecord(ev[0], m->stream));
+                for (int k = 0; k < n_iter; ++k) {
+                    b->enqueue_stage(0, b->cur, b->ctl.p);
+                    b->enqueue_stage(1, b->cur, b->ctl.p);
+                }
+                GR_HIP(hipEventRecord(ev[1], m->stream));
+                const double t = elapsed();
+                if (t < best) {
+                    best = t;
+                    best_cpb = cpb;
+                    best_w3 = w3;
+                }
+            }
+        }
+        b->cpb = best_cpb;
+        b->flux_w3 = best_w3;
+        // 2. graph length: the same number of iterations replayed as the solve loop replays them
+        // (graph, copy of the control block to the host, event)
+        const int total = 40;
+        double best_g = 1e300;
+        int best_k = b->graph_iters;
+        for (int K : {4, 10, 20, 40}) {   // even: a replay returns to the buffer it started from
+            b->graph_iters = K;
+            b->drop_graphs();
+            hipGraphExec_t g = b->graph(b->cur);
+            reset_ctl(1 << 30);
+            GR_HIP(hipGraphLaunch(g, m->stream));   // untimed first replay
+            GR_HIP(hipEventRecord(ev[0], m->stream));
+            for (int r = 0; r < total / K; ++r) {
+                GR_HIP(hipGraphLaunch(g, m->stream));
+                GR_HIP(hipMemcpyAsync(&b->ctl_host[1], b->ctl.p, sizeof c0, hipMemcpyDeviceToHost, m->stream));
+            }
+            GR_HIP(hipEventRecord(ev[1], m->stream));
+            const double t = elapsed();
+            if (t < best_g) {
+                best_g = t;
+                best_k = K;
+            }
+        }
+        b->graph_iters = best_k;
+        b->drop_graphs();
+        for (int k = 0; k < 2; ++k)
+            GR_HIP(hipMemcpyAsync(b->ubuf[k].p, saved[k].p, saved[k].n, hipMemcpyDeviceToDevice, m->stream));
+        GR_HIP(hipStreamSynchronize(m->stream));
+        b->last_iters = 0;
+        schedule[0] = b->cpb;
+        schedule[1] = b->flux_w3;
+        schedule[2] = b->graph_iters;
+    });
+}
+
+int gr_batch_get_schedule(const gr_batch *b, int schedule[3]) {
+    return guarded([&] {
+        require(b && schedule, "null argument");
+        schedule[0] = b->cpb;
+        schedule[1] = b->flux_w3;
+        schedule[2] = b->graph_iters;
+    });
+}
+
+int gr_batch_set_schedule(gr_batch *b, const int schedule[3]) {
+    return guarded([&] {
+        require(b && schedule, "null argument");
+        require(schedule[0] == 2 || schedule[0] == 4 || schedule[0] == 8, "cells per block must be 2, 4 or 8");
+        require(schedule[1] == 0 || schedule[1] == 1, "flux occupancy flag must be 0 or 1");
+        require(schedule[2] >= 2 && schedule[2] <= 200 && schedule[2] % 2 == 0, "graph length must be even, 2..200");
+        b->cpb = schedule[0];
+        b->flux_w3 = schedule[1];
+        b->graph_iters = schedule[2];
+        b->drop_graphs();
     });
 }
 

@@ -389,6 +389,19 @@ int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]);
 int gr_batch_kernel_bytes(const gr_batch *b, double bytes[3]);
 /* Iterations captured per HIP graph (even, 2..200; default 10). */
 int gr_batch_set_graph_iters(gr_batch *b, int iters);
+/* Launch schedule of the batched kernels: schedule[0] cells (waves) per block of the gradient and
+ * flux kernels (2, 4 or 8; default 4), [1] 1 if the flux kernel is capped at 3 waves per SIMD
+ * (default 1), [2] iterations per HIP graph (as gr_batch_set_graph_iters).  Results do not
+ * depend on it (per-cell arithmetic is the same; only the block grouping of the residual
+ * partial sums changes, in the last bits of the residual norms).  The reference launches a
+ * fixed CKW of 256 threads (include/datatypes.cuh:19, src/models/euler.cu:662). */
+int gr_batch_get_schedule(const gr_batch *b, int schedule[3]);
+int gr_batch_set_schedule(gr_batch *b, const int schedule[3]);
+/* Chooses the schedule: each (cells per block, flux occupancy) pair times n_iter gradient +
+ * flux launches back to back, the fastest is kept; then graph lengths 4, 10, 20, 40 time 40
+ * iterations replayed as gr_batch_iterate replays them.  The batch's state is saved and
+ * restored.  schedule: the choice, as gr_batch_get_schedule. */
+int gr_batch_autotune(gr_batch *b, int n_iter, int schedule[3]);
 
 /* ---- post-processing (no reference counterpart) ---------------------------------------------
  * Pressure force on a wall region: F = sum over its faces of p_cell * n * A, n pointing out

@@ -168,3 +168,44 @@ def test_batch_profile_leaves_downloads_unchanged(gr, box):
     for sa, sb in zip(run(False), run(True)):
         for x, y in zip(sa, sb):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_autotuned_and_every_schedule_equal_single_solves(gr, box, dtype):
+    """The launch schedule (cells per block 2/4/8, flux occupancy cap, graph length) changes no
+    cell's arithmetic: after autotuning (which leaves the state unchanged) and under every
+    fixed schedule, each instance equals its single-flow solve bit for bit."""
+    print(f"case: batch schedules vs single solves, {dtype}", flush=True)
+    m = box
+    dt = gr.GR_F64 if dtype == "f64" else gr.GR_F32
+    M = machs(4, seed=21)
+    ref = [single(gr, m, M[k], 12, dt)[0].get_state() for k in range(4)]
+
+    def batch():
+        b = gr.Batch(m, 4, dt, cfl=0.5)
+        for name, kind in BOX_BCS:
+            b.set_bc(name, kind)
+        b.set_freestream(M, P_INF, T_INF)
+        return b
+
+    def check(b, tag):
+        its, _, ran = b.iterate(12)
+        assert list(its) == [12] * 4, tag
+        for k in range(4):
+            for x, y in zip(b.get_state(k), ref[k]):
+                assert np.array_equal(x[:m.n_cells], y[:m.n_cells]), (tag, k)
+
+    b = batch()
+    before = [b.get_state(k) for k in range(4)]
+    chosen = b.autotune(5)
+    print("autotuned schedule:", chosen, flush=True)
+    assert chosen == b.schedule
+    for k in range(4):                      # autotuning left the state as it was
+        for x, y in zip(b.get_state(k), before[k]):
+            assert np.array_equal(x, y)
+    check(b, "autotuned")
+    for cpb in (2, 4, 8):
+        for cap in (False, True):
+            b = batch()
+            b.schedule = {"cells_per_block": cpb, "flux_waves_cap": cap, "graph_iters": 4}
+            check(b, (cpb, cap))
