@@ -173,9 +173,12 @@ def test_bulk_states_round_trip(gr, small):
 def test_trained_network_cuts_iterations_and_keeps_lift_drag(gr, small):
     """Train on 64 solver-generated subsonic flows of the coarse C-grid, then warm-start 16 new
     flows: the warm start reaches 1e-3 of the freestream residual in far fewer iterations than
-    freestream starts, and after the same further work both give the same lift and drag to
-    5e-3 of the sweep's largest lift (the limit cycle the scheme ends in moves the cold
-    solve's own lift by about 1e-3)."""
+    freestream starts (measured p50 19.5 against 711), and after the same further work (20,000
+    iterations, then means over 4,000) both give the same lift and drag: per flow, relative to
+    its own coefficient, measured C_L median 1.6e-4 / worst 2.1e-3, C_D median 8.1e-5 / worst
+    1.7e-2 (the limit cycle the scheme ends in moves the cold solve's own lift by about 1e-3);
+    the asserts are those figures with a margin, and the 1e-4 target is met by neither worst
+    case (recorded in the design notes)."""
     from graphite_amd import warmstart as ws
     m = small
     rng = np.random.default_rng(42)
@@ -199,4 +202,7 @@ def test_trained_network_cuts_iterations_and_keeps_lift_drag(gr, small):
     agree = ws.force_agreement(cold, warm, test, "airfoil", 0.1, 20000, 4000, 250)
     print("force agreement:", agree)
     assert agree["samples"] == 16 and agree["lifting_flows"] >= 1
-    assert agree["mean_cl_rel_diff_p50_max"][1] < 5e-3 and agree["mean_cd_rel_diff_p50_max"][1] < 2e-2
+    cl_p50, cl_max = agree["mean_cl_rel_diff_p50_max"]
+    cd_p50, cd_max = agree["mean_cd_rel_diff_p50_max"]
+    assert cl_p50 < 5e-4 and cd_p50 < 3e-4
+    assert cl_max < 4e-3 and cd_max < 2e-2
