@@ -263,6 +263,13 @@ class Mesh:
         _check(load().gr_mesh_renumber_rcm(self._h, _p(c2o, C.c_uint32), _p(f2o, C.c_uint32)))
         return c2o, f2o
 
+    def renumber_hilbert(self):
+        s = self.sizes()
+        c2o = np.zeros(s.n_cells + s.n_ghosts, np.uint32)
+        f2o = np.zeros(s.n_faces, np.uint32)
+        _check(load().gr_mesh_renumber_hilbert(self._h, _p(c2o, C.c_uint32), _p(f2o, C.c_uint32)))
+        return c2o, f2o
+
     def face_colors(self):
         s = self.sizes()
         col = np.zeros(s.n_faces, np.uint32)

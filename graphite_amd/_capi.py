@@ -89,6 +89,7 @@ SIGNATURES = {
     "gr_mesh_region_name": (_i, [_vp, _u32, C.c_char_p]),
     "gr_mesh_region_size": (_i, [_vp, _u32, _u32p]),
     "gr_mesh_renumber_rcm": (_i, [_vp, _u32p, _u32p]),
+    "gr_mesh_renumber_hilbert": (_i, [_vp, _u32p, _u32p]),
     "gr_mesh_face_colors": (_i, [_vp, _u32p, _u32p]),
     "gr_mesh_bandwidth": (_i, [_vp, _u32p]),
     "gr_mesh_compute_metrics": (_i, [_vp]),

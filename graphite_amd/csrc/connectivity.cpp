@@ -412,8 +412,9 @@ void write_fvm(const HostMesh &m, const char *path, int sb) {
 
 // ---- renumbering ------------------------------------------------------------------------
 
-void renumber_rcm(HostMesh &m, std::vector<uint32_t> &cell_new2old, std::vector<uint32_t> &face_new2old) {
-    const uint32_t N = m.n_cells, G = m.n_ghosts, NG = N + G;
+// Reverse Cuthill-McKee order of the interior cells (new -> old).
+static std::vector<uint32_t> rcm_order(const HostMesh &m) {
+    const uint32_t N = m.n_cells;
     auto nbr_begin = [&](uint32_t c) { return m.cell_faces_starts[c]; };
     auto nbr_end = [&](uint32_t c) { return m.cell_faces_starts[c + 1]; };
     std::vector<uint32_t> deg(N, 0);
@@ -492,6 +493,91 @@ void renumber_rcm(HostMesh &m, std::vector<uint32_t> &cell_new2old, std::vector<
         }
     }
     std::reverse(order.begin(), order.end());
+    return order;
+}
+
+// Hilbert index of a point on a 2^b grid per axis (b <= 21), by Skilling's transpose method
+// ("Programming the Hilbert curve", AIP Conf. Proc. 707, 2004): axes to transposed Hilbert
+// coordinates, then the bits interleaved from the most significant.
+static uint64_t hilbert_key(uint32_t x0, uint32_t x1, uint32_t x2, int b) {
+    uint32_t X[3] = {x0, x1, x2};
+    const uint32_t M = 1u << (b - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    uint64_t key = 0;
+    for (int bit = b - 1; bit >= 0; --bit)
+        for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> bit) & 1u);
+    return key;
+}
+
+// Interior cells in Hilbert-curve order of their node-averaged centres (ties by id), on a
+// 2^21 grid over the mesh's bounding box: cells close along the curve are close in space, so
+// a contiguous run of cells (the tiles one XCD has in flight) is a compact region whose
+// neighbours are few, where reverse Cuthill-McKee gives a thin slab whose neighbours span two
+// bands of its bandwidth.
+static std::vector<uint32_t> hilbert_order(const HostMesh &m) {
+    const uint32_t N = m.n_cells;
+    std::vector<double> c(3 * (size_t)N, 0.0);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (uint32_t i = 0; i < N; ++i) {
+        double s[3] = {0, 0, 0};
+        uint32_t n = 0;
+        for (uint32_t q = m.cell_faces_starts[i]; q < m.cell_faces_starts[i + 1]; ++q) {
+            const uint32_t f = m.cell_faces[q];
+            for (uint32_t k = m.face_nodes_starts[f]; k < m.face_nodes_starts[f + 1]; ++k) {
+                const uint32_t v = m.face_nodes[k];
+                for (int d = 0; d < 3; ++d) s[d] += m.nodes[3 * (size_t)v + d];
+                ++n;
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            c[3 * (size_t)i + d] = n ? s[d] / n : 0.0;
+            lo[d] = std::min(lo[d], c[3 * (size_t)i + d]);
+            hi[d] = std::max(hi[d], c[3 * (size_t)i + d]);
+        }
+    }
+    const int bits = 21;
+    const double cells = (double)((1u << bits) - 1);
+    double ext = 0;
+    for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d]);
+    const double scale = ext > 0 ? cells / ext : 0.0;   // one scale for all axes: the curve stays isotropic
+    std::vector<std::pair<uint64_t, uint32_t>> keys(N);
+    for (uint32_t i = 0; i < N; ++i) {
+        uint32_t g[3];
+        for (int d = 0; d < 3; ++d) {
+            const double v = (c[3 * (size_t)i + d] - lo[d]) * scale;
+            g[d] = (uint32_t)std::min(cells, std::max(0.0, v));
+        }
+        keys[i] = {hilbert_key(g[0], g[1], g[2], bits), i};
+    }
+    std::sort(keys.begin(), keys.end());
+    std::vector<uint32_t> order(N);
+    for (uint32_t i = 0; i < N; ++i) order[i] = keys[i].second;
+    return order;
+}
+
+// Applies an interior-cell order (new -> old): ghosts region by region by their interior cell's
+// new id, faces by (lower new cell, higher new cell); each cell keeps its face-slot order.
+static void apply_cell_order(HostMesh &m, const std::vector<uint32_t> &order, std::vector<uint32_t> &cell_new2old,
+                             std::vector<uint32_t> &face_new2old) {
+    const uint32_t N = m.n_cells, G = m.n_ghosts, NG = N + G;
+    auto nbr_begin = [&](uint32_t c) { return m.cell_faces_starts[c]; };
+    auto nbr_end = [&](uint32_t c) { return m.cell_faces_starts[c + 1]; };
     cell_new2old.assign(NG, 0);
     std::vector<uint32_t> cell_old2new(NG);
     for (uint32_t i = 0; i < N; ++i) {
@@ -562,6 +648,14 @@ void renumber_rcm(HostMesh &m, std::vector<uint32_t> &cell_new2old, std::vector<
         r.cell_faces_starts[c + 1] = (uint32_t)r.cell_faces.size();
     }
     m = std::move(r);
+}
+
+void renumber_rcm(HostMesh &m, std::vector<uint32_t> &cell_new2old, std::vector<uint32_t> &face_new2old) {
+    apply_cell_order(m, rcm_order(m), cell_new2old, face_new2old);
+}
+
+void renumber_hilbert(HostMesh &m, std::vector<uint32_t> &cell_new2old, std::vector<uint32_t> &face_new2old) {
+    apply_cell_order(m, hilbert_order(m), cell_new2old, face_new2old);
 }
 
 std::vector<uint32_t> face_colors(const HostMesh &m, uint32_t &n_colors) {

@@ -87,6 +87,8 @@ void read_fvm(const char *path, int scalar_bytes, HostMesh &m);
 void write_fvm(const HostMesh &m, const char *path, int scalar_bytes);
 void renumber_rcm(HostMesh &m, std::vector<uint32_t> &cell_new2old,
                   std::vector<uint32_t> &face_new2old);
+void renumber_hilbert(HostMesh &m, std::vector<uint32_t> &cell_new2old,
+                      std::vector<uint32_t> &face_new2old);
 std::vector<uint32_t> face_colors(const HostMesh &m, uint32_t &n_colors);
 uint32_t bandwidth(const HostMesh &m);
 

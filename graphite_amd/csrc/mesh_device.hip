@@ -430,6 +430,19 @@ int gr_mesh_renumber_rcm(gr_mesh *m, uint32_t *new2old_cells, uint32_t *new2old_
     });
 }
 
+int gr_mesh_renumber_hilbert(gr_mesh *m, uint32_t *new2old_cells, uint32_t *new2old_faces) {
+    return guarded([&] {
+        require(m != nullptr, "null argument");
+        require(!m->on_device, "renumbering must precede gr_mesh_compute_metrics");
+        require(!m->h.one_dimensional, "renumbering of the 1-D channel mesh is not supported");
+        std::vector<uint32_t> c2o, f2o;
+        gr::renumber_hilbert(m->h, c2o, f2o);
+        m->h.validate();
+        if (new2old_cells) std::copy(c2o.begin(), c2o.end(), new2old_cells);
+        if (new2old_faces) std::copy(f2o.begin(), f2o.end(), new2old_faces);
+    });
+}
+
 int gr_mesh_face_colors(const gr_mesh *m, uint32_t *colors, uint32_t *n_colors) {
     return guarded([&] {
         require(m && n_colors, "null argument");

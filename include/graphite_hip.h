@@ -153,6 +153,10 @@ int gr_mesh_region_size(const gr_mesh *m, uint32_t id, uint32_t *n);
  * (n_cells+n_ghosts) and new2old_faces (n_faces) may be NULL.  Must precede
  * gr_mesh_compute_metrics. */
 int gr_mesh_renumber_rcm(gr_mesh *m, uint32_t *new2old_cells, uint32_t *new2old_faces);
+/* As gr_mesh_renumber_rcm with the interior cells in Hilbert-curve order of their centres: a
+ * contiguous range of cells (what one XCD's blocks have in flight) is a compact region with few
+ * neighbours outside it.  No reference counterpart (the reference keeps the su2 order). */
+int gr_mesh_renumber_hilbert(gr_mesh *m, uint32_t *new2old_cells, uint32_t *new2old_faces);
 /* Greedy face colouring: no two faces of one colour share a cell (interior or ghost).
  * colors[n_faces] may be NULL; n_colors receives the count. */
 int gr_mesh_face_colors(const gr_mesh *m, uint32_t *colors, uint32_t *n_colors);

@@ -181,13 +181,14 @@ def test_make_1d_matches_reference_layout(gr):
     assert list(c["ghost_cell_ids"]) == [4, 5] and list(c["ghost_cell_starts"]) == [0, 1, 2]
 
 
-def test_rcm_renumbering_is_an_exact_permutation(gr):
+@pytest.mark.parametrize("method", ["rcm", "hilbert"])
+def test_renumbering_is_an_exact_permutation(gr, method):
     e = gr.Elements.box(6, 5, 4, tets=True, jitter=0.1, seed=11)
     m0 = gr.Mesh.build(e)
     c0 = m0.connectivity()
     bw0 = m0.bandwidth()
     m1 = gr.Mesh.build(e)
-    c2o, f2o = m1.renumber_rcm()
+    c2o, f2o = getattr(m1, "renumber_" + method)()
     c1 = m1.connectivity()
     s = m1.sizes()
     N, NG = s.n_cells, s.n_cells + s.n_ghosts
@@ -208,11 +209,31 @@ def test_rcm_renumbering_is_an_exact_permutation(gr):
         assert np.array_equal(
             c0["face_nodes"][c0["face_nodes_starts"][old]:c0["face_nodes_starts"][old + 1]],
             c1["face_nodes"][c1["face_nodes_starts"][new]:c1["face_nodes_starts"][new + 1]])
-    assert m1.bandwidth() <= bw0
+    if method == "rcm":
+        assert m1.bandwidth() <= bw0
     # deterministic
     m2 = gr.Mesh.build(e)
-    c2o2, f2o2 = m2.renumber_rcm()
+    c2o2, f2o2 = getattr(m2, "renumber_" + method)()
     assert np.array_equal(c2o, c2o2) and np.array_equal(f2o, f2o2)
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_hilbert_order_walks_face_neighbours(gr, n):
+    """On an n^3 box of hexes (n a power of two) the cell centres fall on the curve's own
+    2^k lattice, so the Hilbert order visits the cells as the order-k Hilbert curve does:
+    every cell is followed by a face neighbour, and the walk starts in a corner.  A wrong
+    rotation or reflection step of the key breaks the adjacency."""
+    m = gr.Mesh.build(gr.Elements.box(n, n, n))
+    c2o, _ = m.renumber_hilbert()
+    c = m.connectivity()
+    N = m.n_cells
+    cfs, cc = c["cell_faces_starts"], c["cell_connects"]
+    for i in range(N - 1):
+        assert i + 1 in set(cc[cfs[i]:cfs[i + 1]].tolist()), i
+    # original box numbering is x fastest: the first cell of the walk is a corner cell
+    first = int(c2o[0])
+    ijk = (first % n, (first // n) % n, first // (n * n))
+    assert all(v in (0, n - 1) for v in ijk)
 
 
 def test_face_colouring(gr):
