@@ -5,7 +5,7 @@ taper 0.56, semispan 1.5 root chords; 20 prism layers (growth 1.2) on the wall a
 out to a half sphere of 25 chords (1,069,112 cells), lengths scaled by 0.01; M 0.80, AoA 2.5
 deg; second order with the reference limiter, mixed precision (binary32 state, gradient,
 limiter and flux; binary64 metrics, least-squares inverse and residual norms); cells
-renumbered by reverse Cuthill-McKee.  A step is one solver iteration (boundary conditions, LSQ
+renumbered along a Hilbert curve of their centres.  A step is one solver iteration (boundary conditions, LSQ
 gradient + limiter, flux + local time-step update, residual norm) with the state resident in
 HBM.
 
@@ -95,7 +95,7 @@ def batched_leg(gr, args):
     from graphite_amd import warmstart as ws
     log("bench: batched C-grid leg")
     m = gr.Mesh.build(gr.Elements.cgrid_naca())
-    m.renumber_rcm()
+    m.renumber_hilbert()
     m.compute_metrics()
     s = m.sizes()
     machs = ws.sample_instances(64, seed=2024)
@@ -104,6 +104,7 @@ def batched_leg(gr, args):
     steps = max(20, args.steps // 4)
     for name, dtype in (("f64", gr.GR_F64), ("mixed", gr.GR_F32)):
         b = ws.make_batch(m, machs, dtype, args.cfl)
+        schedule = b.autotune(10)
         b.iterate(args.warmup)
         t0 = time.perf_counter()
         b.iterate(steps)
@@ -111,6 +112,7 @@ def batched_leg(gr, args):
         prof = b.profile(20)
         kb = b.kernel_bytes()
         res[name] = {"value": s.n_cells * 64 * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / steps * 1e3,
+                     "schedule": schedule,
                      "kernels": {k: {"ms": prof[k], "bytes": kb[k], "GBps": kb[k] / (prof[k] * 1e-3) / 1e9}
                                  for k in prof}}
     return res
@@ -123,7 +125,7 @@ def warm_start_leg(gr, args):
     from graphite_amd import warmstart as ws
     log("bench: warm-start leg (training set)")
     m = gr.Mesh.build(gr.Elements.cgrid_naca())
-    m.renumber_rcm()
+    m.renumber_hilbert()
     m.compute_metrics()
     wd = ws.wall_distance(m)
     N = m.n_cells
@@ -147,10 +149,10 @@ def warm_start_leg(gr, args):
                                           lr_final=2e-5)
     test = ws.sample_instances(64, seed=99)
     log("bench: warm-start leg (cold and warm solves)")
-    cold = ws.make_batch(m, test)
+    cold = ws.make_batch(m, test, tune=True)
     fs = cold.residual_norms()
     cs = ws.solve(cold, 1e-3, fs, 20000)
-    warm = ws.make_batch(m, test)
+    warm = ws.make_batch(m, test, tune=True)
     t0 = time.perf_counter()
     _, used = warm.warm_start(net, wd, norm)
     t_pred = time.perf_counter() - t0
@@ -178,7 +180,9 @@ def warm_start_leg(gr, args):
                                log=log)
     rel = lambda a, b: (np.abs(a - b).max(axis=0) / np.abs(b[:, 0]).max()).tolist()
     it_w, it_c = wsol.iterations, cs.iterations
+    forces = warm_forces(gr, args, m, net, norm, test, s_ref) if args.warm_forces else None
     return {
+        "time_to_settled_forces": forces,
         "workload": "NACA0012 C-grid 384x96x1, 64 new flows (Mach U[0.5,0.85], AoA U[-2,6] deg), binary64, "
                     "to 1e-3 of each flow's freestream residual",
         "training": {"flows": n_flows, "data_seconds": t_data, "rows": int(len(idx)), "steps": args.warm_steps,
@@ -217,8 +221,8 @@ def warm_forces(gr, args, m, net, norm, test, s_ref):
     res = {"horizon_iterations": H, "window_iterations": W, "sample_every": every,
            "criterion": "trailing-window mean of C_L and C_D within tol of the cold run's mean over its last window, "
                         "relative to max(|C|, 0.05 for lift, 0.005 for drag), from then on"}
-    cold = ws.make_batch(m, test)
-    warm = ws.make_batch(m, test)
+    cold = ws.make_batch(m, test, tune=True)
+    warm = ws.make_batch(m, test, tune=True)
     t0 = time.perf_counter()
     warm.warm_start(net, ws.wall_distance(m), norm)
     t_pred = time.perf_counter() - t0
@@ -361,7 +365,7 @@ def ogrid_leg(gr, args):
     mixed-precision per-field error at that point."""
     log("bench: O-grid leg")
     m = gr.Mesh.build(elements(gr))
-    m.renumber_rcm()
+    m.renumber_hilbert()
     m.compute_metrics()
     s = m.sizes()
     res = {"workload": "cylinder O-grid 256x128x4 hexes (131,072 cells), M 0.3, second order + reference limiter",
@@ -471,7 +475,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "transonic swept wing: NACA0012 section, 30 deg sweep, taper 0.56, semispan 1.5 "
                                "chords, 20 prism layers (growth 1.2) + tetrahedra in a half sphere of 25 chords, "
-                               "lengths x0.01; M 0.80, AoA 2.5 deg, second order + reference limiter; mixed "
+                               "lengths x0.01, cells in Hilbert-curve order; M 0.80, AoA 2.5 deg, second order + reference limiter; mixed "
                                "precision (binary32 state/gradient/limiter/flux, binary64 metrics, LSQ inverse "
                                "and residual norms)",
                    "cells": s.n_cells, "ghosts": s.n_ghosts, "faces": s.n_faces, "cfl": args.cfl,

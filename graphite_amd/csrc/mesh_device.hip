@@ -580,11 +580,22 @@ int gr_mesh_get_metrics(const gr_mesh *m, double *face_centers, double *face_nor
 // ---- bandwidth probe -------------------------------------------------------------------------
 
 namespace gr {
-// 16-byte vector copy, grid-stride.
+// 16-byte vector copy, grid-stride, four loads in flight per thread before their stores (the
+// main loop runs only while all four of a thread's elements are in range; the tail copies one
+// at a time).
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const double2 *__restrict__ src, double2 *__restrict__ dst,
                                                        size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    constexpr int U = 4;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * stride < n; i += U * stride) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dst[i + u * stride] = v[u];
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
 }
 }  // namespace gr
 

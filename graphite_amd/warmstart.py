@@ -33,11 +33,15 @@ def sample_instances(n: int, seed: int) -> np.ndarray:
     return np.stack([M * np.cos(a), M * np.sin(a), np.zeros(n)], axis=1)
 
 
-def make_batch(mesh: Mesh, machs, dtype: int = GR_F64, cfl: float = 0.5) -> Batch:
+def make_batch(mesh: Mesh, machs, dtype: int = GR_F64, cfl: float = 0.5, tune: bool = False) -> Batch:
+    """A batch of the C-grid flows at freestream; with tune, its launch schedule is chosen by
+    the autotuner (results do not depend on it)."""
     b = Batch(mesh, len(machs), dtype=dtype, cfl=cfl)
     for name, kind in CGRID_BCS:
         b.set_bc(name, kind)
     b.set_freestream(machs, P_INF, T_INF)
+    if tune:
+        b.autotune(10)
     return b
 
 
@@ -182,3 +186,51 @@ def force_agreement(cold: Batch, warm: Batch, machs, region: str, s_ref: float, 
             "mean_cl_rel_diff_p50_max": [float(np.median(d_cl)), float(d_cl.max())],
             "mean_cd_rel_diff_p50_max": [float(np.median(d_cd)), float(d_cd.max())],
             "cold_cl_cycle_amplitude_p50_max": [float(np.median(amp)), float(amp.max())]}
+
+
+def force_history(b: Batch, machs, region: str, s_ref: float, iters: int, every: int, log=None, tag: str = ""):
+    """Run the batch `iters` further iterations (frozen flows stay frozen: iterate_masked with no
+    tolerance runs every flow), sampling lift and drag every `every` iterations.  A flow that
+    diverges restarts from freestream.  Returns (samples [n_samples, n, 2], seconds of solver
+    time, restarted flags)."""
+    from . import lift_drag
+    n = len(machs)
+    out, secs = [], 0.0
+    restarted = np.zeros(n, bool)
+    done = 0
+    while done < iters:
+        step = min(every, iters - done)
+        t0 = time.perf_counter()
+        div = b.iterate_masked(step)[2]
+        secs += time.perf_counter() - t0
+        if div.any():
+            restarted |= div
+            b.reset_freestream(div)
+        done += step
+        F = b.force(region)
+        out.append([lift_drag(F[k], machs[k], RHO_INF, P_INF, s_ref) for k in range(n)])
+        if log and done % 20000 == 0:
+            log(f"force history{tag}: +{done}")
+    return np.array(out), secs, restarted
+
+
+def time_to_forces(samples: np.ndarray, ref: np.ndarray, every: int, window: int, tol: float,
+                   floor=(0.05, 0.005)) -> np.ndarray:
+    """Iterations until a flow's lift and drag, averaged over the trailing `window` iterations,
+    are within `tol` of ref [n, 2] relative to max(|ref|, floor) (the floors keep the test
+    meaningful for flows with almost no lift) and stay so to the end of the samples; -1 where
+    that never happens.  samples [n_samples, n, 2] every `every` iterations from the start."""
+    w = max(1, window // every)
+    ns, n, _ = samples.shape
+    csum = np.cumsum(np.concatenate([np.zeros((1, n, 2)), samples]), axis=0)
+    means = (csum[w:] - csum[:-w]) / w                    # mean over samples j-w+1..j, j >= w-1
+    scale = np.maximum(np.abs(ref), np.asarray(floor, float)[None, :])
+    rel = np.abs(means - ref[None]) / scale[None]
+    ok = (rel <= tol).all(axis=2)                          # [ns - w + 1, n]
+    out = np.full(n, -1, int)
+    for k in range(n):
+        bad = np.nonzero(~ok[:, k])[0]
+        first = 0 if len(bad) == 0 else bad[-1] + 1       # from here on every window agrees
+        if first < ok.shape[0]:
+            out[k] = (first + w) * every
+    return out
