@@ -161,6 +161,7 @@ SIGNATURES = {
     "gr_nsi_step": (_i, [_vp, _i, _dp, C.POINTER(_i)]),
     "gr_nsi_iterate": (_i, [_vp, _i, _d, C.POINTER(_i), _dp]),
     "gr_nsi_history": (_i, [_vp, _dp, C.POINTER(_i), _i]),
+    "gr_nsi_profile": (_i, [_vp, _i, _dp]),
     "gr_mlp_create": (_i, [_u32, _u32, C.c_uint64, C.POINTER(_vp)]),
     "gr_mlp_free": (None, [_vp]),
     "gr_mlp_n_params": (_i, [_vp, C.POINTER(C.c_size_t)]),

@@ -100,3 +100,25 @@ def test_cavity_develops_and_matches_oracle(gr, orc):
     assert rel(p[:N], ost.p[:N]) < 1e-7
     ux = u[:N, 0].reshape(n, n, n)[n // 2, :, n // 2]      # [k, j, i]: x = z = 1/2 column
     assert ux.min() < -0.05 and ux[-1] > 0.3
+
+
+def test_profile_leaves_the_solve_unchanged(gr):
+    """gr_nsi_profile times the step's parts on the current state and restores it: a profile
+    between two runs of steps changes nothing, bit for bit."""
+    print("case: NSI profile between steps, 8^3 cavity, mixed", flush=True)
+    e = gr.Elements.box(8, 8, 8)
+    m = gr.Mesh.build(e)
+    m.compute_metrics()
+    out = []
+    for with_profile in (False, True):
+        ns = gr.Nsi(m, NU, 0.01, gr.GR_F32, cg_max=100, cg_tol=1e-6)
+        ns.set_wall("ymax", (1.0, 0.0, 0.0))
+        ns.step(10)
+        if with_profile:
+            prof = ns.profile(5)
+            assert all(v > 0 for v in prof.values()), prof
+        c, its = ns.step(5)
+        out.append((ns.get_state(), c, its))
+    for x, y in zip(out[0][0], out[1][0]):
+        assert np.array_equal(x, y)
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
