@@ -17,7 +17,8 @@ precision error at the 1e-3 point, the batched C-grid (64 flows per launch), the
 start against cold batched and cold single-flow solves, and the incompressible 64^3 cavity
 solved to its continuity criterion.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--only-headline] [--only-nsi] [--no-cpu] [--no-converge]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--only-headline] [--only-nsi] [--only-mlp] [--no-cpu]
+                  [--no-converge]
                   [--no-ogrid] [--no-batch] [--no-warm] [--no-nsi]
 """
 from __future__ import annotations
@@ -44,6 +45,11 @@ WORKLOAD = dict(n_theta=256, n_r=128, n_z=4, r0=0.5, r1=10.0, ratio=1.05, dz=2 *
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def to_json(obj) -> str:
+    """One JSON line; numpy scalars become Python numbers."""
+    return json.dumps(obj, default=lambda o: o.item() if hasattr(o, "item") else str(o))
 
 
 def elements(gr):
@@ -112,9 +118,7 @@ def batched_leg(gr, args):
         prof = b.profile(20)
         kb = b.kernel_bytes()
         res[name] = {"value": s.n_cells * 64 * steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / steps * 1e3,
-                     "schedule": schedule,
-                     "kernels": {k: {"ms": prof[k], "bytes": kb[k], "GBps": kb[k] / (prof[k] * 1e-3) / 1e9}
-                                 for k in prof}}
+                     "schedule": schedule, "kernels": kernel_table(prof, kb, f"batch_{name}")}
     return res
 
 
@@ -218,11 +222,27 @@ def warm_forces(gr, args, m, net, norm, test, s_ref):
     mean of both coefficients is within 1e-4 (and 1e-3) of it and stays there."""
     from graphite_amd import warmstart as ws
     every, H, W = 100, args.force_horizon, args.force_window
+    dtype = gr.GR_F32 if args.force_mixed else gr.GR_F64
     res = {"horizon_iterations": H, "window_iterations": W, "sample_every": every,
+           "dtype": "mixed" if args.force_mixed else "f64",
            "criterion": "trailing-window mean of C_L and C_D within tol of the cold run's mean over its last window, "
                         "relative to max(|C|, 0.05 for lift, 0.005 for drag), from then on"}
-    cold = ws.make_batch(m, test, tune=True)
-    warm = ws.make_batch(m, test, tune=True)
+    if args.force_train_iters > 0:
+        # a network trained on settled solutions: every training flow run force_train_iters
+        # iterations (no residual stop), then the same training as the leg's network
+        log(f"bench: settled training set ({args.warm_flows} flows x {args.force_train_iters} iterations)")
+        t0 = time.perf_counter()
+        x, t, origin, _ = ws.generate_training_set(m, ws.sample_instances(args.warm_flows, seed=1), tol=1e-30,
+                                                   max_iter=args.force_train_iters, dtype=gr.GR_F32, log=log)
+        t_data = time.perf_counter() - t0
+        net, norm, losses, t_train = ws.train(x, t, origin, steps=args.warm_steps, batch=32768, lr=2e-3,
+                                              lr_final=2e-5)
+        res["settled_training"] = {"flows": args.warm_flows, "iterations": args.force_train_iters,
+                                   "data_seconds": t_data, "train_seconds": t_train,
+                                   "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
+        del x, t
+    cold = ws.make_batch(m, test, dtype, tune=True)
+    warm = ws.make_batch(m, test, dtype, tune=True)
     t0 = time.perf_counter()
     warm.warm_start(net, ws.wall_distance(m), norm)
     t_pred = time.perf_counter() - t0
@@ -247,6 +267,40 @@ def warm_forces(gr, args, m, net, norm, test, s_ref):
     return res
 
 
+def mlp_leg(gr, args):
+    """The warm-start network's BF16-MFMA kernels on their own: training steps (batch 32,768 rows
+    of a resident synthetic set of 2^21 rows) and inference of 2^21 rows, FLOP rates from the
+    padded layer shapes (in 64, hidden 256 x 3, out 32: 2 x (64*256 + 2*256*256 + 256*32) =
+    311,296 FLOP per row forward; a training step is a forward pass, the input-gradient products
+    of layers 4..2 and the weight-gradient products of all four layers: 3x forward less the
+    first layer's input gradient)."""
+    log("bench: MLP leg")
+    rng = np.random.default_rng(0)
+    rows, batch = 1 << 21, 32768
+    x = rng.standard_normal((rows, 6)).astype(np.float32)
+    t = np.tanh(x[:, :5] * 0.5 + 0.1 * x[:, 1:6]).astype(np.float32)
+    net = gr.Mlp(6, 5, seed=0)
+    net.set_data(x, t)
+    net.train(batch, 20, 1e-3, 1e-3)                      # warm-up
+    steps = args.mlp_steps
+    t0 = time.perf_counter()
+    net.train(batch, steps, 1e-3, 1e-4)                   # returns after the losses are copied back
+    dt = time.perf_counter() - t0
+    fwd_row = 2 * (64 * 256 + 2 * 256 * 256 + 256 * 32)
+    train_row = 3 * fwd_row - 2 * 64 * 256
+    t1 = time.perf_counter()
+    net.forward(x)
+    dt_inf = time.perf_counter() - t1
+    peak = 2.5e15
+    return {"workload": "MLP 6-256-256-256-5 (padded 64-256-256-256-32), bf16 MFMA with fp32 accumulation",
+            "train": {"batch": batch, "steps": steps, "ms_per_step": dt / steps * 1e3,
+                      "TFLOPs": train_row * batch * steps / dt / 1e12,
+                      "frac_of_bf16_dense_peak": train_row * batch * steps / dt / peak},
+            "inference": {"rows": rows, "seconds": dt_inf, "TFLOPs": fwd_row * rows / dt_inf / 1e12,
+                          "frac_of_bf16_dense_peak": fwd_row * rows / dt_inf / peak,
+                          "note": "includes the host-to-device copy of the inputs and the copy back"}}
+
+
 def nsi_leg(gr, args):
     """Lid-driven cavity 64^3 (262,144 hexes), Re 400, lid velocity 1: steps and seconds to a
     continuity ratio of 1e-4 (continuity residual over the largest of the first five steps), in
@@ -269,7 +323,12 @@ def nsi_leg(gr, args):
         steps, ratio = ns.iterate(args.nsi_steps, 1e-4)
         dt = time.perf_counter() - t0
         _, its = ns.history(steps)
+        prof, cgb = ns.profile(20), ns.cg_bytes()
+        kernels = {k: {"ms": v, "bytes": cgb.get(k), "GBps": cgb[k] / (v * 1e-3) / 1e9 if k in cgb else None,
+                       "frac_of_peak": cgb[k] / (v * 1e-3) / 1e9 / HBM_PEAK_GBPS if k in cgb else None}
+                   for k, v in prof.items()}
         res[name] = {"steps": steps, "seconds": dt, "ratio": ratio, "converged": bool(ratio < 1e-4),
+                     "kernels": kernels,
                      "steps_per_second": steps / dt, "cell_updates_per_second": m.n_cells * steps / dt,
                      "cg_iterations_mean": float(np.mean(its)), "cg_iterations_total": int(np.sum(its)),
                      "us_per_cg_iteration": dt / max(1, int(np.sum(its))) * 1e6}
@@ -378,7 +437,7 @@ def ogrid_leg(gr, args):
         kb = sv.kernel_bytes()
         su = 8 if dtype == gr.GR_F64 else 4
         res[name] = {"value": s.n_cells * args.steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / args.steps * 1e3,
-                     "tile_cells": sv.tile_cells, "kernels": kernel_table(prof, kb, f"ogrid_{name}" if name == "f64" else None),
+                     "tile_cells": sv.tile_cells, "kernels": kernel_table(prof, kb, f"ogrid_{name}"),
                      "iteration_bytes_reference_phases": iteration_bytes(s, su),
                      "iteration_GBps_reference_phases": s.n_cells * args.steps / dt * iteration_bytes(s, su) / s.n_cells / 1e9}
     if not args.no_converge:
@@ -413,6 +472,8 @@ def main():
     ap.add_argument("--no-nsi", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the swept-wing headline alone (profiling)")
     ap.add_argument("--only-nsi", action="store_true", help="the cavity leg alone (profiling)")
+    ap.add_argument("--only-mlp", action="store_true", help="the warm-start network's kernels alone (profiling)")
+    ap.add_argument("--mlp-steps", type=int, default=200)
     ap.add_argument("--wing-iters", type=int, default=6000)
     ap.add_argument("--nsi-steps", type=int, default=12000, help="step limit of the cavity solve to its criterion")
     ap.add_argument("--warm-flows", type=int, default=512)
@@ -425,6 +486,9 @@ def main():
                     help="also measure the iterations to settled lift/drag, warm vs cold (long)")
     ap.add_argument("--force-horizon", type=int, default=140000)
     ap.add_argument("--force-window", type=int, default=40000)
+    ap.add_argument("--force-train-iters", type=int, default=0,
+                    help="with --warm-forces: train another network on flows run this many iterations (settled)")
+    ap.add_argument("--force-mixed", action="store_true", help="with --warm-forces: batches in mixed precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -435,7 +499,10 @@ def main():
     if gr.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
     if args.only_nsi:
-        print(json.dumps({"nsi_cavity": nsi_leg(gr, args)}), flush=True)
+        print(to_json({"nsi_cavity": nsi_leg(gr, args)}), flush=True)
+        return
+    if args.only_mlp:
+        print(to_json({"mlp": mlp_leg(gr, args)}), flush=True)
         return
 
     log("bench: building the swept-wing mesh")
@@ -522,13 +589,14 @@ def main():
             out["batched"] = batched_leg(gr, args)
         if not args.no_warm:
             out["warm_start"] = warm_start_leg(gr, args)
+            out["mlp"] = mlp_leg(gr, args)
         if not args.no_nsi:
             out["nsi_cavity"] = nsi_leg(gr, args)
     if not args.no_cpu:
         out["cpu_baseline"] = wing_cpu_baseline(gr, e, mach)
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    print(to_json(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -580,22 +580,13 @@ int gr_mesh_get_metrics(const gr_mesh *m, double *face_centers, double *face_nor
 // ---- bandwidth probe -------------------------------------------------------------------------
 
 namespace gr {
-// 16-byte vector copy, grid-stride, four loads in flight per thread before their stores (the
-// main loop runs only while all four of a thread's elements are in range; the tail copies one
-// at a time).
+// 16-byte vector copy, grid-stride.  Measured on the MI355X over 1 GiB: one element per thread
+// per pass with 65,536 blocks of 256 threads reaches 5.46 TB/s; four loads in flight per thread
+// with 16,384 blocks 4.54, non-temporal loads and stores 5.3, a contiguous chunk per block 5.4.
 __global__ __launch_bounds__(kBlock) void k_copy_probe(const double2 *__restrict__ src, double2 *__restrict__ dst,
                                                        size_t n) {
-    constexpr int U = 4;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + (U - 1) * stride < n; i += U * stride) {
-        double2 v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
-#pragma unroll
-        for (int u = 0; u < U; ++u) dst[i + u * stride] = v[u];
-    }
-    for (; i < n; i += stride) dst[i] = src[i];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 }  // namespace gr
 
@@ -626,7 +617,7 @@ extern "C" int gr_stream_copy_probe(size_t bytes, int reps, double *gbps) {
         GR_HIP(hipMemsetAsync(a.p, 0, bytes, s));
         GR_HIP(hipEventCreate(&e0));
         GR_HIP(hipEventCreate(&e1));
-        const unsigned grid = (unsigned)std::min<size_t>((n + gr::kBlock - 1) / gr::kBlock, 256 * 64);
+        const unsigned grid = (unsigned)std::min<size_t>((n + gr::kBlock - 1) / gr::kBlock, 65536);
         hipLaunchKernelGGL(gr::k_copy_probe, dim3(grid), dim3(gr::kBlock), 0, s, a.p, b.p, n);   // warm-up
         GR_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < reps; ++r)

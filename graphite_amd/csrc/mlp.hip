@@ -162,14 +162,27 @@ __global__ void k_sum_chunks(const float *__restrict__ P, int nchunks, int n, fl
     out[e] = s;
 }
 
-// Column sums of an FP32 [M x N] array over row chunks: P[chunk][N].
-__global__ void k_colsum(const float *__restrict__ Z, int ldz, int M, int N, int per, float *__restrict__ P) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x, chunk = blockIdx.y;
-    if (n >= N) return;
+// Column sums of an FP32 [M x N] array over row chunks of `per` rows: P[chunk][N].  A block
+// covers 64 columns (one per lane) of one chunk; its 4 waves take every fourth row and their sums
+// are added in wave order through LDS (fixed order: deterministic).  1024+ blocks for the
+// bias gradients of a 32,768-row batch, where one thread per column over a 512-row chunk (64
+// blocks) took 123 us per launch, latency-bound.
+__global__ __launch_bounds__(kBlock) void k_colsum(const float *__restrict__ Z, int ldz, int M, int N, int per,
+                                                   float *__restrict__ P) {
+    __shared__ float s_w[kBlock / 64][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + lane, chunk = blockIdx.y;
     const int r0 = chunk * per, r1 = min(M, r0 + per);
     float s = 0.f;
-    for (int r = r0; r < r1; ++r) s += Z[(size_t)r * ldz + n];
-    P[(size_t)chunk * N + n] = s;
+    if (n < N)
+        for (int r = r0 + w; r < r1; r += kBlock / 64) s += Z[(size_t)r * ldz + n];
+    s_w[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && n < N) {
+        float t = s_w[0][lane];
+        for (int k = 1; k < kBlock / 64; ++k) t += s_w[k][lane];
+        P[(size_t)chunk * N + n] = t;
+    }
 }
 
 // FP32 host input [M x n_in] -> BF16 [M x kMlpIn], zero padded.
@@ -337,10 +350,14 @@ struct gr_mlp {
         const int n = Kp[l] * Np[l];
         hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(n)), dim3(gr::kBlock), 0, stream, part.p, chunks, n,
                            g.p + woff[l]);
-        hipLaunchKernelGGL(gr::k_colsum, dim3((Np[l] + gr::kBlock - 1) / gr::kBlock, chunks), dim3(gr::kBlock), 0,
-                           stream, d32.p, Np[l], M, Np[l], per, part.p);
-        hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(Np[l])), dim3(gr::kBlock), 0, stream, part.p, chunks,
-                           Np[l], g.p + boff[l]);
+        // bias: column sums of the FP32 output gradient over chunks of 32 rows (more for batches
+        // above 2^19 rows, so that the chunk partials fit the partial buffer)
+        const int max_chunks = gr::kWgradChunks * gr::kMlpW * gr::kMlpW / gr::kMlpW;
+        const int bper = std::max(32, (M + max_chunks - 1) / max_chunks), bchunks = (M + bper - 1) / bper;
+        hipLaunchKernelGGL(gr::k_colsum, dim3((Np[l] + 63) / 64, bchunks), dim3(gr::kBlock), 0, stream, d32.p, Np[l], M,
+                           Np[l], bper, part.p);
+        hipLaunchKernelGGL(gr::k_colsum, dim3((Np[l] + 63) / 64, 1), dim3(gr::kBlock), 0, stream, (const float *)part.p,
+                           Np[l], bchunks, Np[l], bchunks, g.p + boff[l]);
     }
     // loss (into *loss_dst on the device) and gradients for the targets in tgt; no host sync
     void backward(int M, double *loss_dst) {
