@@ -6,8 +6,10 @@
 // Layout: activations row-major [rows][cols] BF16; each layer keeps its weights twice in
 // BF16, W^T [out][in] for the forward product and W [in][out] for the input-gradient product,
 // so every MFMA operand fragment is one contiguous 16-byte load.  The weight gradient
-// X^T * dZ reduces over rows; its tiles are transposed through LDS.  Every reduction runs in
-// a fixed order, so training is deterministic.
+// X^T * dZ reduces over rows: in training, activations and output gradients are also written
+// in strips of eight rows per column (strip_index), where its fragments are contiguous too.
+// The tanh derivative and the bias gradient's first stage run in the input-gradient product's
+// epilogue.  Every reduction runs in a fixed order, so training is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,135 +24,241 @@ namespace gr {
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMlpIn = 64;     // padded input width (n_in <= 64)
 constexpr int kMlpOut = 32;    // padded output width (n_out <= 32)
 constexpr int kMlpW = 256;     // hidden width
-constexpr int kKC = 64;        // K chunk of the dense kernel staged in LDS
-constexpr int kRowsPerBlock = 128;
-constexpr int kWgradChunks = 64;   // row chunks (blocks along rows) of the weight-gradient kernels
+constexpr int kWgradChunks = 32;   // most row chunks (blocks along rows) of a weight-gradient product
 
-__device__ inline bf16x8 zero8() {
-    bf16x8 z;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) z[j] = (bf16)0.0f;
-    return z;
-}
+// Strip layout of an activation or gradient matrix [M x C] (the weight-gradient operands): the
+// eight rows 8b..8b+7 of column c are contiguous, at ((b * C + c) * 8 + r % 8).  A weight
+// gradient reduces over rows, and the MFMA takes eight consecutive k per lane, so each of its
+// operand fragments is one 16-byte load from this layout, with no transposition.
+__device__ inline size_t strip_index(int r, int c, int C) { return ((size_t)(r >> 3) * C + c) * 8 + (r & 7); }
 
-// Y[M x N] = act(X[M x K] * B + bias), B given as BT = B^T [N x K] row-major.  ACT 1: tanh.
-// K % 32 == 0, N == 16*NT.  Block: 4 waves x 32 rows; each wave 2 row groups x NT column
-// tiles of 16x16 MFMA accumulators.  In the 16x16x32 MFMA, lane l holds row/column (l mod 16)
-// of its operand and eight consecutive k of group l/16; the accumulator element j of lane l
-// is row 4*(l/16)+j, column l mod 16.
-template <int ACT, typename OUT, int NT>
-__global__ __launch_bounds__(kBlock) void k_dense(const bf16 *__restrict__ X, int ldx, int M, int K,
-                                                  const bf16 *__restrict__ BT, const float *__restrict__ bias,
-                                                  OUT *__restrict__ Y, int ldy) {
-    constexpr int N = 16 * NT;
-    __shared__ bf16 sB[N][kKC + 8];
+enum { kEpiTanh = 0, kEpiLinear = 1, kEpiDtanh = 2 };
+
+// Y[M x N] = epilogue(X[M x K] * B), B given as BT = B^T [N x K] row-major, K % 32 == 0.  Wave
+// tile 32 rows x 16*TN columns (2 x TN accumulators of the 16x16x32 MFMA: lane l holds
+// row/column l mod 16 of its operand and eight consecutive k of group l/16; accumulator element
+// j of lane l is row 4*(l/16)+j, column l mod 16), operands loaded straight from global memory
+// (the weights stay in L2): the rows four k-steps ahead of the MFMAs, the weights one; a
+// block's 4 waves take the N/(16*TN) column tiles of 4*16*TN/N row groups.  Epilogues:
+//   kEpiTanh:   tanh(acc + bias) -> Y (BF16 row-major) and, if Ys is set, Ys (strip layout);
+//   kEpiLinear: acc + bias -> Y32 (FP32 row-major);
+//   kEpiDtanh:  acc * (1 - h^2) with h the layer output read from Hs (strip layout) -> Y (if
+//               set) and Ys; column sums of the FP32 values over the wave's rows (< M) into
+//               colpart[row0 / 32][N] (the bias gradient's first stage, fixed order).
+// BF16 outputs are staged in LDS so that global stores are 16-byte rows and strips.  Rows at
+// or past M repeat row M-1 (loads stay unconditional, so they pipeline): their row-major
+// entries are not stored, and their strip entries (finite) are masked by every reader.
+template <int EPI, int TN, int K>
+__global__ __launch_bounds__(kBlock) void k_gemm(const bf16 *__restrict__ X, int ldx, int M,
+                                                 const bf16 *__restrict__ BT, int N, const float *__restrict__ bias,
+                                                 const bf16 *__restrict__ Hs, bf16 *__restrict__ Y,
+                                                 bf16 *__restrict__ Ys, float *__restrict__ Y32,
+                                                 float *__restrict__ colpart) {
+    constexpr int TC = 16 * TN;
+    constexpr int S = K / 32, D = S < 4 ? S : 4;   // k-steps; k-steps of A in flight
+    __shared__ __attribute__((aligned(16))) bf16 sT[kBlock / 64][32][TC + 8];   // row-major tile of each wave
+    __shared__ __attribute__((aligned(16))) bf16 sS[kBlock / 64][4][TC][8];     // the same tile in strips
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int row0 = blockIdx.x * kRowsPerBlock + w * 32;
     const int lr = lane & 15, lg = lane >> 4;
-    f32x4 acc[2][NT];
+    const int CT = N / TC, RG = (kBlock / 64) / CT;
+    const int row0 = (blockIdx.x * RG + w / CT) * 32, n0 = (w % CT) * TC;
+    const bf16 *xa[2], *xb[TN];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) xa[r] = X + (size_t)min(row0 + r * 16 + lr, M - 1) * ldx + 8 * lg;
+#pragma unroll
+    for (int t = 0; t < TN; ++t) xb[t] = BT + (size_t)(n0 + t * 16 + lr) * K + 8 * lg;
+    f32x4 acc[2][TN];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < K; kc += kKC) {
-        const int kn = min(kKC, K - kc);
-        __syncthreads();
-        for (int v = threadIdx.x; v < N * (kn / 8); v += blockDim.x) {
-            const int n = v / (kn / 8), c8 = v - n * (kn / 8);
-            *(bf16x8 *)&sB[n][c8 * 8] = *(const bf16x8 *)&BT[(size_t)n * K + kc + c8 * 8];
-        }
-        __syncthreads();
-        for (int ks = 0; ks < kn; ks += 32) {
-            bf16x8 a[2];
+        for (int t = 0; t < TN; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // A fragments (from HBM) D k-steps ahead of the MFMAs, weight fragments (L2) one step ahead;
+    // the scheduling barriers keep the loads where they are written instead of sinking them to
+    // their uses
+    bf16x8 fa[D][2], fb[2][TN];
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int row = row0 + r * 16 + lr;
-                a[r] = row < M ? *(const bf16x8 *)&X[(size_t)row * ldx + kc + ks + 8 * lg] : zero8();
-            }
+    for (int s = 0; s < D; ++s)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const bf16x8 b = *(const bf16x8 *)&sB[t * 16 + lr][ks + 8 * lg];
-                acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b, acc[0][t], 0, 0, 0);
-                acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b, acc[1][t], 0, 0, 0);
-            }
+        for (int r = 0; r < 2; ++r) fa[s][r] = *(const bf16x8 *)(xa[r] + 32 * s);
+#pragma unroll
+    for (int t = 0; t < TN; ++t) fb[0][t] = *(const bf16x8 *)xb[t];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if (s + 1 < S) {
+#pragma unroll
+            for (int t = 0; t < TN; ++t) fb[(s + 1) & 1][t] = *(const bf16x8 *)(xb[t] + 32 * (s + 1));
         }
+        __builtin_amdgcn_sched_barrier(0);
+        const int q = s % D;
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+            acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[q][0], fb[s & 1][t], acc[0][t], 0, 0, 0);
+            acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[q][1], fb[s & 1][t], acc[1][t], 0, 0, 0);
+        }
+        if (s + D < S) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) fa[q][r] = *(const bf16x8 *)(xa[r] + 32 * (s + D));
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (EPI == kEpiLinear) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = t * 16 + lr;
-            const float bb = bias ? bias[col] : 0.f;
+            for (int t = 0; t < TN; ++t) {
+                const int col = n0 + t * 16 + lr;
+                const float bb = bias[col];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int row = row0 + r * 16 + 4 * lg + j;
-                if (row < M) {
-                    float v = acc[r][t][j] + bb;
-                    if (ACT == 1) v = tanhf(v);
-                    Y[(size_t)row * ldy + col] = (OUT)v;
+                for (int j = 0; j < 4; ++j) {
+                    const int row = row0 + r * 16 + 4 * lg + j;
+                    if (row < M) Y32[(size_t)row * N + col] = acc[r][t][j] + bb;
                 }
             }
-        }
-}
-
-// Partial weight gradient P[chunk][K][N] = X[rows of chunk]^T * D[rows of chunk] with
-// X [M x K], D [M x N] (N == 16*NT) BF16; block (kt, chunk) computes rows kt*64..kt*64+63 of
-// the K dimension.  Tiles of 64 rows are written transposed to LDS so that the MFMA
-// fragments (eight consecutive rows) are contiguous.
-template <int NT>
-__global__ __launch_bounds__(kBlock) void k_wgrad(const bf16 *__restrict__ X, int ldx, const bf16 *__restrict__ D,
-                                                  int ldd, int M, int K, int per, float *__restrict__ P) {
-    constexpr int N = 16 * NT;
-    __shared__ bf16 sX[64][64 + 8];   // [k_in local][row local]
-    __shared__ bf16 sD[N][64 + 8];    // [n][row local]
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int lr = lane & 15, lg = lane >> 4;
-    const int kt = blockIdx.x, chunk = blockIdx.y;
-    const int r_begin = chunk * per, r_end = min(M, r_begin + per);
-    f32x4 acc[NT];
+    } else {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int r0 = r_begin; r0 < r_end; r0 += 64) {
+        for (int t = 0; t < TN; ++t) {
+            const int col = n0 + t * 16 + lr;
+            float csum = 0.f;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int R = row0 + r * 16 + 4 * lg;   // first of this lane's 4 rows (a multiple of 4)
+                float v[4];
+                if constexpr (EPI == kEpiTanh) {
+                    const float bb = bias[col];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = tanhf(acc[r][t][j] + bb);
+                } else {
+                    const bf16x4 h4 = *(const bf16x4 *)&Hs[strip_index(R, col, N)];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float h = (float)h4[j];
+                        v[j] = acc[r][t][j] * (1.f - h * h);
+                        if (R + j < M) csum += v[j];
+                    }
+                }
+                bf16x4 o;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    o[j] = (bf16)v[j];
+                    sT[w][r * 16 + 4 * lg + j][t * 16 + lr] = o[j];
+                }
+                *(bf16x4 *)&sS[w][2 * r + (lg >> 1)][t * 16 + lr][4 * (lg & 1)] = o;
+            }
+            if constexpr (EPI == kEpiDtanh) {
+                // the four lane groups' sums, (g0 + g1) + (g2 + g3) in every lane
+                csum += __shfl_xor(csum, 16);
+                csum += __shfl_xor(csum, 32);
+                if (lg == 0) colpart[(size_t)(row0 / 32) * N + col] = csum;
+            }
+        }
         __syncthreads();
-        // X rows r0..r0+63, columns kt*64..+63 -> sX transposed (zeros past M)
-        for (int v = threadIdx.x; v < 64 * 8; v += blockDim.x) {
-            const int rr = v >> 3, c8 = v & 7;
-            const int row = r0 + rr;
-            bf16x8 x = row < r_end ? *(const bf16x8 *)&X[(size_t)row * ldx + kt * 64 + c8 * 8] : zero8();
-#pragma unroll
-            for (int j = 0; j < 8; ++j) sX[c8 * 8 + j][rr] = x[j];
+        if (Y) {
+            for (int e = lane; e < 32 * (TC / 8); e += 64) {
+                const int row = e / (TC / 8), c8 = e - row * (TC / 8);
+                if (row0 + row < M)
+                    *(bf16x8 *)&Y[(size_t)(row0 + row) * N + n0 + c8 * 8] = *(const bf16x8 *)&sT[w][row][c8 * 8];
+            }
         }
-        for (int v = threadIdx.x; v < 64 * (N / 8); v += blockDim.x) {
-            const int rr = v / (N / 8), c8 = v - rr * (N / 8);
-            const int row = r0 + rr;
-            bf16x8 d = row < r_end ? *(const bf16x8 *)&D[(size_t)row * ldd + c8 * 8] : zero8();
-#pragma unroll
-            for (int j = 0; j < 8; ++j) sD[c8 * 8 + j][rr] = d[j];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < 64; ks += 32) {
-            const bf16x8 a = *(const bf16x8 *)&sX[w * 16 + lr][ks + 8 * lg];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const bf16x8 b = *(const bf16x8 *)&sD[t * 16 + lr][ks + 8 * lg];
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        if (Ys) {
+            for (int e = lane; e < 4 * TC; e += 64) {
+                const int b = e / TC, c = e - b * TC;
+                *(bf16x8 *)&Ys[strip_index(row0 + 8 * b, n0 + c, N)] = *(const bf16x8 *)&sS[w][b][c][0];
             }
         }
     }
-    float *Pc = P + (size_t)chunk * K * N;
+}
+
+// Fragments of one 32-row step of the weight gradient: this lane's strip (rows rs..rs+7) of
+// four 16-column groups of A and TN of D; rows at or past r_end set to zero in both.
+template <int TN>
+struct WgradStep {
+    bf16x8 a[4], b[TN];
+    __device__ void load(const bf16 *__restrict__ As, int K, const bf16 *__restrict__ Ds, int N, int rs, int i0,
+                         int n0, int lr) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+        for (int u = 0; u < 4; ++u) a[u] = *(const bf16x8 *)&As[strip_index(rs, i0 + u * 16 + lr, K)];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = kt * 64 + w * 16 + 4 * lg + j;
-            Pc[(size_t)k * N + t * 16 + lr] = acc[t][j];
+        for (int t = 0; t < TN; ++t) b[t] = *(const bf16x8 *)&Ds[strip_index(rs, n0 + t * 16 + lr, N)];
+    }
+    // r0: the step's first row (wave-uniform, so only the step that reaches r_end takes the
+    // masking branch); rs: this lane's strip
+    __device__ void mma(f32x4 (&acc)[4][TN], int r0, int rs, int r_end) {
+        if (r0 + 32 > r_end) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const bool ok = rs + e < r_end;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u][e] = ok ? a[u][e] : (bf16)0.f;
+#pragma unroll
+                for (int t = 0; t < TN; ++t) b[t][e] = ok ? b[t][e] : (bf16)0.f;
+            }
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int t = 0; t < TN; ++t) acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u], b[t], acc[u][t], 0, 0, 0);
+    }
+};
+
+// Partial weight gradient P[chunk][K][N] = A[rows of chunk]^T * D[rows of chunk] from the strip
+// layouts As [M/8][K][8] and Ds [M/8][N][8] (K % 64 == 0, N % (16*TN) == 0, per % 32 == 0).
+// Block (tile, chunk): a 64 x 16*TN tile of the gradient; its 4 waves take the chunk's 32-row
+// steps w, w+4, ..., each loading the next step's fragments before the current step's MFMAs,
+// and add their tiles through LDS in wave order (fixed order: deterministic).  Rows at or past M
+// are masked to zero in both operands.
+template <int TN>
+__global__ __launch_bounds__(kBlock) void k_wgrad(const bf16 *__restrict__ As, int K, const bf16 *__restrict__ Ds,
+                                                  int N, int M, int per, float *__restrict__ P) {
+    constexpr int TC = 16 * TN, W = kBlock / 64;
+    __shared__ f32x4 s_acc[W - 1][4][TN][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int tiles_n = N / TC;
+    const int i0 = (blockIdx.x / tiles_n) * 64, n0 = (blockIdx.x % tiles_n) * TC;
+    const int r_begin = blockIdx.y * per, r_end = min(M, r_begin + per);
+    f32x4 acc[4][TN];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) acc[u][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // each step's loads issued one step ahead (r0 is wave-uniform)
+    WgradStep<TN> cur, nxt;
+    int r0 = r_begin + 32 * w;
+    if (r0 < r_end) cur.load(As, K, Ds, N, r0 + 8 * lg, i0, n0, lr);
+    for (; r0 < r_end; r0 += 32 * W) {
+        const int r1 = r0 + 32 * W;
+        if (r1 < r_end) nxt.load(As, K, Ds, N, r1 + 8 * lg, i0, n0, lr);
+        __builtin_amdgcn_sched_barrier(0);   // the next step's loads go out before these MFMAs
+        cur.mma(acc, r0, r0 + 8 * lg, r_end);
+        cur = nxt;
+    }
+    if (w > 0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int t = 0; t < TN; ++t) s_acc[w - 1][u][t][lane] = acc[u][t];
+    }
+    __syncthreads();
+    if (w == 0) {
+        float *Pc = P + (size_t)blockIdx.y * K * N;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int t = 0; t < TN; ++t) {
+                f32x4 v = acc[u][t];
+#pragma unroll
+                for (int ww = 0; ww < W - 1; ++ww) v += s_acc[ww][u][t][lane];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) Pc[(size_t)(i0 + u * 16 + 4 * lg + j) * N + n0 + t * 16 + lr] = v[j];
+            }
+    }
 }
 
 // out[e] = sum over chunks c (in order) of P[c][e]
@@ -158,15 +266,16 @@ __global__ void k_sum_chunks(const float *__restrict__ P, int nchunks, int n, fl
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     float s = 0.f;
+#pragma unroll 8
     for (int c = 0; c < nchunks; ++c) s += P[(size_t)c * n + e];
     out[e] = s;
 }
 
 // Column sums of an FP32 [M x N] array over row chunks of `per` rows: P[chunk][N].  A block
 // covers 64 columns (one per lane) of one chunk; its 4 waves take every fourth row and their sums
-// are added in wave order through LDS (fixed order: deterministic).  1024+ blocks for the
-// bias gradients of a 32,768-row batch, where one thread per column over a 512-row chunk (64
-// blocks) took 123 us per launch, latency-bound.
+// are added in wave order through LDS (fixed order: deterministic); eight loads in flight per
+// thread.  Latency-bound: one thread per column over a 512-row chunk took 123 us per launch on a
+// 32,768-row batch, and one block over 1,024 rows without the unrolling 60 us.
 __global__ __launch_bounds__(kBlock) void k_colsum(const float *__restrict__ Z, int ldz, int M, int N, int per,
                                                    float *__restrict__ P) {
     __shared__ float s_w[kBlock / 64][64];
@@ -174,8 +283,10 @@ __global__ __launch_bounds__(kBlock) void k_colsum(const float *__restrict__ Z, 
     const int n = blockIdx.x * 64 + lane, chunk = blockIdx.y;
     const int r0 = chunk * per, r1 = min(M, r0 + per);
     float s = 0.f;
-    if (n < N)
+    if (n < N) {
+#pragma unroll 8
         for (int r = r0 + w; r < r1; r += kBlock / 64) s += Z[(size_t)r * ldz + n];
+    }
     s_w[w][lane] = s;
     __syncthreads();
     if (w == 0 && n < N) {
@@ -185,32 +296,39 @@ __global__ __launch_bounds__(kBlock) void k_colsum(const float *__restrict__ Z, 
     }
 }
 
-// FP32 host input [M x n_in] -> BF16 [M x kMlpIn], zero padded.
-__global__ void k_pack_input(const float *__restrict__ x, int M, int n_in, bf16 *__restrict__ X) {
+// FP32 host input [M x n_in] -> BF16 [M x kMlpIn], zero padded, and (if Xs is set) its strip
+// layout.
+__global__ void k_pack_input(const float *__restrict__ x, int M, int n_in, bf16 *__restrict__ X,
+                             bf16 *__restrict__ Xs) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)M * kMlpIn) return;
     const int r = (int)(e / kMlpIn), c = (int)(e % kMlpIn);
-    X[e] = (bf16)(c < n_in ? x[(size_t)r * n_in + c] : 0.f);
+    const bf16 v = (bf16)(c < n_in ? x[(size_t)r * n_in + c] : 0.f);
+    X[e] = v;
+    if (Xs) Xs[strip_index(r, c, kMlpIn)] = v;
 }
 
 // Mini-batch of a resident training set: row r of the batch is row (a*(pos0+r) + b) mod Mall of
 // the set (a coprime to Mall, so one pass of pos over 0..Mall-1 visits every row once).
 __global__ void k_gather_rows(const float *__restrict__ xall, const float *__restrict__ tall, uint64_t Mall, int n_in,
                               int n_out, uint64_t a, uint64_t b, uint64_t pos0, int M, bf16 *__restrict__ X,
-                              float *__restrict__ T) {
+                              bf16 *__restrict__ Xs, float *__restrict__ T) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)M * kMlpIn) return;
     const int r = (int)(e / kMlpIn), c = (int)(e % kMlpIn);
     const uint64_t src = (a * ((pos0 + r) % Mall) + b) % Mall;
-    X[e] = (bf16)(c < n_in ? xall[src * n_in + c] : 0.f);
+    const bf16 v = (bf16)(c < n_in ? xall[src * n_in + c] : 0.f);
+    X[e] = v;
+    Xs[strip_index(r, c, kMlpIn)] = v;
     if (c < n_out) T[(size_t)r * n_out + c] = tall[src * n_out + c];
 }
 
-// Mean squared error: dOut = 2 (out - t) / (M n_out) (BF16 for the GEMMs, FP32 for the bias
-// gradient), padded columns 0; per-block partial sums of the squared error.
+// Mean squared error: dOut = 2 (out - t) / (M n_out) (BF16 row-major and in strips for the
+// GEMMs, FP32 for the bias gradient), padded columns 0; per-block partial sums of the squared
+// error.
 __global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ out, const float *__restrict__ t, int M,
-                                                int n_out, bf16 *__restrict__ d16, float *__restrict__ d32,
-                                                double *__restrict__ partials) {
+                                                int n_out, bf16 *__restrict__ d16, bf16 *__restrict__ d16s,
+                                                float *__restrict__ d32, double *__restrict__ partials) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double se = 0;
     if (e < (size_t)M * kMlpOut) {
@@ -222,6 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ out, c
             g = 2.f * diff / ((float)M * (float)n_out);
         }
         d16[e] = (bf16)g;
+        d16s[strip_index(r, c, kMlpOut)] = (bf16)g;
         d32[e] = g;
     }
     const double bs = block_sum(se);
@@ -235,17 +354,6 @@ __global__ __launch_bounds__(kBlock) void k_loss_final(const double *__restrict_
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
     const double t = block_sum(s);
     if (threadIdx.x == 0) *dst = t / denom;
-}
-
-// dZ = dH * (1 - H^2) for tanh layers (H the layer's BF16 output).
-__global__ void k_dtanh(const float *__restrict__ dH, const bf16 *__restrict__ H, size_t n, bf16 *__restrict__ d16,
-                        float *__restrict__ d32) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const float h = (float)H[e];
-    const float g = dH[e] * (1.f - h * h);
-    d16[e] = (bf16)g;
-    d32[e] = g;
 }
 
 // Adam on the FP32 master parameters.
@@ -289,9 +397,11 @@ struct gr_mlp {
     hipStream_t stream = nullptr;
     gr::DevArray<float> p, m, v, g;
     gr::DevArray<gr::bf16> WT[4], Wb[4];
-    int cap = 0;   // rows of the activation buffers
-    gr::DevArray<float> xin, tgt, out, d32, dH, part;
-    gr::DevArray<gr::bf16> X0, H[3], d16;
+    int cap = 0;   // rows of the activation buffers (a multiple of 4096)
+    gr::DevArray<float> xin, tgt, out, d32, part, colpart;
+    // activations row-major and in strips (X0s, Hs), output-gradient rows ping-pong (d16[2],
+    // row-major and d16s[2] in strips)
+    gr::DevArray<gr::bf16> X0, X0s, H[3], Hs[3], d16[2], d16s[2];
     gr::DevArray<double> lpart, loss;
     gr::DevArray<float> xall, tall;   // resident training set
     uint64_t n_rows = 0;
@@ -310,10 +420,17 @@ struct gr_mlp {
         tgt.alloc((size_t)cap * gr::kMlpOut);
         out.alloc((size_t)cap * gr::kMlpOut);
         X0.alloc((size_t)cap * gr::kMlpIn);
-        for (auto &h : H) h.alloc((size_t)cap * gr::kMlpW);
-        d16.alloc((size_t)cap * gr::kMlpW);
-        d32.alloc((size_t)cap * gr::kMlpW);
-        dH.alloc((size_t)cap * gr::kMlpW);
+        X0s.alloc((size_t)cap * gr::kMlpIn);
+        for (int l = 0; l < 3; ++l) {
+            H[l].alloc((size_t)cap * gr::kMlpW);
+            Hs[l].alloc((size_t)cap * gr::kMlpW);
+        }
+        for (int k = 0; k < 2; ++k) {
+            d16[k].alloc((size_t)cap * gr::kMlpW);
+            d16s[k].alloc((size_t)cap * gr::kMlpW);
+        }
+        d32.alloc((size_t)cap * gr::kMlpOut);
+        colpart.alloc((size_t)cap / 32 * gr::kMlpW);
         part.alloc((size_t)gr::kWgradChunks * gr::kMlpW * gr::kMlpW);
         lpart.alloc((size_t)cap * gr::kMlpOut / gr::kBlock + 1);
     }
@@ -322,66 +439,77 @@ struct gr_mlp {
             hipLaunchKernelGGL(gr::k_pack_weights, dim3(gr::blocks_for((size_t)Kp[l] * Np[l])), dim3(gr::kBlock), 0, s,
                                p.p + woff[l], Kp[l], Np[l], WT[l].p, Wb[l].p);
     }
-    void upload_x(const float *x, int M) {
+    // strips: also write the input's strip layout (training)
+    void upload_x(const float *x, int M, bool strips) {
         GR_HIP(hipMemcpyAsync(xin.p, x, (size_t)M * n_in * sizeof(float), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(gr::k_pack_input, dim3(gr::blocks_for((size_t)M * gr::kMlpIn)), dim3(gr::kBlock), 0, stream,
-                           xin.p, M, (int)n_in, X0.p);
+                           xin.p, M, (int)n_in, X0.p, strips ? X0s.p : nullptr);
     }
-    void forward(int M, hipStream_t s) {
-        const dim3 grid((M + gr::kRowsPerBlock - 1) / gr::kRowsPerBlock), blk(gr::kBlock);
-        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, s, X0.p, gr::kMlpIn, M, Kp[0], WT[0].p,
-                           p.p + boff[0], H[0].p, gr::kMlpW);
-        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, s, H[0].p, gr::kMlpW, M, Kp[1], WT[1].p,
-                           p.p + boff[1], H[1].p, gr::kMlpW);
-        hipLaunchKernelGGL((gr::k_dense<1, gr::bf16, 16>), grid, blk, 0, s, H[1].p, gr::kMlpW, M, Kp[2], WT[2].p,
-                           p.p + boff[2], H[2].p, gr::kMlpW);
-        hipLaunchKernelGGL((gr::k_dense<0, float, 2>), grid, blk, 0, s, H[2].p, gr::kMlpW, M, Kp[3], WT[3].p,
-                           p.p + boff[3], out.p, gr::kMlpOut);
+    static unsigned gemm_blocks(int M, int N, int TN) {
+        const int rows_per_block = 32 * ((gr::kBlock / 64) / (N / (16 * TN)));
+        return (unsigned)((M + rows_per_block - 1) / rows_per_block);
     }
-    // weight and bias gradients of layer l from its input A (BF16, lda) and d16/d32 (ld = Np)
-    template <int NT>
-    void wgrad(int l, const gr::bf16 *A, int lda, int M) {
-        // row chunks of a multiple of 64 rows, at most kWgradChunks of them
-        const int want = std::min(gr::kWgradChunks, (M + 63) / 64);
-        const int per = (((M + want - 1) / want) + 63) / 64 * 64;
+    // strips: also write the hidden layers' strip layouts (training)
+    void forward(int M, hipStream_t s, bool strips) {
+        const dim3 blk(gr::kBlock), gh(gemm_blocks(M, gr::kMlpW, 4)), go(gemm_blocks(M, gr::kMlpOut, 2));
+        const gr::bf16 *in[3] = {X0.p, H[0].p, H[1].p};
+        const int ld[3] = {gr::kMlpIn, gr::kMlpW, gr::kMlpW};
+        for (int l = 0; l < 3; ++l)
+            hipLaunchKernelGGL(l == 0 ? (gr::k_gemm<gr::kEpiTanh, 4, gr::kMlpIn>) : (gr::k_gemm<gr::kEpiTanh, 4, gr::kMlpW>),
+                               gh, blk, 0, s, in[l], ld[l], M, WT[l].p, Np[l], p.p + boff[l], (const gr::bf16 *)nullptr,
+                               H[l].p, strips ? Hs[l].p : nullptr, (float *)nullptr, (float *)nullptr);
+        hipLaunchKernelGGL((gr::k_gemm<gr::kEpiLinear, 2, gr::kMlpW>), go, blk, 0, s, H[2].p, gr::kMlpW, M, WT[3].p,
+                           Np[3], p.p + boff[3], (const gr::bf16 *)nullptr, (gr::bf16 *)nullptr, (gr::bf16 *)nullptr,
+                           out.p, (float *)nullptr);
+    }
+    // column sums of the FP32 rows Z [rows x N] (ld N) into dst, in two fixed-order passes
+    // (about sqrt(rows) chunks of at least 32 rows, into part)
+    void colsum(const float *Z, int rows, int N, float *dst) {
+        const int per = std::max(32, (int)std::ceil(std::sqrt((double)rows))), chunks = (rows + per - 1) / per;
+        hipLaunchKernelGGL(gr::k_colsum, dim3((N + 63) / 64, chunks), dim3(gr::kBlock), 0, stream, Z, N, rows, N, per,
+                           part.p);
+        hipLaunchKernelGGL(gr::k_colsum, dim3((N + 63) / 64, 1), dim3(gr::kBlock), 0, stream, (const float *)part.p, N,
+                           chunks, N, chunks, dst);
+    }
+    // weight gradient of layer l from the strips of its input (As) and output gradient (Ds):
+    // row chunks (multiples of 32 rows; at most kWgradChunks, up to 512 blocks), partials summed
+    // in order
+    template <int TN>
+    void wgrad(int l, const gr::bf16 *As, const gr::bf16 *Ds, int M) {
+        const int tiles = (Kp[l] / 64) * (Np[l] / (16 * TN));
+        const int want = std::max(1, std::min({(M + 127) / 128, 512 / tiles, gr::kWgradChunks}));
+        const int per = ((M + want - 1) / want + 31) / 32 * 32;
         const int chunks = (M + per - 1) / per;
-        hipLaunchKernelGGL((gr::k_wgrad<NT>), dim3(Kp[l] / 64, chunks), dim3(gr::kBlock), 0, stream, A, lda, d16.p,
-                           Np[l], M, Kp[l], per, part.p);
+        hipLaunchKernelGGL((gr::k_wgrad<TN>), dim3(tiles, chunks), dim3(gr::kBlock), 0, stream, As, Kp[l], Ds, Np[l], M,
+                           per, part.p);
         const int n = Kp[l] * Np[l];
         hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(n)), dim3(gr::kBlock), 0, stream, part.p, chunks, n,
                            g.p + woff[l]);
-        // bias: column sums of the FP32 output gradient over chunks of 32 rows (more for batches
-        // above 2^19 rows, so that the chunk partials fit the partial buffer)
-        const int max_chunks = gr::kWgradChunks * gr::kMlpW * gr::kMlpW / gr::kMlpW;
-        const int bper = std::max(32, (M + max_chunks - 1) / max_chunks), bchunks = (M + bper - 1) / bper;
-        hipLaunchKernelGGL(gr::k_colsum, dim3((Np[l] + 63) / 64, bchunks), dim3(gr::kBlock), 0, stream, d32.p, Np[l], M,
-                           Np[l], bper, part.p);
-        hipLaunchKernelGGL(gr::k_colsum, dim3((Np[l] + 63) / 64, 1), dim3(gr::kBlock), 0, stream, (const float *)part.p,
-                           Np[l], bchunks, Np[l], bchunks, g.p + boff[l]);
     }
-    // loss (into *loss_dst on the device) and gradients for the targets in tgt; no host sync
+    // loss (into *loss_dst on the device) and gradients for the targets in tgt (the forward pass
+    // must have written the strips); no host sync
     void backward(int M, double *loss_dst) {
         const size_t no = (size_t)M * gr::kMlpOut;
         const unsigned nb = gr::blocks_for(no);
-        hipLaunchKernelGGL(gr::k_mse, dim3(nb), dim3(gr::kBlock), 0, stream, out.p, tgt.p, M, (int)n_out, d16.p, d32.p,
-                           lpart.p);
-        hipLaunchKernelGGL(gr::k_loss_final, dim3(1), dim3(gr::kBlock), 0, stream, lpart.p, (int)nb,
-                           (double)M * n_out, loss_dst);
-        const dim3 grid((M + gr::kRowsPerBlock - 1) / gr::kRowsPerBlock), blk(gr::kBlock);
-        // layer 4 (linear): dW4 = H3^T dOut, dH3 = dOut W4^T
-        wgrad<2>(3, H[2].p, gr::kMlpW, M);
-        hipLaunchKernelGGL((gr::k_dense<0, float, 16>), grid, blk, 0, stream, d16.p, gr::kMlpOut, M, Np[3], Wb[3].p,
-                           (const float *)nullptr, dH.p, gr::kMlpW);
+        const dim3 blk(gr::kBlock), gh(gemm_blocks(M, gr::kMlpW, 4));
+        hipLaunchKernelGGL(gr::k_mse, dim3(nb), blk, 0, stream, out.p, tgt.p, M, (int)n_out, d16[0].p, d16s[0].p,
+                           d32.p, lpart.p);
+        hipLaunchKernelGGL(gr::k_loss_final, dim3(1), blk, 0, stream, lpart.p, (int)nb, (double)M * n_out, loss_dst);
+        // layer 4 (linear): dW4 = H3^T dOut, db4 = column sums of dOut
+        wgrad<2>(3, Hs[2].p, d16s[0].p, M);
+        colsum(d32.p, M, gr::kMlpOut, g.p + boff[3]);
+        // layers 3..1: dZ_l = (dZ_{l+1} W_{l+1}^T) * (1 - H_l^2) in the input-gradient product's
+        // epilogue (with the first stage of the bias gradient), then dW_l = A_l^T dZ_l
+        int cur = 0;
         for (int l = 2; l >= 0; --l) {
-            const size_t nh = (size_t)M * gr::kMlpW;
-            hipLaunchKernelGGL(gr::k_dtanh, dim3(gr::blocks_for(nh)), dim3(gr::kBlock), 0, stream, dH.p, H[l].p, nh,
-                               d16.p, d32.p);
-            const gr::bf16 *A = l == 0 ? X0.p : H[l - 1].p;
-            const int lda = l == 0 ? gr::kMlpIn : gr::kMlpW;
-            wgrad<16>(l, A, lda, M);
-            if (l > 0)
-                hipLaunchKernelGGL((gr::k_dense<0, float, 16>), grid, blk, 0, stream, d16.p, gr::kMlpW, M, Np[l],
-                                   Wb[l].p, (const float *)nullptr, dH.p, gr::kMlpW);
+            const int nxt = 1 - cur;
+            hipLaunchKernelGGL(l == 2 ? (gr::k_gemm<gr::kEpiDtanh, 4, gr::kMlpOut>) : (gr::k_gemm<gr::kEpiDtanh, 4, gr::kMlpW>),
+                               gh, blk, 0, stream, d16[cur].p, Np[l + 1], M, Wb[l + 1].p, Kp[l + 1],
+                               (const float *)nullptr, (const gr::bf16 *)Hs[l].p, l > 0 ? d16[nxt].p : nullptr,
+                               d16s[nxt].p, (float *)nullptr, colpart.p);
+            wgrad<4>(l, l == 0 ? X0s.p : Hs[l - 1].p, d16s[nxt].p, M);
+            colsum(colpart.p, (M + 31) / 32, gr::kMlpW, g.p + boff[l]);
+            cur = nxt;
         }
         GR_HIP(hipGetLastError());
     }
@@ -412,7 +540,7 @@ void mlp_run_device(gr_mlp *n, size_t M, hipStream_t s, const std::function<void
     for (size_t r0 = 0; r0 < M; r0 += kMlpChunk) {
         const int m = (int)std::min<size_t>(kMlpChunk, M - r0);
         fill(n->X0.p, r0, m);
-        n->forward(m, s);
+        n->forward(m, s, false);
         consume(n->out.p, r0, m);
         GR_HIP(hipGetLastError());
     }
@@ -526,8 +654,8 @@ int gr_mlp_forward(gr_mlp *n, const float *x, int M, float *y) {
         for (int r0 = 0; r0 < M; r0 += gr::kMlpChunk) {
             const int m = std::min(gr::kMlpChunk, M - r0);
             n->ensure(m);
-            n->upload_x(x + (size_t)r0 * n->n_in, m);
-            n->forward(m, n->stream);
+            n->upload_x(x + (size_t)r0 * n->n_in, m, false);
+            n->forward(m, n->stream, false);
             GR_HIP(hipGetLastError());
             o.resize((size_t)m * gr::kMlpOut);
             n->out.download(o.data(), o.size(), n->stream);
@@ -544,9 +672,9 @@ int gr_mlp_loss_grad(gr_mlp *n, const float *x, const float *t, int M, double *l
         require(t && loss, "null argument");
         require(M <= (1 << 20), "at most 2^20 rows per gradient step");
         n->ensure(M);
-        n->upload_x(x, M);
+        n->upload_x(x, M, true);
         GR_HIP(hipMemcpyAsync(n->tgt.p, t, (size_t)M * n->n_out * sizeof(float), hipMemcpyHostToDevice, n->stream));
-        n->forward(M, n->stream);
+        n->forward(M, n->stream, true);
         n->backward(M, n->loss.p);
         *loss = n->host_loss();
         if (grad) {
@@ -563,9 +691,9 @@ int gr_mlp_train_step(gr_mlp *n, const float *x, const float *t, int M, float lr
         require(M <= (1 << 20), "at most 2^20 rows per training step");
         require(lr > 0, "learning rate must be positive");
         n->ensure(M);
-        n->upload_x(x, M);
+        n->upload_x(x, M, true);
         GR_HIP(hipMemcpyAsync(n->tgt.p, t, (size_t)M * n->n_out * sizeof(float), hipMemcpyHostToDevice, n->stream));
-        n->forward(M, n->stream);
+        n->forward(M, n->stream, true);
         n->backward(M, n->loss.p);
         n->adam(lr);
         GR_HIP(hipGetLastError());
@@ -609,8 +737,8 @@ int gr_mlp_train(gr_mlp *n, int batch, int steps, float lr, float lr_final, uint
             }
             hipLaunchKernelGGL(gr::k_gather_rows, dim3(gr::blocks_for((size_t)batch * gr::kMlpIn)), dim3(gr::kBlock), 0,
                                n->stream, n->xall.p, n->tall.p, Mall, (int)n->n_in, (int)n->n_out, a, b,
-                               within * (uint64_t)batch, batch, n->X0.p, n->tgt.p);
-            n->forward(batch, n->stream);
+                               within * (uint64_t)batch, batch, n->X0.p, n->X0s.p, n->tgt.p);
+            n->forward(batch, n->stream, true);
             n->backward(batch, ldev.p + s);
             const double cosf_ = 0.5 * (1.0 + std::cos(M_PI * (double)s / (double)steps));
             n->adam((float)(lr_final + (lr - lr_final) * cosf_));
