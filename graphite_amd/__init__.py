@@ -570,21 +570,27 @@ class Batch:
         _check(load().gr_batch_set_graph_iters(self._h, iters))
 
     def autotune(self, n_iter: int = 10) -> dict:
-        """Chooses the launch schedule (cells per block, flux occupancy cap, graph length) by
-        timed runs; the state is left unchanged.  Returns the choice."""
-        sch = (C.c_int * 3)()
+        """Chooses the launch schedule (cells per block, flux occupancy cap, flux instance
+        groups, graph length) by timed runs; the state is left unchanged.  Returns the choice."""
+        sch = (C.c_int * 4)()
         _check(load().gr_batch_autotune(self._h, n_iter, sch))
-        return {"cells_per_block": sch[0], "flux_waves_cap": bool(sch[1]), "graph_iters": sch[2]}
+        return self._schedule_dict(sch)
+
+    @staticmethod
+    def _schedule_dict(sch) -> dict:
+        return {"cells_per_block": sch[0], "flux_waves_cap": bool(sch[1]), "graph_iters": sch[2],
+                "instance_groups": sch[3]}
 
     @property
     def schedule(self) -> dict:
-        sch = (C.c_int * 3)()
+        sch = (C.c_int * 4)()
         _check(load().gr_batch_get_schedule(self._h, sch))
-        return {"cells_per_block": sch[0], "flux_waves_cap": bool(sch[1]), "graph_iters": sch[2]}
+        return self._schedule_dict(sch)
 
     @schedule.setter
     def schedule(self, s: dict) -> None:
-        sch = (C.c_int * 3)(int(s["cells_per_block"]), int(bool(s["flux_waves_cap"])), int(s["graph_iters"]))
+        sch = (C.c_int * 4)(int(s["cells_per_block"]), int(bool(s["flux_waves_cap"])), int(s["graph_iters"]),
+                            int(s.get("instance_groups", 1)))
         _check(load().gr_batch_set_schedule(self._h, sch))
 
     def set_states(self, rho, rhou, rhoe) -> None:

@@ -136,16 +136,43 @@ __global__ __launch_bounds__(CPB *kBatchLanes) void k_batch_gradlim(MeshView m, 
     }
 }
 
-// The flux + update of one cell per wave, CPB cells per block; block partials of the
-// per-instance residual.
-template <typename S, bool CORR, int CPB>
+// The flux + update, CPB waves per block; block partials of the per-instance residual.
+//
+// G instance groups: the 64 instances are split into G groups of 64/G lanes, and a wave takes G
+// consecutive cells for one group (lane l: cell G*(wave's first) + l/(64/G), instance
+// h*(64/G) + l%(64/G)).  The grid is G sweeps over the cells, one per group, and each XCD's
+// contiguous range of blocks lies within one sweep, so the bytes an XCD loads between a
+// neighbour row's first load and its re-reads by the cell's other neighbours shrink G-fold: with
+// G = 1 a binary64 neighbour costs 12.8 KB and the window of cells in flight on an XCD is larger
+// than its 4 MB L2.  Every lane evaluates the same expressions as with G = 1, so the states are
+// bitwise the same; only the order of the residual's per-instance sum changes.
+template <int CPB, int G>
+__device__ inline void batch_flux_slot(uint32_t N, uint32_t &i, uint32_t &k, uint32_t &h) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if constexpr (G == 1) {
+        i = batch_cell<CPB>();
+        k = lane;
+        h = 0;
+    } else {
+        constexpr uint32_t LG = kBatchLanes / G;
+        const uint32_t T = (N + CPB * G - 1) / (CPB * G);   // blocks per sweep
+        const uint32_t v = xcd_tile(blockIdx.x, gridDim.x);   // gridDim.x == G * T
+        h = v / T;
+        const uint32_t t = v - h * T;
+        i = (t * CPB + w) * G + lane / LG;
+        k = h * LG + lane % LG;
+    }
+}
+
+template <typename S, bool CORR, int CPB, int G>
 __device__ __forceinline__ void batch_flux(const MeshView &m, const BatchArgs<S> &a, const BcTable &bc,
                                            const double *__restrict__ inst_mach, double cfl,
                                            double *__restrict__ partials, const BatchCtl *ctl) {
     if (ctl->stop) return;
     __shared__ double s_err[CPB][kBatchLanes];
-    const uint32_t i = batch_cell<CPB>();
-    const uint32_t k = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t i, k, h;
+    batch_flux_slot<CPB, G>(m.N, i, k, h);
+    const uint32_t w = threadIdx.x >> 6;
     double e_loc = 0;
     if (i < m.N) {
         const size_t st = a.stride, me = (size_t)i * kBatchLanes + k;
@@ -231,30 +258,38 @@ __device__ __forceinline__ void batch_flux(const MeshView &m, const BatchArgs<S>
             e_loc = ((r * r + ((u0 * u0 + u1 * u1) + u2 * u2)) + en * en) * V;
         }
     }
-    s_err[w][k] = e_loc;
+    s_err[w][threadIdx.x & 63] = e_loc;
     __syncthreads();
     if (threadIdx.x < kBatchLanes) {
+        // row of instance sums: instance kk of this block's group from lanes sub*LG + kk%LG of
+        // every wave (waves outer, cells inner: a fixed order); the other groups' entries 0
+        constexpr uint32_t LG = kBatchLanes / G;
+        const uint32_t kk = threadIdx.x;
         double t = 0;
+        if (kk / LG == h) {
 #pragma unroll
-        for (int ww = 0; ww < CPB; ++ww) t += s_err[ww][threadIdx.x];
-        partials[(size_t)blockIdx.x * kBatchLanes + threadIdx.x] = t;
+            for (int ww = 0; ww < CPB; ++ww)
+#pragma unroll
+                for (int sub = 0; sub < G; ++sub) t += s_err[ww][sub * LG + kk % LG];
+        }
+        partials[(size_t)blockIdx.x * kBatchLanes + kk] = t;
     }
 }
 
 // Two occupancies of the same flux: left to the compiler (the binary64 kernel takes 174 VGPRs,
 // 2 waves per SIMD) and capped at 3 waves per SIMD (168 VGPRs, 6 spilled: 360 -> 325 us on the
 // 64-flow C-grid; 4 waves spill 45 and take 580 us).  The autotuner times both.
-template <typename S, bool CORR, int CPB>
+template <typename S, bool CORR, int CPB, int G>
 __global__ __launch_bounds__(CPB *kBatchLanes) void k_batch_flux(MeshView m, BatchArgs<S> a, BcTable bc,
                                                                  const double *__restrict__ inst_mach, double cfl,
                                                                  double *__restrict__ partials, const BatchCtl *ctl) {
-    batch_flux<S, CORR, CPB>(m, a, bc, inst_mach, cfl, partials, ctl);
+    batch_flux<S, CORR, CPB, G>(m, a, bc, inst_mach, cfl, partials, ctl);
 }
-template <typename S, bool CORR, int CPB>
+template <typename S, bool CORR, int CPB, int G>
 __global__ __launch_bounds__(CPB *kBatchLanes) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_batch_flux_w3(
     MeshView m, BatchArgs<S> a, BcTable bc, const double *__restrict__ inst_mach, double cfl,
     double *__restrict__ partials, const BatchCtl *ctl) {
-    batch_flux<S, CORR, CPB>(m, a, bc, inst_mach, cfl, partials, ctl);
+    batch_flux<S, CORR, CPB, G>(m, a, bc, inst_mach, cfl, partials, ctl);
 }
 
 // First stage of the per-instance residual sums: block r adds the flux blocks' partial rows
@@ -501,10 +536,13 @@ struct gr_batch {
     int graph_iters = 10;
     int last_iters = 0;
     // launch schedule (the autotuner's choices): cells (waves) per block of the gradient and
-    // flux kernels, and whether the flux is capped at 3 waves per SIMD
+    // flux kernels, whether the flux is capped at 3 waves per SIMD, and the flux's instance
+    // groups (1, 2 or 4 sweeps over the cells, each with 64/groups lanes per cell)
     int cpb = 4;
     int flux_w3 = 1;
+    int groups = 1;
     static constexpr int kMinCpb = 2;   // partial rows are allocated for the smallest choice
+    static constexpr int kMaxGroups = 4;
 
     ~gr_batch() {
         if (mesh && mesh->stream) (void)hipStreamSynchronize(mesh->stream);
@@ -520,6 +558,24 @@ struct gr_batch {
         }
     }
     uint32_t nblocks() const { return (mesh->h.n_cells + cpb - 1) / cpb; }
+    // A group's row of a cell is 64/groups lanes of the state's width: kept at least one 128-byte
+    // cache line (binary32 rows of 16 lanes half-fill the lines the other groups' sweeps fetch
+    // again: measured 2.5x the flux traffic on the 64-flow C-grid).
+    bool groups_ok(int g) const { return cfg.dtype == GR_F64 ? g <= 4 : g <= 2; }
+    void get_schedule(int schedule[4]) const {
+        schedule[0] = cpb;
+        schedule[1] = flux_w3;
+        schedule[2] = graph_iters;
+        schedule[3] = groups;
+    }
+    // flux grid: groups sweeps of ceil(N / (cpb * groups)) blocks; its partial rows
+    uint32_t flux_blocks() const {
+        const uint32_t per = (uint32_t)(cpb * groups);
+        return (uint32_t)groups * ((mesh->h.n_cells + per - 1) / per);
+    }
+    static size_t partial_rows(uint32_t n_cells) {   // the largest flux grid of any schedule
+        return ((size_t)n_cells + kMinCpb - 1) / kMinCpb + kMaxGroups;
+    }
     template <typename S> gr::BatchArgs<S> args(int b) const {
         return gr::BatchArgs<S>{(const S *)ubuf[b].p, (S *)ubuf[1 - b].p, (S *)grad.p, (S *)lim.p, stride};
     }
@@ -538,18 +594,12 @@ struct gr_batch {
     }
     // Stage 0: gradient+limiter, 1: flux+update, 2: per-instance reduction, of an iteration
     // reading buffer b.
-    template <typename S, int CPB> void launch_cpb(int stage, int b, gr::BatchCtl *c) {
+    template <typename S, int CPB, int G> void launch_flux(int b, const gr::BatchCtl *cc) {
         gr_mesh *m = mesh;
-        const dim3 grid(nblocks()), blk(CPB * gr::kBatchLanes);
-        const gr::BatchCtl *cc = c;
-        if (stage == 0) {
-            hipLaunchKernelGGL((gr::k_batch_gradlim<S, CPB>), grid, blk, 0, m->stream, m->view(), args<S>(b), table(),
-                               inst_mach.p, cfg.limiter, cfg.corrected, cc);
-            return;
-        }
-#define GR_BATCH_FLUX(KERNEL, CORR)                                                                                   \
-    hipLaunchKernelGGL((gr::KERNEL<S, CORR, CPB>), grid, blk, 0, m->stream, m->view(), args<S>(b), table(), inst_mach.p, \
-                       cfg.cfl, partials.p, cc)
+        const dim3 grid(flux_blocks()), blk(CPB * gr::kBatchLanes);
+#define GR_BATCH_FLUX(KERNEL, CORR)                                                                                      \
+    hipLaunchKernelGGL((gr::KERNEL<S, CORR, CPB, G>), grid, blk, 0, m->stream, m->view(), args<S>(b), table(),          \
+                       inst_mach.p, cfg.cfl, partials.p, cc)
         if (flux_w3) {
             if (cfg.corrected) GR_BATCH_FLUX(k_batch_flux_w3, true);
             else GR_BATCH_FLUX(k_batch_flux_w3, false);
@@ -558,6 +608,23 @@ struct gr_batch {
             else GR_BATCH_FLUX(k_batch_flux, false);
         }
 #undef GR_BATCH_FLUX
+    }
+    template <typename S, int CPB> void launch_cpb(int stage, int b, gr::BatchCtl *c) {
+        gr_mesh *m = mesh;
+        const gr::BatchCtl *cc = c;
+        if (stage == 0) {
+            hipLaunchKernelGGL((gr::k_batch_gradlim<S, CPB>), dim3(nblocks()), dim3(CPB * gr::kBatchLanes), 0, m->stream,
+                               m->view(), args<S>(b), table(), inst_mach.p, cfg.limiter, cfg.corrected, cc);
+            return;
+        }
+        if (groups == 2) {
+            launch_flux<S, CPB, 2>(b, cc);
+        } else if constexpr (sizeof(S) == 8) {
+            if (groups == 4) launch_flux<S, CPB, 4>(b, cc);
+            else launch_flux<S, CPB, 1>(b, cc);
+        } else {
+            launch_flux<S, CPB, 1>(b, cc);
+        }
     }
     // Stage 0: gradient+limiter, 1: flux+update, 2: per-instance reduction, of an iteration
     // reading buffer b.
@@ -569,7 +636,7 @@ struct gr_batch {
             else launch_cpb<S, 4>(stage, b, c);
         } else {
             hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream, partials.p,
-                               nblocks(), (const gr::BatchCtl *)c, partials2.p);
+                               flux_blocks(), (const gr::BatchCtl *)c, partials2.p);
             hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,
                                partials2.p, gr::kReduceBlocks, c, hist.p, out.p);
         }
@@ -606,7 +673,7 @@ struct gr_batch {
         const double su = (double)cfg.dtype, I = n_inst;
         out3[0] = I * su * (5 * N + 20 * N) + 8.0 * (3 * (N + G) + 6 * N + 3 * F + 3 * G) + 4.0 * (N + 1 + 2 * slots);
         out3[1] = I * su * (25 * N + 5 * N) + 8.0 * (3 * (N + G) + N + 7 * F) + 4.0 * (N + 1 + 2 * slots);
-        out3[2] = 8.0 * 64 * (nblocks() + 2.0 * gr::kReduceBlocks);
+        out3[2] = 8.0 * 64 * (flux_blocks() + 2.0 * gr::kReduceBlocks);
     }
     // Conservative freestream state of lane k (padding lanes: instance 0).
     void freestream_values(uint32_t k, double val[5]) const {
@@ -735,7 +802,7 @@ struct gr_batch {
         // the iteration's gradient and flux launches (stages 0 and 1) on the current state
         for (int stage = 0; stage < 2; ++stage) enqueue_stage(stage, cur, ctl.p);
         hipLaunchKernelGGL(gr::k_batch_reduce, dim3(gr::kReduceBlocks), dim3(gr::kBlock), 0, m->stream, partials.p,
-                           nblocks(), (const gr::BatchCtl *)nullptr, partials2.p);
+                           flux_blocks(), (const gr::BatchCtl *)nullptr, partials2.p);
         hipLaunchKernelGGL(gr::k_batch_finalize, dim3(1), dim3(gr::kFinalParts * gr::kBatchLanes), 0, m->stream,
                            partials2.p, gr::kReduceBlocks, (gr::BatchCtl *)nullptr, (double *)nullptr, out.p);
         GR_HIP(hipGetLastError());
@@ -841,7 +908,7 @@ int gr_batch_create(gr_mesh *m, uint32_t n_inst, const gr_solver_config *cfg, gr
             b->mach_h.assign(3 * n_inst, 0.0);
             b->region_kind.assign(m->h.n_regions, -1);
             b->ctl.alloc(1);
-            b->partials.alloc(((size_t)m->h.n_cells + gr_batch::kMinCpb - 1) / gr_batch::kMinCpb * gr::kBatchLanes);
+            b->partials.alloc(gr_batch::partial_rows(m->h.n_cells) * gr::kBatchLanes);
             b->partials2.alloc((size_t)gr::kReduceBlocks * gr::kBatchLanes);
             b->out.alloc(gr::kBatchLanes);
             b->lane_flags.alloc(gr::kBatchLanes);
@@ -1010,7 +1077,7 @@ int gr_batch_profile(gr_batch *b, int n_iter, double ms[3]) {
     });
 }
 
-int gr_batch_autotune(gr_batch *b, int n_iter, int schedule[3]) {
+int gr_batch_autotune(gr_batch *b, int n_iter, int schedule[4]) {
     return guarded([&] {
         require(b && schedule, "null argument");
         require(n_iter > 0 && n_iter <= 1000, "n_iter must be in 1..1000");
@@ -1047,32 +1114,39 @@ int gr_batch_autotune(gr_batch *b, int n_iter, int schedule[3]) {
             GR_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
             return (double)t;
         };
-        // 1. cells per block x flux occupancy: gradient + flux launches back to back
+        // 1. cells per block x flux occupancy x instance groups: gradient + flux launches back
+        // to back
         reset_ctl(1 << 30);
         double best = 1e300;
-        int best_cpb = b->cpb, best_w3 = b->flux_w3;
+        int best_cpb = b->cpb, best_w3 = b->flux_w3, best_groups = b->groups;
         for (int cpb : {2, 4, 8}) {
             for (int w3 : {1, 0}) {
-                b->cpb = cpb;
-                b->flux_w3 = w3;
-                b->enqueue_stage(0, b->cur, b->ctl.p);   // first use of the variant, untimed
-                b->enqueue_stage(1, b->cur, b->ctl.p);
-                GR_HIP(hipEventRecord(ev[0], m->stream));
-                for (int k = 0; k < n_iter; ++k) {
-                    b->enqueue_stage(0, b->cur, b->ctl.p);
+                for (int groups : {1, 2, 4}) {
+                    if (!b->groups_ok(groups)) continue;
+                    b->cpb = cpb;
+                    b->flux_w3 = w3;
+                    b->groups = groups;
+                    b->enqueue_stage(0, b->cur, b->ctl.p);   // first use of the variant, untimed
                     b->enqueue_stage(1, b->cur, b->ctl.p);
-                }
-                GR_HIP(hipEventRecord(ev[1], m->stream));
-                const double t = elapsed();
-                if (t < best) {
-                    best = t;
-                    best_cpb = cpb;
-                    best_w3 = w3;
+                    GR_HIP(hipEventRecord(ev[0], m->stream));
+                    for (int k = 0; k < n_iter; ++k) {
+                        b->enqueue_stage(0, b->cur, b->ctl.p);
+                        b->enqueue_stage(1, b->cur, b->ctl.p);
+                    }
+                    GR_HIP(hipEventRecord(ev[1], m->stream));
+                    const double t = elapsed();
+                    if (t < best) {
+                        best = t;
+                        best_cpb = cpb;
+                        best_w3 = w3;
+                        best_groups = groups;
+                    }
                 }
             }
         }
         b->cpb = best_cpb;
         b->flux_w3 = best_w3;
+        b->groups = best_groups;
         // 2. graph length: the same number of iterations replayed as the solve loop replays them
         // (graph, copy of the control block to the host, event)
         const int total = 40;
@@ -1102,30 +1176,29 @@ int gr_batch_autotune(gr_batch *b, int n_iter, int schedule[3]) {
             GR_HIP(hipMemcpyAsync(b->ubuf[k].p, saved[k].p, saved[k].n, hipMemcpyDeviceToDevice, m->stream));
         GR_HIP(hipStreamSynchronize(m->stream));
         b->last_iters = 0;
-        schedule[0] = b->cpb;
-        schedule[1] = b->flux_w3;
-        schedule[2] = b->graph_iters;
+        b->get_schedule(schedule);
     });
 }
 
-int gr_batch_get_schedule(const gr_batch *b, int schedule[3]) {
+int gr_batch_get_schedule(const gr_batch *b, int schedule[4]) {
     return guarded([&] {
         require(b && schedule, "null argument");
-        schedule[0] = b->cpb;
-        schedule[1] = b->flux_w3;
-        schedule[2] = b->graph_iters;
+        b->get_schedule(schedule);
     });
 }
 
-int gr_batch_set_schedule(gr_batch *b, const int schedule[3]) {
+int gr_batch_set_schedule(gr_batch *b, const int schedule[4]) {
     return guarded([&] {
         require(b && schedule, "null argument");
         require(schedule[0] == 2 || schedule[0] == 4 || schedule[0] == 8, "cells per block must be 2, 4 or 8");
         require(schedule[1] == 0 || schedule[1] == 1, "flux occupancy flag must be 0 or 1");
         require(schedule[2] >= 2 && schedule[2] <= 200 && schedule[2] % 2 == 0, "graph length must be even, 2..200");
+        require(schedule[3] == 1 || schedule[3] == 2 || schedule[3] == 4, "instance groups must be 1, 2 or 4");
+        require(b->groups_ok(schedule[3]), "binary32 batches take 1 or 2 instance groups");
         b->cpb = schedule[0];
         b->flux_w3 = schedule[1];
         b->graph_iters = schedule[2];
+        b->groups = schedule[3];
         b->drop_graphs();
     });
 }

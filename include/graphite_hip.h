@@ -395,17 +395,20 @@ int gr_batch_kernel_bytes(const gr_batch *b, double bytes[3]);
 int gr_batch_set_graph_iters(gr_batch *b, int iters);
 /* Launch schedule of the batched kernels: schedule[0] cells (waves) per block of the gradient and
  * flux kernels (2, 4 or 8; default 4), [1] 1 if the flux kernel is capped at 3 waves per SIMD
- * (default 1), [2] iterations per HIP graph (as gr_batch_set_graph_iters).  Results do not
- * depend on it (per-cell arithmetic is the same; only the block grouping of the residual
- * partial sums changes, in the last bits of the residual norms).  The reference launches a
- * fixed CKW of 256 threads (include/datatypes.cuh:19, src/models/euler.cu:662). */
-int gr_batch_get_schedule(const gr_batch *b, int schedule[3]);
-int gr_batch_set_schedule(gr_batch *b, const int schedule[3]);
-/* Chooses the schedule: each (cells per block, flux occupancy) pair times n_iter gradient +
- * flux launches back to back, the fastest is kept; then graph lengths 4, 10, 20, 40 time 40
- * iterations replayed as gr_batch_iterate replays them.  The batch's state is saved and
- * restored.  schedule: the choice, as gr_batch_get_schedule. */
-int gr_batch_autotune(gr_batch *b, int n_iter, int schedule[3]);
+ * (default 1), [2] iterations per HIP graph (as gr_batch_set_graph_iters), [3] instance groups of
+ * the flux kernel (1, 2 or 4 in binary64, 1 or 2 in binary32; default 1): the flux sweeps the
+ * cells once per group of 64/groups instances, a wave taking that many consecutive cells, which
+ * shrinks the neighbour rows an XCD must keep in L2.  Results do not depend on it (per-cell arithmetic is the same; only the block
+ * grouping of the residual partial sums changes, in the last bits of the residual norms).  The
+ * reference launches a fixed CKW of 256 threads (include/datatypes.cuh:19,
+ * src/models/euler.cu:662). */
+int gr_batch_get_schedule(const gr_batch *b, int schedule[4]);
+int gr_batch_set_schedule(gr_batch *b, const int schedule[4]);
+/* Chooses the schedule: each (cells per block, flux occupancy, instance groups) choice times
+ * n_iter gradient + flux launches back to back, the fastest is kept; then graph lengths 4, 10,
+ * 20, 40 time 40 iterations replayed as gr_batch_iterate replays them.  The batch's state is
+ * saved and restored.  schedule: the choice, as gr_batch_get_schedule. */
+int gr_batch_autotune(gr_batch *b, int n_iter, int schedule[4]);
 
 /* ---- post-processing (no reference counterpart) ---------------------------------------------
  * Pressure force on a wall region: F = sum over its faces of p_cell * n * A, n pointing out

@@ -172,9 +172,10 @@ def test_batch_profile_leaves_downloads_unchanged(gr, box):
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_autotuned_and_every_schedule_equal_single_solves(gr, box, dtype):
-    """The launch schedule (cells per block 2/4/8, flux occupancy cap, graph length) changes no
-    cell's arithmetic: after autotuning (which leaves the state unchanged) and under every
-    fixed schedule, each instance equals its single-flow solve bit for bit."""
+    """The launch schedule (cells per block 2/4/8, flux occupancy cap, flux instance groups 1/2/4,
+    graph length) changes no cell's arithmetic: after autotuning (which leaves the state
+    unchanged) and under every fixed schedule, each instance equals its single-flow solve bit for
+    bit."""
     print(f"case: batch schedules vs single solves, {dtype}", flush=True)
     m = box
     dt = gr.GR_F64 if dtype == "f64" else gr.GR_F32
@@ -206,6 +207,8 @@ def test_autotuned_and_every_schedule_equal_single_solves(gr, box, dtype):
     check(b, "autotuned")
     for cpb in (2, 4, 8):
         for cap in (False, True):
-            b = batch()
-            b.schedule = {"cells_per_block": cpb, "flux_waves_cap": cap, "graph_iters": 4}
-            check(b, (cpb, cap))
+            for groups in ((1, 2, 4) if dtype == "f64" else (1, 2)):
+                b = batch()
+                b.schedule = {"cells_per_block": cpb, "flux_waves_cap": cap, "graph_iters": 4,
+                              "instance_groups": groups}
+                check(b, (cpb, cap, groups))
