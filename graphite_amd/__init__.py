@@ -826,26 +826,25 @@ class Nsi:
         return c[:n], np.array(it[:n], dtype=int)
 
     def profile(self, n_iter: int = 20) -> dict:
-        """Mean durations (ms) of the step's parts and CG kernels from back-to-back launches
-        (the solve is unchanged)."""
-        ms = np.zeros(4)
+        """Mean durations (ms) of the step's parts and of one CG iteration from back-to-back
+        launches (the solve is unchanged)."""
+        ms = np.zeros(3)
         _check(load().gr_nsi_profile(self._h, n_iter, _p(ms, C.c_double)))
-        return dict(zip(("predictor_graph", "cg_spmv", "cg_update", "correction_graph"), ms.tolist()))
+        return dict(zip(("predictor_graph", "cg_iteration", "correction_graph"), ms.tolist()))
 
     def cg_bytes(self) -> dict:
-        """Algorithmic bytes of the two kernels of a CG iteration (each array element once per
-        launch): spmv reads z and d, writes d and q, and reads the operator's rows (start, one
-        neighbour index and one coefficient per interior neighbour); update reads x, d, q, r and
-        the diagonal, writes x, r and z (the block partials each block reduces come from L2 and
-        are not counted)."""
+        """Algorithmic bytes of one CG iteration's kernel (each array element once per launch):
+        it reads x, p, r, w, s and the diagonal and writes x, p, r, w, s (the new r, w, s into the
+        other slot), and reads the operator's rows (start, one neighbour index and one
+        coefficient per interior neighbour); the block partials each block reduces come from L2
+        and are not counted."""
         s = self.mesh.sizes()
         N = s.n_cells
         c = self.mesh.connectivity()
         cfs, cc = c["cell_faces_starts"], c["cell_connects"]
         nnz = int((cc[:cfs[N]] < N).sum())
         S = 8 if self.dtype == GR_F64 else 4
-        return {"cg_spmv": float(4 * S * N + 4 * (N + 1) + (4 + S) * nnz),
-                "cg_update": float(7 * S * N + 8 * N)}
+        return {"cg_iteration": float(10 * S * N + 8 * N + 4 * (N + 1) + (4 + S) * nnz)}
 
 
 # ---- case files and VTU output ----------------------------------------------------------------

@@ -202,10 +202,10 @@ __global__ __launch_bounds__(kBlock) void k_nsi_div(MeshView m, const T *__restr
 
 // Scalars of the conjugate-gradient loop (device resident).
 struct CgScalars {
-    double bsum, bnorm, rz, dq, rr, alpha, beta, xsum;   // the latest values (for the host)
+    double bsum, bnorm, gam, del, rr, alpha, beta, xsum;   // the latest values (for the host)
     int done, iters;
-    double rz_k[2];   // r.z of iteration k at [k & 1]: formed by the spmv of iteration k, read by its update
-    int it_k[2];      // k at [k & 1]: updates done before the spmv of iteration k
+    double gam_k[2], alpha_k[2];   // gamma and alpha of iteration k at [k & 1]
+    int it_k[2];                   // k at [k & 1]: iterations done before iteration k
 };
 
 // Fixed-order sum of nb block partials (every thread of the block calls it; the result is valid
@@ -214,6 +214,49 @@ __device__ inline double sum_parts(const double *__restrict__ p, int nb) {
     double s = 0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) s += p[k];
     return block_sum(s);
+}
+
+// The CG kernels run blocks of kCgBlock threads: a quarter of the partials a 256-thread grid
+// leaves, which every block of the next CG launch reads again.
+constexpr int kCgBlock = 1024;
+
+// Sums of three values over a block of kCgBlock threads (valid in thread 0): wave sums by a
+// shuffle tree, then the wave sums in wave order (fixed order: equal inputs give equal bits).
+__device__ inline void cg_block_sum3(double &a, double &b, double &c) {
+    constexpr int NW = kCgBlock / 64;
+    __shared__ double ws[3][NW];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c += __shfl_xor(c, off, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        ws[0][w] = a;
+        ws[1][w] = b;
+        ws[2][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = b = c = 0;
+        for (int k = 0; k < NW; ++k) {
+            a += ws[0][k];
+            b += ws[1][k];
+            c += ws[2][k];
+        }
+    }
+}
+
+// Fixed-order sums of the three partial rows q[0..nb), q[nb..2nb), q[2nb..3nb) (valid in thread 0).
+__device__ inline void cg_sum_parts3(const double *__restrict__ q, int nb, double &a, double &b, double &c) {
+    a = b = c = 0;
+    for (int k = threadIdx.x; k < nb; k += kCgBlock) {
+        a += q[k];
+        b += q[nb + k];
+        c += q[2 * nb + k];
+    }
+    cg_block_sum3(a, b, c);
 }
 
 // Sum of partials in a fixed order into *out (one block).
@@ -226,165 +269,165 @@ __global__ __launch_bounds__(kBlock) void k_nsi_sum(const double *__restrict__ p
     if (threadIdx.x == 0) *out = t;
 }
 
-// r = b - mean(b), x = 0, z = r / diag, d = z (the first iteration's direction: beta = 0 makes
-// the spmv form z + 0 d); partials of r.z and r.r.
+// The preconditioned residual of a cell: r / diag (0 where the cell has no interior neighbour).
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_cg_init(uint32_t N, const S *__restrict__ b, const double *__restrict__ diag,
-                                                    CgScalars *cg, S *__restrict__ x, S *__restrict__ r,
-                                                    S *__restrict__ z, S *__restrict__ d, double *__restrict__ p_rz,
-                                                    double *__restrict__ p_rr) {
+__device__ inline S cg_precond(S r, double diag) {
+    return diag > 0 ? (S)((double)r / diag) : S(0);
+}
+
+// Jacobi-preconditioned conjugate gradients in the Chronopoulos-Gear form, whose two inner
+// products (gamma = r.u, delta = w.u, with u = M^-1 r and w = A u) are formed together, so an
+// iteration needs one grid-wide synchronisation instead of two:
+//   beta_k = gamma_k / gamma_{k-1},  alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1})
+//   (k = 0: beta 0, alpha = gamma / delta),
+//   p = u + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,  u = M^-1 r,  w = A u.
+// In exact arithmetic its iterates are those of the textbook algorithm.  The residual, w and s
+// are double-buffered (slot k & 1 read, the other written): a cell's w = A u needs u of its
+// neighbours after the update, which each block forms on the fly from their r, w and s of the
+// previous iteration (cg_next_u), so no block reads a vector another block has already
+// overwritten.
+
+// u after this iteration's update, from the cell's r, w and s before it.
+template <typename S>
+__device__ inline S cg_next_u(S r, S w, S s_old, double diag, S alpha, S beta, bool first) {
+    const S s = first ? w : w + beta * s_old;
+    return cg_precond<S>(r - alpha * s, diag);
+}
+
+// Set-up: r = b - mean(b), x = 0, u = M^-1 r, w = A u (neighbours' u from b on the fly); block
+// partials of gamma, delta and r.r into q[0 .. 3 nb).
+template <typename S>
+__global__ __launch_bounds__(kCgBlock) void k_cg_init(uint32_t N, const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ col, const S *__restrict__ coef,
+                                                    const S *__restrict__ b, const double *__restrict__ diag,
+                                                    const CgScalars *cg, S *__restrict__ x, S *__restrict__ r,
+                                                    S *__restrict__ w, double *__restrict__ q, int nb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double rz = 0, rr = 0;
+    const S mean = (S)(cg->bsum / (double)N);
+    double g = 0, d = 0, rr = 0;
     if (i < N) {
-        const S ri = b[i] - (S)(cg->bsum / (double)N);
-        const S zi = diag[i] > 0 ? (S)((double)ri / diag[i]) : S(0);
+        const S ri = b[i] - mean;
+        const S ui = cg_precond<S>(ri, diag[i]);
+        S wi = S(0);
+        for (uint32_t k = start[i]; k < start[i + 1]; ++k) {
+            const uint32_t j = col[k];
+            wi += coef[k] * (ui - cg_precond<S>(b[j] - mean, diag[j]));
+        }
         x[i] = S(0);
         r[i] = ri;
-        z[i] = zi;
-        d[i] = zi;
-        rz = (double)ri * (double)zi;
+        w[i] = wi;
+        g = (double)ri * (double)ui;
+        d = (double)wi * (double)ui;
         rr = (double)ri * (double)ri;
     }
-    const double t1 = block_sum(rz);
-    __syncthreads();
-    const double t2 = block_sum(rr);
+    cg_block_sum3(g, d, rr);
     if (threadIdx.x == 0) {
-        p_rz[blockIdx.x] = t1;
-        p_rr[blockIdx.x] = t2;
+        q[blockIdx.x] = g;
+        q[nb + blockIdx.x] = d;
+        q[2 * nb + blockIdx.x] = rr;
     }
 }
 
-// One CG iteration k in two kernels, each of whose blocks first reduces the previous kernel's
-// block partials itself (the same fixed-order sum in every block, so every block takes the same
-// decision; block 0 records the scalars), with no separate one-block reduction kernels and no
-// device-scope fences.  slot = k & 1 (static in a replayed graph of an even number of
-// iterations) selects the double-buffered scalars.  (Reducing in the last block to finish
-// instead needs a device-scope fence in every block, which writes back the XCD's L2: 66 us per
-// iteration on the 64^3 cavity, against 23 us for separate one-block kernels.)
-//
-// spmv: r.z and r.r of the previous update (or of the set-up); stop when |r| < tol |b| or k >=
-// cg_max (k >= 1); beta = r.z / r.z_prev (0 at k = 0); d = z + beta d_old for the cell and, on
-// the fly, for its neighbours (d double-buffered: no block reads a direction another block has
-// already overwritten); q = A d; partials of d.q.
+// After the set-up: |b| (= |r| of the set-up) and the start flags (one block).
+__global__ __launch_bounds__(kBlock) void k_cg_start(const double *__restrict__ q, int nb, CgScalars *cg, int cg_max) {
+    const double rr = sum_parts(q + 2 * nb, nb);
+    if (threadIdx.x == 0) {
+        cg->rr = rr;
+        cg->bnorm = sqrt(rr);
+        cg->iters = 0;
+        cg->beta = 0.0;
+        cg->gam_k[0] = cg->gam_k[1] = 0.0;
+        cg->alpha_k[0] = cg->alpha_k[1] = 0.0;
+        cg->it_k[0] = cg->it_k[1] = 0;
+        cg->done = (cg->bnorm == 0.0 || cg_max == 0) ? 1 : 0;
+    }
+}
+
+// Iteration k (slot = k & 1: static in a replayed graph of an even number of iterations).  Every
+// block first sums the previous launch's partials itself, in the same fixed order as every other
+// block, so all blocks take the same decisions with no one-block reduction kernel and no
+// device-scope fence (block 0 records the scalars): stop when |r| < tol |b| or k >= cg_max
+// (k >= 1), or when the alpha denominator is 0.  Then per cell: p, s, x, r, u and w = A u (the
+// neighbours' u by cg_next_u), and the partials of the next gamma, delta and r.r.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(uint32_t N, const uint32_t *__restrict__ start,
+__global__ __launch_bounds__(kCgBlock) void k_cg_iter(uint32_t N, const uint32_t *__restrict__ start,
                                                     const uint32_t *__restrict__ col, const S *__restrict__ coef,
-                                                    const S *__restrict__ z, const S *__restrict__ d_old,
-                                                    S *__restrict__ d_new,
-                                                    S *__restrict__ q, const double *__restrict__ p_rz,
-                                                    const double *__restrict__ p_rr, double *__restrict__ p_dq, int nb,
-                                                    CgScalars *cg, int slot, double tol, int cg_max) {
+                                                    const double *__restrict__ diag, S *__restrict__ x,
+                                                    S *__restrict__ p, const S *__restrict__ r_in,
+                                                    const S *__restrict__ w_in, const S *__restrict__ s_in,
+                                                    S *__restrict__ r_out, S *__restrict__ w_out,
+                                                    S *__restrict__ s_out, const double *__restrict__ q_in,
+                                                    double *__restrict__ q_out, int nb, CgScalars *cg, int slot,
+                                                    double tol, int cg_max) {
     if (cg->done) return;
-    __shared__ double s_beta;
-    __shared__ int s_stop;
-    const double rz = sum_parts(p_rz, nb);
-    __syncthreads();
-    const double rr = sum_parts(p_rr, nb);
+    __shared__ double s_alpha, s_beta;
+    __shared__ int s_stop, s_first;
+    double gam, del, rr;
+    cg_sum_parts3(q_in, nb, gam, del, rr);
     if (threadIdx.x == 0) {
         const int k = cg->it_k[slot];
-        const bool stop = k > 0 && ((tol > 0 && sqrt(rr) < tol * cg->bnorm) || k >= cg_max);
-        double beta = 0.0;
-        if (k > 0) {
-            const double prev = cg->rz_k[slot ^ 1];
-            beta = prev != 0.0 ? rz / prev : 0.0;
+        bool stop = k > 0 && ((tol > 0 && sqrt(rr) < tol * cg->bnorm) || k >= cg_max);
+        double alpha = 0.0, beta = 0.0;
+        if (!stop) {
+            double den = del;
+            if (k > 0) {
+                const double gprev = cg->gam_k[slot ^ 1];
+                beta = gprev != 0.0 ? gam / gprev : 0.0;
+                den = del - beta * gam / cg->alpha_k[slot ^ 1];
+            }
+            if (den == 0.0) stop = true;
+            else alpha = gam / den;
         }
+        s_alpha = alpha;
         s_beta = beta;
         s_stop = stop ? 1 : 0;
+        s_first = k == 0 ? 1 : 0;
         if (blockIdx.x == 0) {
-            cg->rz_k[slot] = rz;
+            cg->gam_k[slot] = gam;
+            cg->alpha_k[slot] = alpha;
             cg->it_k[slot ^ 1] = k + 1;
             cg->iters = k;
-            cg->rz = rz;
+            cg->gam = gam;
+            cg->del = del;
             cg->rr = rr;
+            cg->alpha = alpha;
             cg->beta = beta;
             if (stop) cg->done = 1;
         }
     }
     __syncthreads();
     if (s_stop) return;
-    const S beta = (S)s_beta;
+    const S al = (S)s_alpha, be = (S)s_beta;
+    const bool first = s_first != 0;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double dq = 0;
+    double g = 0, d = 0, rr1 = 0;
     if (i < N) {
-        const S di = z[i] + beta * d_old[i];
-        S sum = S(0);
+        const double di = diag[i];
+        const S ri = r_in[i], wi = w_in[i];
+        const S ui = cg_precond<S>(ri, di);
+        const S pi = first ? ui : ui + be * p[i];
+        const S si = first ? wi : wi + be * s_in[i];
+        x[i] += al * pi;
+        const S r1 = ri - al * si;
+        const S u1 = cg_precond<S>(r1, di);
+        S w1 = S(0);
         for (uint32_t k = start[i]; k < start[i + 1]; ++k) {
             const uint32_t j = col[k];
-            sum += coef[k] * (di - (z[j] + beta * d_old[j]));
+            w1 += coef[k] * (u1 - cg_next_u<S>(r_in[j], w_in[j], first ? S(0) : s_in[j], diag[j], al, be, first));
         }
-        d_new[i] = di;
-        q[i] = sum;
-        dq = (double)di * (double)sum;
+        p[i] = pi;
+        s_out[i] = si;
+        r_out[i] = r1;
+        w_out[i] = w1;
+        g = (double)r1 * (double)u1;
+        d = (double)w1 * (double)u1;
+        rr1 = (double)r1 * (double)r1;
     }
-    const double t = block_sum(dq);
-    if (threadIdx.x == 0) p_dq[blockIdx.x] = t;
-}
-
-// update: d.q from the spmv's partials, alpha = r.z / d.q (stop when d.q = 0); x += alpha d,
-// r -= alpha q, z = r / diag; partials of r.z and r.r.
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_cg_update(uint32_t N, const double *__restrict__ diag, CgScalars *cg,
-                                                      S *__restrict__ x, S *__restrict__ r, S *__restrict__ z,
-                                                      const S *__restrict__ d, const S *__restrict__ q,
-                                                      const double *__restrict__ p_dq, int nb,
-                                                      double *__restrict__ p_rz, double *__restrict__ p_rr, int slot) {
-    if (cg->done) return;
-    __shared__ double s_alpha;
-    __shared__ int s_stop;
-    const double dqs = sum_parts(p_dq, nb);
+    cg_block_sum3(g, d, rr1);
     if (threadIdx.x == 0) {
-        s_stop = dqs == 0.0 ? 1 : 0;
-        s_alpha = dqs == 0.0 ? 0.0 : cg->rz_k[slot] / dqs;
-        if (blockIdx.x == 0) {
-            cg->dq = dqs;
-            cg->alpha = s_alpha;
-            if (s_stop) cg->done = 1;
-        }
-    }
-    __syncthreads();
-    if (s_stop) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double rz = 0, rr = 0;
-    if (i < N) {
-        const S al = (S)s_alpha;
-        x[i] += al * d[i];
-        const S ri = r[i] - al * q[i];
-        r[i] = ri;
-        const S zi = diag[i] > 0 ? (S)((double)ri / diag[i]) : S(0);
-        z[i] = zi;
-        rz = (double)ri * (double)zi;
-        rr = (double)ri * (double)ri;
-    }
-    const double t1 = block_sum(rz);
-    __syncthreads();
-    const double t2 = block_sum(rr);
-    if (threadIdx.x == 0) {
-        p_rz[blockIdx.x] = t1;
-        p_rr[blockIdx.x] = t2;
-    }
-}
-
-// After init: r.z, |b| and the start flags (one block).
-__global__ __launch_bounds__(kBlock) void k_cg_start(const double *__restrict__ p_rz, const double *__restrict__ p_rr,
-                                                     int nb, CgScalars *cg, int cg_max) {
-    double a = 0, b = 0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-        a += p_rz[k];
-        b += p_rr[k];
-    }
-    const double rz = block_sum(a);
-    __syncthreads();
-    const double rr = block_sum(b);
-    if (threadIdx.x == 0) {
-        cg->rz = rz;
-        cg->rr = rr;
-        cg->bnorm = sqrt(rr);
-        cg->iters = 0;
-        cg->beta = 0.0;
-        cg->rz_k[0] = cg->rz_k[1] = rz;
-        cg->it_k[0] = 0;
-        cg->it_k[1] = 0;
-        cg->done = (cg->bnorm == 0.0 || cg_max == 0) ? 1 : 0;
+        q_out[blockIdx.x] = g;
+        q_out[nb + blockIdx.x] = d;
+        q_out[2 * nb + blockIdx.x] = rr1;
     }
 }
 
@@ -487,10 +530,10 @@ struct gr_nsi {
     gr::DevArray<char> st;                // bulk S: u* or du* (3 blocks), p', grad u (9), grad p (3), grad p' (3)
     gr::DevArray<double> carrier;         // T = binary64: u (3 blocks), p
     gr::DevArray<double> flux;            // T: F, F*
-    gr::DevArray<char> cgv;               // b, d (second buffer), r, z, d, q; x = p' (state block 7)
+    gr::DevArray<char> cgv;               // b, r[2], w[2], s[2], p; x = p' (state block 3)
     gr::DevArray<uint32_t> c0, c1;
     gr::DevArray<double> fgeo;            // nx, ny, nz, D, a (face blocks)
-    gr::DevArray<double> diag, part, part2, part3, cont;   // part/part2: r.z / r.r partials, part3: d.q
+    gr::DevArray<double> diag, part, part2, cont;   // CG partials of slots 0 and 1 (3 per block each)
     gr::DevArray<uint32_t> lap_start, lap_col;             // CG operator rows: interior neighbours
     gr::DevArray<char> lap_coef;                           // their face coefficients, bulk precision
     gr::DevArray<gr::CgScalars> cg;
@@ -531,6 +574,7 @@ struct gr_nsi {
         return gr::NsiFaces{c0.p, c1.p, fgeo.p, fgeo.p + fs, fgeo.p + 2 * fs, fgeo.p + 3 * fs, fgeo.p + 4 * fs};
     }
     unsigned nb() const { return gr::blocks_for(mesh->h.n_cells); }
+    unsigned cg_nb() const { return (mesh->h.n_cells + gr::kCgBlock - 1) / gr::kCgBlock; }
 
     template <typename S, typename T, int NC> void gradient(T *const *vals, S *const *grads) {
         gr::NsiVals<T, NC> u{};
@@ -568,40 +612,43 @@ struct gr_nsi {
                            cgb<S>(0), part.p, part2.p);
         hipLaunchKernelGGL(gr::k_nsi_sum, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)part.p, (int)B,
                            cont.p, 0, (const gr::CgScalars *)cg.p);
-        // CG set-up: mean of b, r, z, d = z, then r.z and |b|
-        S *b = cgb<S>(0), *r = cgb<S>(2), *z = cgb<S>(3), *d = cgb<S>(4);
-        S *x = blk<S>(3);
+        // CG set-up: mean of b, r, x = 0, w = A M^-1 r, the first gamma, delta, r.r; |b|
+        S *b = cgb<S>(0);
         gr::CgScalars *c = cg.p;
-        double *pa = part.p, *pb = part2.p;
+        double *pa = part.p;
         hipLaunchKernelGGL(gr::k_nsi_partial_sum<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)b, pa);
         hipLaunchKernelGGL(gr::k_nsi_sum, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)pa, (int)B,
                            &c->bsum, 0, (const gr::CgScalars *)c);
-        hipLaunchKernelGGL(gr::k_cg_init<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const S *)b,
-                           (const double *)diag.p, c, x, r, z, d, pa, pb);
-        hipLaunchKernelGGL(gr::k_cg_start, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)pa,
-                           (const double *)pb, (int)B, c, cfg.cg_max);
+        const unsigned BC = cg_nb();
+        hipLaunchKernelGGL(gr::k_cg_init<S>, dim3(BC), dim3(gr::kCgBlock), 0, m->stream, N,
+                           (const uint32_t *)lap_start.p, (const uint32_t *)lap_col.p, (const S *)lap_coef.p,
+                           (const S *)b, (const double *)diag.p, (const gr::CgScalars *)c, blk<S>(3), cg_r<S>(0),
+                           cg_w<S>(0), part.p, (int)BC);
+        hipLaunchKernelGGL(gr::k_cg_start, dim3(1), dim3(gr::kBlock), 0, m->stream, (const double *)part.p, (int)BC, c,
+                           cfg.cg_max);
     }
 
-    // kCgChunk CG iterations; every kernel returns at once after convergence.  Iteration k reads
-    // direction buffer k % 2 (block 4, then block 1) and writes the other.
-    template <typename S> void enqueue_cg() {
+    // CG vectors: r, w, s of slot k (iteration k reads slot k & 1 and writes the other), p
+    template <typename S> S *cg_r(int k) const { return cgb<S>(1 + k); }
+    template <typename S> S *cg_w(int k) const { return cgb<S>(3 + k); }
+    template <typename S> S *cg_s(int k) const { return cgb<S>(5 + k); }
+    template <typename S> S *cg_p() const { return cgb<S>(7); }
+    double *cg_q(int k) const { return k ? part2.p : part.p; }
+
+    // CG iteration of slot k (k = 0, 1)
+    template <typename S> void launch_cg_iter(int k, double tol, int cg_max) {
         gr_mesh *m = mesh;
-        const uint32_t N = m->h.n_cells;
-        const unsigned B = nb();
-        S *r = cgb<S>(2), *z = cgb<S>(3), *q = cgb<S>(5);
-        S *dbuf[2] = {cgb<S>(4), cgb<S>(1)};
-        S *x = blk<S>(3);
-        gr::CgScalars *c = cg.p;
-        for (int it = 0; it < kCgChunk; ++it) {
-            S *d_old = dbuf[it & 1], *d_new = dbuf[(it + 1) & 1];
-            hipLaunchKernelGGL(gr::k_cg_spmv<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N,
-                               (const uint32_t *)lap_start.p, (const uint32_t *)lap_col.p, (const S *)lap_coef.p,
-                               (const S *)z, (const S *)d_old, d_new, q, (const double *)part.p,
-                               (const double *)part2.p, part3.p, (int)B, c, it & 1, cfg.cg_tol, cfg.cg_max);
-            hipLaunchKernelGGL(gr::k_cg_update<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N, (const double *)diag.p,
-                               c, x, r, z, (const S *)d_new, (const S *)q, (const double *)part3.p, (int)B, part.p,
-                               part2.p, it & 1);
-        }
+        const unsigned B = cg_nb();
+        hipLaunchKernelGGL(gr::k_cg_iter<S>, dim3(B), dim3(gr::kCgBlock), 0, m->stream, m->h.n_cells,
+                           (const uint32_t *)lap_start.p, (const uint32_t *)lap_col.p, (const S *)lap_coef.p,
+                           (const double *)diag.p, blk<S>(3), cg_p<S>(), (const S *)cg_r<S>(k), (const S *)cg_w<S>(k),
+                           (const S *)cg_s<S>(k), cg_r<S>(k ^ 1), cg_w<S>(k ^ 1), cg_s<S>(k ^ 1),
+                           (const double *)cg_q(k), cg_q(k ^ 1), (int)B, cg.p, k, tol, cg_max);
+    }
+
+    // kCgChunk CG iterations (one kernel each); every kernel returns at once after convergence.
+    template <typename S> void enqueue_cg() {
+        for (int it = 0; it < kCgChunk; ++it) launch_cg_iter<S>(it & 1, cfg.cg_tol, cfg.cg_max);
     }
 
     // Zero mean of p', its ghosts and gradient, and step 5 (corrections of F, u, p; ghosts).
@@ -753,7 +800,7 @@ int gr_nsi_create(gr_mesh *m, const gr_nsi_config *cfg, gr_nsi **out) {
             GR_HIP(hipMemsetAsync(s->carrier.p, 0, s->carrier.n * sizeof(double), m->stream));
             s->flux.alloc(2 * gr::pad64(F));
             GR_HIP(hipMemsetAsync(s->flux.p, 0, s->flux.n * sizeof(double), m->stream));
-            s->cgv.alloc(6 * s->cs * s->es());
+            s->cgv.alloc(8 * s->cs * s->es());
             GR_HIP(hipMemsetAsync(s->cgv.p, 0, s->cgv.n, m->stream));
             // face tables: the lower-numbered cell lists the face first
             std::vector<uint32_t> a0(F, 0xFFFFFFFFu), a1(F, 0xFFFFFFFFu);
@@ -802,9 +849,8 @@ int gr_nsi_create(gr_mesh *m, const gr_nsi_config *cfg, gr_nsi **out) {
                 }
                 GR_HIP(hipStreamSynchronize(m->stream));   // the host start array is freed at the end of this scope
             }
-            s->part.alloc(gr::blocks_for(N) + 1);
-            s->part2.alloc(gr::blocks_for(N) + 1);
-            s->part3.alloc(gr::blocks_for(N) + 1);
+            s->part.alloc(3 * (size_t)gr::blocks_for(N) + 1);
+            s->part2.alloc(3 * (size_t)gr::blocks_for(N) + 1);
             s->cont.alloc(1);
             s->cg.alloc(1);
             GR_HIP(hipMemsetAsync(s->cg.p, 0, sizeof(gr::CgScalars), m->stream));
@@ -904,25 +950,25 @@ int gr_nsi_iterate(gr_nsi *s, int max_steps, double tol, int *steps, double *rat
     });
 }
 
-int gr_nsi_profile(gr_nsi *s, int n_iter, double ms[4]) {
+int gr_nsi_profile(gr_nsi *s, int n_iter, double ms[3]) {
     return guarded([&] {
         require(s && ms, "null argument");
         require(n_iter > 0 && n_iter <= 10000, "n_iter must be in 1..10000");
         gr_mesh *m = s->mesh;
         // Mean durations from HIP events around n_iter back-to-back launches on the library's
         // stream: [0] the predictor graph (ghosts, gradients, predictor, fluxes, divergence, CG
-        // set-up: it writes scratch blocks only), [1] k_cg_spmv, [2] k_cg_update (the CG
-        // kernels on the set-up's vectors with the stopping test disabled), [3] the correction
-        // graph (velocity, pressure and face fluxes are saved before and restored after).  The
-        // next step recomputes every scratch block, so the solve is unchanged.
+        // set-up: it writes scratch blocks only), [1] k_cg_iter (on the set-up's vectors, slot
+        // 0 only, with the stopping test disabled: every launch does its work), [2] the
+        // correction graph (velocity, pressure and face fluxes are saved before and restored
+        // after).  The next step recomputes every scratch block, so the solve is unchanged.
         const bool f64 = s->cfg.dtype == GR_F64;
         if (f64) s->ensure_graphs<double>();
         else s->ensure_graphs<float>();
-        hipEvent_t ev[6] = {};
+        hipEvent_t ev[4] = {};
         struct Guard {
             hipEvent_t *e;
             ~Guard() {
-                for (int k = 0; k < 6; ++k)
+                for (int k = 0; k < 4; ++k)
                     if (e[k]) (void)hipEventDestroy(e[k]);
             }
         } guard{ev};
@@ -935,37 +981,19 @@ int gr_nsi_profile(gr_nsi *s, int n_iter, double ms[4]) {
         GR_HIP(hipEventRecord(ev[0], m->stream));
         for (int k = 0; k < n_iter; ++k) GR_HIP(hipGraphLaunch(s->g_pre, m->stream));
         GR_HIP(hipEventRecord(ev[1], m->stream));
-        const uint32_t N = m->h.n_cells;
-        const unsigned B = s->nb();
-        auto cg_kernels = [&](auto tag) {
-            using S = decltype(tag);
-            S *r = s->cgb<S>(2), *z = s->cgb<S>(3), *q = s->cgb<S>(5), *d0 = s->cgb<S>(4), *d1 = s->cgb<S>(1);
-            S *x = s->blk<S>(3);
-            gr::CgScalars *c = s->cg.p;
-            // tol 0 and an iteration limit no run reaches: every launch does its work (slot 0
-            // only, so the iteration counter stays below the limit)
-            for (int k = 0; k < n_iter; ++k)
-                hipLaunchKernelGGL(gr::k_cg_spmv<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N,
-                                   (const uint32_t *)s->lap_start.p, (const uint32_t *)s->lap_col.p,
-                                   (const S *)s->lap_coef.p, (const S *)z, (const S *)d0, d1, q,
-                                   (const double *)s->part.p, (const double *)s->part2.p, s->part3.p, (int)B, c, 0,
-                                   0.0, 1 << 30);
-            GR_HIP(hipEventRecord(ev[2], m->stream));
-            for (int k = 0; k < n_iter; ++k)
-                hipLaunchKernelGGL(gr::k_cg_update<S>, dim3(B), dim3(gr::kBlock), 0, m->stream, N,
-                                   (const double *)s->diag.p, c, x, r, z, (const S *)d1, (const S *)q,
-                                   (const double *)s->part3.p, (int)B, s->part.p, s->part2.p, 0);
-            GR_HIP(hipEventRecord(ev[3], m->stream));
-        };
-        if (f64) cg_kernels(double());
-        else cg_kernels(float());
+        // tol 0 and an iteration limit no run reaches; slot 0 only keeps the counter at 0
+        for (int k = 0; k < n_iter; ++k) {
+            if (f64) s->launch_cg_iter<double>(0, 0.0, 1 << 30);
+            else s->launch_cg_iter<float>(0, 0.0, 1 << 30);
+        }
+        GR_HIP(hipEventRecord(ev[2], m->stream));
         for (int k = 0; k < n_iter; ++k) GR_HIP(hipGraphLaunch(s->g_post, m->stream));
-        GR_HIP(hipEventRecord(ev[4], m->stream));
+        GR_HIP(hipEventRecord(ev[3], m->stream));
         GR_HIP(hipMemcpyAsync(s->carrier.p, saved_u.p, s->carrier.n * sizeof(double), hipMemcpyDeviceToDevice, m->stream));
         GR_HIP(hipMemcpyAsync(s->flux.p, saved_f.p, s->flux.n * sizeof(double), hipMemcpyDeviceToDevice, m->stream));
         GR_HIP(hipGetLastError());
         GR_HIP(hipStreamSynchronize(m->stream));
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 3; ++k) {
             float t = 0;
             GR_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
             ms[k] = t / n_iter;

@@ -477,10 +477,10 @@ int gr_nsi_iterate(gr_nsi *s, int max_steps, double tol, int *steps, double *rat
 /* The last n steps' continuity residuals and CG iterations. */
 int gr_nsi_history(const gr_nsi *s, double *cont, int *cg_iters, int n);
 /* Mean durations (ms) from HIP events around n_iter back-to-back launches: [0] the predictor
- * graph (ghosts, gradients, predictor, fluxes, divergence, CG set-up), [1] k_cg_spmv, [2]
- * k_cg_update (a CG iteration is these two kernels), [3] the correction graph.  The solve is
- * left unchanged (velocity, pressure and face fluxes are restored). */
-int gr_nsi_profile(gr_nsi *s, int n_iter, double ms[4]);
+ * graph (ghosts, gradients, predictor, fluxes, divergence, CG set-up), [1] k_cg_iter (one CG
+ * iteration), [2] the correction graph.  The solve is left unchanged (velocity, pressure and
+ * face fluxes are restored). */
+int gr_nsi_profile(gr_nsi *s, int n_iter, double ms[3]);
 
 /* ---- warm-start network (no reference counterpart) -------------------------------------------
  * A 4-layer MLP n_in -> 256 -> 256 -> 256 -> n_out (tanh on the hidden layers) trained and run

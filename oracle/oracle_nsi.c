@@ -15,8 +15,9 @@
  *   2. face flux  F*_f = A (avg(u*) . n^ + dt (avg(grad p) . n^ - (p_c1 - p_c0)/D)) (interior),
  *                 F*_f = A u_wall . n^ (boundary).
  *   3. Poisson    sum_f a (p'_i - p'_j) = -(1/dt) sum_f F*_out  (interior faces; Neumann walls),
- *      Jacobi-preconditioned conjugate gradients from p' = 0, right-hand side and solution
- *      shifted to zero mean (the Neumann problem fixes p' up to a constant).
+ *      Jacobi-preconditioned conjugate gradients (Chronopoulos-Gear recurrences) from p' = 0,
+ *      right-hand side and solution shifted to zero mean (the Neumann problem fixes p' up to a
+ *      constant).
  *   4. correct    F = F* - dt a (p'_c1 - p'_c0),  u = u* - dt grad p',  p = p + p'.
  *   continuity residual = sum_i |sum_f F*_out| (the divergence the projection removes). */
 #include <math.h>
@@ -119,52 +120,65 @@ static void remove_mean(double *x, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) x[i] -= s;
 }
 
+/* Jacobi-preconditioned CG in the Chronopoulos-Gear form (the device's k_cg_iter): with u = M^-1 r
+ * and w = A u, gamma = r.u and delta = w.u are formed together after each update, and
+ *   beta = gamma / gamma_prev,  alpha = gamma / (delta - beta gamma / alpha_prev)
+ *   (first iteration: beta 0, alpha = gamma / delta),
+ *   p = u + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s.
+ * An iteration is one x update; the loop stops before iteration k >= 1 once |r| < cg_tol |b| or
+ * k = cg_max, or when the alpha denominator is 0.  Returns the iterations done. */
 int orc_nsi_poisson(const orc_mesh *m, const double *b_in, double *x, int cg_max, double cg_tol) {
     const uint32_t N = m->n_cells;
     faceinfo fi = face_info(m);
-    double *b = malloc(N * sizeof(double)), *r = malloc(N * sizeof(double)), *z = malloc(N * sizeof(double));
-    double *d = malloc(N * sizeof(double)), *q = malloc(N * sizeof(double)), *dg = malloc(N * sizeof(double));
-    memcpy(b, b_in, N * sizeof(double));
-    remove_mean(b, N);
+    double *r = malloc(N * sizeof(double)), *u = calloc(N + 1, sizeof(double)), *w = malloc(N * sizeof(double));
+    double *p = malloc(N * sizeof(double)), *s = malloc(N * sizeof(double)), *dg = malloc(N * sizeof(double));
+    memcpy(r, b_in, N * sizeof(double));
+    remove_mean(r, N);
     for (uint32_t i = 0; i < N; ++i) {
-        double s = 0;
+        double a = 0;
         for (uint32_t k = m->cell_faces_starts[i]; k < m->cell_faces_starts[i + 1]; ++k)
-            if (m->cell_connects[k] < N) s += fi.a[m->cell_faces[k]];
-        dg[i] = s;
+            if (m->cell_connects[k] < N) a += fi.a[m->cell_faces[k]];
+        dg[i] = a;
         x[i] = 0;
-        r[i] = b[i];
-        z[i] = s > 0 ? r[i] / s : 0.0;
-        d[i] = z[i];
+        u[i] = a > 0 ? r[i] / a : 0.0;
     }
-    const double bb = sqrt(dotN(b, b, N));
-    double rz = dotN(r, z, N);
+    apply_A(m, &fi, u, w);
+    double gam = dotN(r, u, N), del = dotN(w, u, N), rr = dotN(r, r, N);
+    const double bb = sqrt(rr);
+    double gam_prev = 0, alpha_prev = 0;
     int it = 0;
-    for (; it < cg_max; ++it) {
-        if (bb == 0) break;
-        apply_A(m, &fi, d, q);
-        const double dq = dotN(d, q, N);
-        if (dq == 0) break;
-        const double alpha = rz / dq;
-        for (uint32_t i = 0; i < N; ++i) {
-            x[i] += alpha * d[i];
-            r[i] -= alpha * q[i];
-        }
-        if (cg_tol > 0 && sqrt(dotN(r, r, N)) < cg_tol * bb) {
+    if (bb > 0 && cg_max > 0) {
+        for (;;) {
+            if (it > 0 && ((cg_tol > 0 && sqrt(rr) < cg_tol * bb) || it >= cg_max)) break;
+            double beta = 0, den = del;
+            if (it > 0) {
+                beta = gam_prev != 0 ? gam / gam_prev : 0.0;
+                den = del - beta * gam / alpha_prev;
+            }
+            if (den == 0) break;
+            const double alpha = gam / den;
+            for (uint32_t i = 0; i < N; ++i) {
+                p[i] = it == 0 ? u[i] : u[i] + beta * p[i];
+                s[i] = it == 0 ? w[i] : w[i] + beta * s[i];
+                x[i] += alpha * p[i];
+                r[i] -= alpha * s[i];
+                u[i] = dg[i] > 0 ? r[i] / dg[i] : 0.0;
+            }
+            apply_A(m, &fi, u, w);
+            gam_prev = gam;
+            alpha_prev = alpha;
+            gam = dotN(r, u, N);
+            del = dotN(w, u, N);
+            rr = dotN(r, r, N);
             ++it;
-            break;
         }
-        for (uint32_t i = 0; i < N; ++i) z[i] = dg[i] > 0 ? r[i] / dg[i] : 0.0;
-        const double rz1 = dotN(r, z, N);
-        const double beta = rz1 / rz;
-        rz = rz1;
-        for (uint32_t i = 0; i < N; ++i) d[i] = z[i] + beta * d[i];
     }
     remove_mean(x, N);
-    free(b);
     free(r);
-    free(z);
-    free(d);
-    free(q);
+    free(u);
+    free(w);
+    free(p);
+    free(s);
     free(dg);
     face_free(&fi);
     return it;
