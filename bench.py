@@ -262,6 +262,23 @@ def warm_forces(gr, args, m, net, norm, test, s_ref):
                     if both.any() else None,
                     "speedup_per_flow_p50": float(np.median(tc[both] / tw[both])) if both.any() else None}
         log(f"bench: time to forces within {tol:.0e}: {res[key]}")
+    # settled apart or still drifting: the warm-cold difference of the final-window means against
+    # each run's own change between its last two windows (relative to max(|C|, floor), the larger
+    # of lift and drag)
+    wn = W // every
+    if sc.shape[0] >= 2 * wn:
+        scale = np.maximum(np.abs(ref), np.array([0.05, 0.005]))[None]
+        mw, mc_prev, mw_prev = sw[-wn:].mean(0), sc[-2 * wn:-wn].mean(0), sw[-2 * wn:-wn].mean(0)
+        d = (np.abs(mw - ref) / scale[0]).max(1)
+        drift = np.maximum((np.abs(ref - mc_prev) / scale[0]).max(1), (np.abs(mw - mw_prev) / scale[0]).max(1))
+        pct = lambda v: [float(np.median(v)), float(v.max())]
+        res["final_windows"] = {
+            "warm_vs_cold_rel_diff_p50_max": pct(d), "own_drift_p50_max": pct(drift),
+            "flows_apart_by_more_than_1e-3": int((d > 1e-3).sum()),
+            "flows_apart_by_3x_their_drift_and_1e-3": int(((d > 1e-3) & (d > 3 * drift)).sum()),
+            "definition": "d = |warm - cold| of the last-window means, drift = larger change of either run's mean "
+                          "between its last two windows; both relative to max(|C|, 0.05 lift / 0.005 drag)"}
+        log(f"bench: final windows: {res['final_windows']}")
     res["restarted"] = {"cold": int(rc.sum()), "warm": int(rw.sum())}
     res["batch_seconds_per_iteration"] = {"cold": per_iter_c, "warm": per_iter_w}
     return res
