@@ -65,6 +65,53 @@ def test_poisson_manufactured(orc, box):
     assert np.allclose(xs, x.reshape(-1), rtol=0, atol=1e-10)
 
 
+def _box_operator(n):
+    """The Poisson operator of the uniform n^3 box as a dense matrix: (A x)_i = h sum_j (x_i -
+    x_j) over face neighbours j (a_f = A/D = h^2/h); cell order i + n (j + n k)."""
+    h, N = 1.0 / n, n ** 3
+    A = np.zeros((N, N))
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                c = cell(n, i, j, k)
+                for di, dj, dk in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                    ii, jj, kk = i + di, j + dj, k + dk
+                    if 0 <= ii < n and 0 <= jj < n and 0 <= kk < n:
+                        A[c, c] += h
+                        A[c, cell(n, ii, jj, kk)] -= h
+    return A
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3, 7, 12])
+def test_poisson_iterates_equal_textbook_pcg(orc, box, iters):
+    """The oracle's CG (Chronopoulos-Gear recurrences) has, in exact arithmetic, the iterates of
+    the textbook Jacobi-preconditioned CG (one residual-norm product and one search-direction
+    product per iteration); after k iterations from x = 0 both agree to rounding.  A wrong beta
+    or alpha recurrence still converges eventually but departs here within a few iterations."""
+    n, om = box
+    A = _box_operator(n)
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(n ** 3)
+    b -= b.mean()
+    dinv = 1.0 / np.diag(A)
+    x, r = np.zeros_like(b), b.copy()
+    z = dinv * r
+    d, rz = z.copy(), r @ z
+    for _ in range(iters):
+        q = A @ d
+        alpha = rz / (d @ q)
+        x += alpha * d
+        r -= alpha * q
+        z = dinv * r
+        rz1 = r @ z
+        d = z + (rz1 / rz) * d
+        rz = rz1
+    xs, its = orc.nsi_poisson(om, b, iters, 0.0)
+    assert its == iters
+    x -= x.mean()
+    assert np.abs(xs - x).max() <= 1e-11 * np.abs(x).max()
+
+
 def test_projection_is_exact(orc, box):
     """After a step solved to a tight tolerance every cell's net outflow vanishes."""
     n, om = box
