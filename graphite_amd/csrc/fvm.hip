@@ -15,37 +15,14 @@ namespace gr {
 
 __global__ __launch_bounds__(kBlock) void k_finalize(const double *partials, uint32_t n, double *out, SolverCtl *ctl,
                                                      double *hist) {
-    if (ctl && ctl->stop) return;
-    // eight independent loads in flight per thread (a single running sum waits on each load in
-    // turn), the eight sums then combined in a fixed order
-    double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const uint32_t st = blockDim.x;
-    uint32_t k = threadIdx.x;
-    for (; k + 7 * st < n; k += 8 * st)
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a8[u] += partials[k + u * st];
-    for (; k < n; k += st) a8[0] += partials[k];
-    const double acc = ((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]));
-    const double tot = block_sum(acc);
+    if (ctl && !ctl->pending) return;
+    const double tot = block_sum_partials(partials, n);
     if (threadIdx.x != 0) return;
     if (!ctl) {
         if (out) out[0] = tot;
         return;
     }
-    const int it = ctl->iter;
-    if (it < ctl->hist_cap) hist[it] = tot;
-    if (it == 0 && ctl->ref <= 0) ctl->ref = tot;
-    ctl->last = tot;
-    ctl->ratio = tot / ctl->ref;
-    ctl->iter = it + 1;
-    if (!isfinite(tot)) {
-        ctl->diverged = 1;
-        ctl->stop = 1;
-    } else if (ctl->tol > 0 && ctl->ratio < ctl->tol) {
-        ctl->converged = 1;
-        ctl->stop = 1;
-    }
-    if (ctl->iter >= ctl->max_iter) ctl->stop = 1;
+    record_iteration(ctl, hist, tot);
 }
 
 template <typename S>
@@ -791,7 +768,7 @@ struct gr_solver {
     template <typename S> void launch_gradlim(int b, gr::SolverCtl *c) {
         gr_mesh *m = mesh;
         hipLaunchKernelGGL(gr::k_gradlim_tile<S>, dim3(ntiles), dim3(gr::kBlock), 0, m->stream, m->view(), sargs<S>(b),
-                           table(), tiles.p, ntiles, cfg.limiter, cfg.corrected, c);
+                           table(), tiles.p, ntiles, cfg.limiter, cfg.corrected, c, (const double *)partials.p, hist.p);
     }
     template <typename S> void launch_flux(int b, gr::SolverCtl *c) {
         gr_mesh *m = mesh;
@@ -804,6 +781,8 @@ struct gr_solver {
                                tiles.p, ntiles, cfg.cfl, partials.p, c);
     }
     // One iteration reading buffer b; without a control block the caller reduces the partials.
+    // With one, the iteration's residual is summed and recorded by the next iteration's gradient
+    // launch (or by finish_pending after the last).
     void enqueue(int b, gr::SolverCtl *c) {
         if (cfg.dtype == GR_F64) {
             launch_gradlim<double>(b, c);
@@ -812,9 +791,12 @@ struct gr_solver {
             launch_gradlim<float>(b, c);
             launch_flux<float>(b, c);
         }
-        if (c)
-            hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, mesh->stream, partials.p, ntiles,
-                               (double *)nullptr, c, hist.p);
+    }
+    // Sums and records the last iteration's residual if no gradient launch has (k_finalize
+    // returns at once when nothing is pending).
+    void finish_pending(gr::SolverCtl *c) {
+        hipLaunchKernelGGL(gr::k_finalize, dim3(1), dim3(gr::kBlock), 0, mesh->stream, partials.p, ntiles,
+                           (double *)nullptr, c, hist.p);
     }
     // Writes the boundary-condition ghost states of buffer b's interior into buffer b (the
     // iteration kernels evaluate them on the fly and never store them).
@@ -1044,6 +1026,8 @@ int gr_solver_iterate(gr_solver *s, int max_iter, double tol, double reference, 
                 if (s->ctl_host[1 + ((L - 1) & 1)].stop) break;
             }
         }
+        s->finish_pending(s->ctl.p);
+        GR_HIP(hipGetLastError());
         GR_HIP(hipStreamSynchronize(m->stream));
         gr::SolverCtl fin;
         GR_HIP(hipMemcpy(&fin, s->ctl.p, sizeof fin, hipMemcpyDeviceToHost));

@@ -38,6 +38,7 @@ struct SolverCtl {
     int converged;
     int diverged;
     int hist_cap;      // capacity of the history array
+    int pending;       // set by a flux launch: its tile partials await the sum
 };
 
 template <typename S> struct Tol;
@@ -417,8 +418,44 @@ __global__ __launch_bounds__(kBlock) void k_euler_flux(MeshView m, EulerArgs<S> 
     if (threadIdx.x == 0 && partials) partials[blockIdx.x] = bs;
 }
 
+// Sum of n partials by one block in a fixed order (valid in thread 0): eight independent loads
+// in flight per thread (a single running sum waits on each load in turn), the eight sums then
+// combined in a fixed order, then the block sum.
+__device__ inline double block_sum_partials(const double *__restrict__ partials, uint32_t n) {
+    double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t st = blockDim.x;
+    uint32_t k = threadIdx.x;
+    for (; k + 7 * st < n; k += 8 * st)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a8[u] += partials[k + u * st];
+    for (; k < n; k += st) a8[0] += partials[k];
+    const double acc = ((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]));
+    return block_sum(acc);
+}
+
+// Records an iteration's residual sum tot in ctl (one thread): history, normalisation (the
+// first residual unless given), convergence and divergence, the iteration limit; clears pending.
+__device__ inline void record_iteration(SolverCtl *ctl, double *hist, double tot) {
+    const int it = ctl->iter;
+    if (it < ctl->hist_cap) hist[it] = tot;
+    if (it == 0 && ctl->ref <= 0) ctl->ref = tot;
+    ctl->last = tot;
+    ctl->ratio = tot / ctl->ref;
+    ctl->iter = it + 1;
+    ctl->pending = 0;
+    if (!isfinite(tot)) {
+        ctl->diverged = 1;
+        ctl->stop = 1;
+    } else if (ctl->tol > 0 && ctl->ratio < ctl->tol) {
+        ctl->converged = 1;
+        ctl->stop = 1;
+    }
+    if (ctl->iter >= ctl->max_iter) ctl->stop = 1;
+}
+
 // Sum of n partials by one block in a fixed order; result to out[0].  With ctl: records the
-// iteration (history, normalisation, convergence, stop flags) instead of writing out.
+// iteration (history, normalisation, convergence, stop flags) instead of writing out, and only
+// if a flux launch left partials pending.
 __global__ __launch_bounds__(kBlock) void k_finalize(const double *partials, uint32_t n, double *out, SolverCtl *ctl,
                                                      double *hist);
 

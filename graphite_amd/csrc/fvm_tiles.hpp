@@ -312,11 +312,31 @@ __device__ inline void recon_at(const S *__restrict__ gl, uint32_t c, const S u[
 
 // Gradient (LSQ) + neighbour min/max + limiter of the five conservative components
 // (FvmEulerStepPrepare, src/models/euler.cu:597-647).
+//
+// In the solver loop (ctl set) block 0 first finishes the previous iteration: the fixed-order
+// sum of its flux launch's tile partials (FvmFieldReduceSum, src/fvmops.cu:369-383) and the
+// stopping rule (src/models/euler.cu:158-169), which k_finalize would otherwise do in a launch
+// of its own; the flux launch that follows reads the stop flag.  Gradients computed by the
+// other blocks after the solve has stopped are scratch and never used.
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<S> a, BcTable bc,
                                                          const uint32_t *__restrict__ tiles, uint32_t ntiles,
-                                                         int kind, int corr, const SolverCtl *ctl) {
-    if (ctl && ctl->stop) return;
+                                                         int kind, int corr, SolverCtl *ctl,
+                                                         const double *__restrict__ partials, double *hist) {
+    if (ctl) {
+        __shared__ int s_stop;
+        if (blockIdx.x == 0 && ctl->pending) {   // uniform across the block
+            const double tot = block_sum_partials(partials, ntiles);
+            if (threadIdx.x == 0) {
+                record_iteration(ctl, hist, tot);
+                s_stop = ctl->stop;
+            }
+            __syncthreads();
+            if (s_stop) return;
+        } else if (ctl->stop) {
+            return;
+        }
+    }
     // LDS rows per slot column.  Binary32 (kTerms): 0..14 the slot's gradient terms
     // d*(u_j-u_i)/|d|^2, component c direction k in row 3c+k, and 15..19 u_j.  Binary64: 0..4
     // u_j - u_i, 5..7 d = x_j - x_i, 8 1/|d|^2, 9..13 u_j -- 27.8 KB instead of 39.7 KB, five
@@ -485,8 +505,11 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
 template <typename S, bool CORR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 4 : 8, 8))) void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
                                                       const uint32_t *__restrict__ tiles, uint32_t ntiles, double cfl,
-                                                      double *partials, const SolverCtl *ctl) {
-    if (ctl && ctl->stop) return;
+                                                      double *partials, SolverCtl *ctl) {
+    if (ctl) {
+        if (ctl->stop) return;
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->pending = 1;   // partials for the next launch to sum
+    }
     __shared__ uint32_t s_start[kTileCells + 1];
     __shared__ S s_c[5][kTileSlots];   // central flux term ((F0+F1)*ds)*0.5
     __shared__ S s_d[5][kTileSlots];   // dissipation term ((eig*(U1-U0))*0.5)*ds
