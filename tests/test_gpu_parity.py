@@ -222,6 +222,31 @@ def test_solver_iterations_match_oracle(gr, orc, c1, corrected):
     assert normwise(r, s.rho) <= TOL and normwise(u, s.rhou) <= TOL and normwise(e, s.rhoe) <= TOL
 
 
+def test_c1_500_iterations_match_oracle(gr, orc, c1):
+    """C1's full run: 500 local-time-step iterations of the device solver (binary64, HIP graphs,
+    the residual summed in the next iteration's gradient launch) from the smooth perturbed field,
+    all sides far field, against the oracle: states and the 500-entry residual history."""
+    print("case: C1 500 iterations, binary64, against the oracle", flush=True)
+    m, om, _ = c1
+    mach = (0.5, 0.0, 0.0)
+    state = perturbed_state(om.cell_centers, mach=mach)
+    bcs = [(nm, "farfield", mach) for nm in m.region_names()]
+    n_iter = 500
+    sv, res = _solver_run(gr, m, gr.GR_F64, bcs, state, n_iter)
+    assert res.iterations == n_iter
+    hist = sv.history(n_iter)
+    s = _oracle_state(orc, om, *state)
+    obcs = [(r, orc.BC_FARFIELD, mach, P_INF, T_INF) for r in range(len(bcs))]
+    n, ohist = orc.iterate(om, s, obcs, 0.5, n_iter, threads=8)
+    assert n == n_iter
+    r, u, e = sv.get_state()
+    errs = (normwise(hist, ohist), normwise(r, s.rho), normwise(u, s.rhou), normwise(e, s.rhoe))
+    print(f"500 iterations: residual history {errs[0]:.1e}, states {max(errs[1:]):.1e} normwise; "
+          f"residual drop {hist[-1] / hist[0]:.2e}", flush=True)
+    assert max(errs) <= TOL
+    assert hist[-1] < hist[0]
+
+
 @pytest.mark.parametrize("mach", [(0.5, 0.1, 0.0), (1.6, 0.0, 0.2)])
 def test_solver_mixed_bcs_and_unset_region_match_oracle(gr, orc, c1, mach):
     # supersonic and subsonic far field, slip walls, and one region without a boundary
