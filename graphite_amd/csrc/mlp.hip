@@ -261,14 +261,35 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const bf16 *__restrict__ As, i
     }
 }
 
-// out[e] = sum over chunks c (in order) of P[c][e]
-__global__ void k_sum_chunks(const float *__restrict__ P, int nchunks, int n, float *__restrict__ out) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+// Up to four independent sums of one kind in one launch (one per layer): blocks
+// [start[j], start[j+1]) belong to job j.
+constexpr int kMaxJobs = 4;
+__device__ inline int job_of(const int *start, int n, int b) {
+    int j = 0;
+    while (j + 1 < n && b >= start[j + 1]) ++j;
+    return j;
+}
+
+// out[e] = sum over chunks c (in order) of P[c][e], e < n: one thread per element.
+struct ChunkSum {
+    const float *P;
+    float *out;
+    int nchunks, n;
+};
+struct ChunkSums {
+    ChunkSum job[kMaxJobs];
+    int start[kMaxJobs + 1];
+    int n;
+};
+__global__ __launch_bounds__(kBlock) void k_sum_chunks(ChunkSums J) {
+    const int j = job_of(J.start, J.n, (int)blockIdx.x);
+    const ChunkSum q = J.job[j];
+    const int e = ((int)blockIdx.x - J.start[j]) * blockDim.x + threadIdx.x;
+    if (e >= q.n) return;
     float s = 0.f;
 #pragma unroll 8
-    for (int c = 0; c < nchunks; ++c) s += P[(size_t)c * n + e];
-    out[e] = s;
+    for (int c = 0; c < q.nchunks; ++c) s += q.P[(size_t)c * q.n + e];
+    q.out[e] = s;
 }
 
 // Column sums of an FP32 [M x N] array over row chunks of `per` rows: P[chunk][N].  A block
@@ -276,11 +297,28 @@ __global__ void k_sum_chunks(const float *__restrict__ P, int nchunks, int n, fl
 // are added in wave order through LDS (fixed order: deterministic); eight loads in flight per
 // thread.  Latency-bound: one thread per column over a 512-row chunk took 123 us per launch on a
 // 32,768-row batch, and one block over 1,024 rows without the unrolling 60 us.
-__global__ __launch_bounds__(kBlock) void k_colsum(const float *__restrict__ Z, int ldz, int M, int N, int per,
-                                                   float *__restrict__ P) {
+// Jobs: Z [M x N] (ld N) summed over chunks of `per` rows into P [chunks x N]; a job's blocks
+// are its (64-column block, chunk) pairs, column blocks inner.
+struct ColSum {
+    const float *Z;
+    float *P;
+    int M, N, per;
+};
+struct ColSums {
+    ColSum job[kMaxJobs];
+    int start[kMaxJobs + 1];
+    int n;
+};
+__global__ __launch_bounds__(kBlock) void k_colsum(ColSums J) {
     __shared__ float s_w[kBlock / 64][64];
+    const int j = job_of(J.start, J.n, (int)blockIdx.x);
+    const ColSum q = J.job[j];
+    const float *__restrict__ Z = q.Z;
+    float *__restrict__ P = q.P;
+    const int ldz = q.N, M = q.M, N = q.N, per = q.per;
+    const int nb = (N + 63) / 64, b = (int)blockIdx.x - J.start[j];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int n = blockIdx.x * 64 + lane, chunk = blockIdx.y;
+    const int n = (b % nb) * 64 + lane, chunk = b / nb;
     const int r0 = chunk * per, r1 = min(M, r0 + per);
     float s = 0.f;
     if (n < N) {
@@ -356,10 +394,17 @@ __global__ __launch_bounds__(kBlock) void k_loss_final(const double *__restrict_
     if (threadIdx.x == 0) *dst = t / denom;
 }
 
-// Adam on the FP32 master parameters.
-__global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
-                       const float *__restrict__ g, size_t n, float lr, float b1, float b2, float eps, float c1,
-                       float c2) {
+// Where each layer's weights sit in the parameter vector and their BF16 copies.
+struct PackTable {
+    int woff[4], Kp[4], Np[4];
+    bf16 *WT[4], *Wb[4];
+};
+
+// Adam on the FP32 master parameters, and the BF16 copies of each updated weight (W [in][out]
+// and W^T [out][in], as k_pack_weights writes them).
+__global__ __launch_bounds__(kBlock) void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                                                 const float *__restrict__ g, size_t n, float lr, float b1, float b2,
+                                                 float eps, float c1, float c2, PackTable T) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const float gg = g[e];
@@ -367,7 +412,18 @@ __global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__re
     const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
     m[e] = mm;
     v[e] = vv;
-    p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    const float pe = p[e] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    p[e] = pe;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const long o = (long)e - T.woff[l];
+        if (o >= 0 && o < (long)T.Kp[l] * T.Np[l]) {
+            const int k = (int)(o / T.Np[l]), c = (int)(o - (long)k * T.Np[l]);
+            const bf16 b = (bf16)pe;
+            T.Wb[l][o] = b;
+            T.WT[l][(size_t)c * T.Kp[l] + k] = b;
+        }
+    }
 }
 
 // BF16 copies of layer weights W [in][out] (FP32, padded to Kp x Np): WT [Np][Kp] and W [Kp][Np].
@@ -398,7 +454,7 @@ struct gr_mlp {
     gr::DevArray<float> p, m, v, g;
     gr::DevArray<gr::bf16> WT[4], Wb[4];
     int cap = 0;   // rows of the activation buffers (a multiple of 4096)
-    gr::DevArray<float> xin, tgt, out, d32, part, colpart;
+    gr::DevArray<float> xin, tgt, out, d32, part, cpart, colpart;
     // activations row-major and in strips (X0s, Hs), output-gradient rows ping-pong (d16[2],
     // row-major and d16s[2] in strips)
     gr::DevArray<gr::bf16> X0, X0s, H[3], Hs[3], d16[2], d16s[2];
@@ -430,8 +486,9 @@ struct gr_mlp {
             d16s[k].alloc((size_t)cap * gr::kMlpW);
         }
         d32.alloc((size_t)cap * gr::kMlpOut);
-        colpart.alloc((size_t)cap / 32 * gr::kMlpW);
-        part.alloc((size_t)gr::kWgradChunks * gr::kMlpW * gr::kMlpW);
+        colpart.alloc((size_t)3 * (cap / 32) * gr::kMlpW);
+        part.alloc(4 * kPartLayer);
+        cpart.alloc(4 * kCpartLayer);
         lpart.alloc((size_t)cap * gr::kMlpOut / gr::kBlock + 1);
     }
     void pack(hipStream_t s) {
@@ -462,29 +519,59 @@ struct gr_mlp {
                            Np[3], p.p + boff[3], (const gr::bf16 *)nullptr, (gr::bf16 *)nullptr, (gr::bf16 *)nullptr,
                            out.p, (float *)nullptr);
     }
-    // column sums of the FP32 rows Z [rows x N] (ld N) into dst, in two fixed-order passes
-    // (about sqrt(rows) chunks of at least 32 rows, into part)
-    void colsum(const float *Z, int rows, int N, float *dst) {
-        const int per = std::max(32, (int)std::ceil(std::sqrt((double)rows))), chunks = (rows + per - 1) / per;
-        hipLaunchKernelGGL(gr::k_colsum, dim3((N + 63) / 64, chunks), dim3(gr::kBlock), 0, stream, Z, N, rows, N, per,
-                           part.p);
-        hipLaunchKernelGGL(gr::k_colsum, dim3((N + 63) / 64, 1), dim3(gr::kBlock), 0, stream, (const float *)part.p, N,
-                           chunks, N, chunks, dst);
-    }
-    // weight gradient of layer l from the strips of its input (As) and output gradient (Ds):
-    // row chunks (multiples of 32 rows; at most kWgradChunks, up to 512 blocks), partials summed
-    // in order
+    // Partial buffers per layer l: weight-gradient chunk partials at part + l * kPartLayer,
+    // first-pass bias column sums at cpart + l * kCpartLayer.
+    static constexpr size_t kPartLayer = (size_t)gr::kWgradChunks * gr::kMlpW * gr::kMlpW;
+    static constexpr size_t kCpartLayer = (size_t)1040 * gr::kMlpW;   // sqrt(2^20) + 16 chunks
+    int wchunks[4] = {0, 0, 0, 0};
+    // bias column sums of layer l from rows Z [rows x N]: about sqrt(rows) chunks of at least
+    // 32 rows
+    static int colsum_per(int rows) { return std::max(32, (int)std::ceil(std::sqrt((double)rows))); }
+    // weight gradient of layer l from the strips of its input (As) and output gradient (Ds) into
+    // its chunk partials: row chunks of multiples of 32 rows, at most kWgradChunks, up to 512
+    // blocks (summed in order by sums())
     template <int TN>
     void wgrad(int l, const gr::bf16 *As, const gr::bf16 *Ds, int M) {
         const int tiles = (Kp[l] / 64) * (Np[l] / (16 * TN));
         const int want = std::max(1, std::min({(M + 127) / 128, 512 / tiles, gr::kWgradChunks}));
         const int per = ((M + want - 1) / want + 31) / 32 * 32;
-        const int chunks = (M + per - 1) / per;
-        hipLaunchKernelGGL((gr::k_wgrad<TN>), dim3(tiles, chunks), dim3(gr::kBlock), 0, stream, As, Kp[l], Ds, Np[l], M,
-                           per, part.p);
-        const int n = Kp[l] * Np[l];
-        hipLaunchKernelGGL(gr::k_sum_chunks, dim3(gr::blocks_for(n)), dim3(gr::kBlock), 0, stream, part.p, chunks, n,
-                           g.p + woff[l]);
+        wchunks[l] = (M + per - 1) / per;
+        hipLaunchKernelGGL((gr::k_wgrad<TN>), dim3(tiles, wchunks[l]), dim3(gr::kBlock), 0, stream, As, Kp[l], Ds,
+                           Np[l], M, per, part.p + l * kPartLayer);
+    }
+    // The ordered sums of all four layers, three launches: weight-gradient chunk partials, then
+    // the two passes of the bias column sums (layer 4 over the FP32 output gradient d32, layers
+    // 1-3 over the per-32-row sums the input-gradient products left in colpart).
+    void sums(int M) {
+        gr::ChunkSums cs{};
+        cs.n = 4;
+        for (int l = 0, b = 0; l < 4; ++l) {
+            const int n = Kp[l] * Np[l];
+            cs.job[l] = gr::ChunkSum{part.p + l * kPartLayer, g.p + woff[l], wchunks[l], n};
+            cs.start[l] = b;
+            b += (int)gr::blocks_for(n);
+            cs.start[l + 1] = b;
+        }
+        hipLaunchKernelGGL(gr::k_sum_chunks, dim3(cs.start[4]), dim3(gr::kBlock), 0, stream, cs);
+        gr::ColSums c1{}, c2{};
+        c1.n = c2.n = 4;
+        const int hrows = (M + 31) / 32;
+        for (int l = 0, b1 = 0, b2 = 0; l < 4; ++l) {
+            const int N = Np[l], rows = l == 3 ? M : hrows, per = colsum_per(rows);
+            const int chunks = (rows + per - 1) / per, nb = (N + 63) / 64;
+            const float *Z = l == 3 ? d32.p : colpart.p + (size_t)l * (cap / 32) * gr::kMlpW;
+            float *P = cpart.p + l * kCpartLayer;
+            c1.job[l] = gr::ColSum{Z, P, rows, N, per};
+            c1.start[l] = b1;
+            b1 += nb * chunks;
+            c1.start[l + 1] = b1;
+            c2.job[l] = gr::ColSum{P, g.p + boff[l], chunks, N, chunks};
+            c2.start[l] = b2;
+            b2 += nb;
+            c2.start[l + 1] = b2;
+        }
+        hipLaunchKernelGGL(gr::k_colsum, dim3(c1.start[4]), dim3(gr::kBlock), 0, stream, c1);
+        hipLaunchKernelGGL(gr::k_colsum, dim3(c2.start[4]), dim3(gr::kBlock), 0, stream, c2);
     }
     // loss (into *loss_dst on the device) and gradients for the targets in tgt (the forward pass
     // must have written the strips); no host sync
@@ -495,9 +582,8 @@ struct gr_mlp {
         hipLaunchKernelGGL(gr::k_mse, dim3(nb), blk, 0, stream, out.p, tgt.p, M, (int)n_out, d16[0].p, d16s[0].p,
                            d32.p, lpart.p);
         hipLaunchKernelGGL(gr::k_loss_final, dim3(1), blk, 0, stream, lpart.p, (int)nb, (double)M * n_out, loss_dst);
-        // layer 4 (linear): dW4 = H3^T dOut, db4 = column sums of dOut
+        // layer 4 (linear): dW4 = H3^T dOut
         wgrad<2>(3, Hs[2].p, d16s[0].p, M);
-        colsum(d32.p, M, gr::kMlpOut, g.p + boff[3]);
         // layers 3..1: dZ_l = (dZ_{l+1} W_{l+1}^T) * (1 - H_l^2) in the input-gradient product's
         // epilogue (with the first stage of the bias gradient), then dW_l = A_l^T dZ_l
         int cur = 0;
@@ -506,20 +592,28 @@ struct gr_mlp {
             hipLaunchKernelGGL(l == 2 ? (gr::k_gemm<gr::kEpiDtanh, 4, gr::kMlpOut>) : (gr::k_gemm<gr::kEpiDtanh, 4, gr::kMlpW>),
                                gh, blk, 0, stream, d16[cur].p, Np[l + 1], M, Wb[l + 1].p, Kp[l + 1],
                                (const float *)nullptr, (const gr::bf16 *)Hs[l].p, l > 0 ? d16[nxt].p : nullptr,
-                               d16s[nxt].p, (float *)nullptr, colpart.p);
+                               d16s[nxt].p, (float *)nullptr, colpart.p + (size_t)l * (cap / 32) * gr::kMlpW);
             wgrad<4>(l, l == 0 ? X0s.p : Hs[l - 1].p, d16s[nxt].p, M);
-            colsum(colpart.p, (M + 31) / 32, gr::kMlpW, g.p + boff[l]);
             cur = nxt;
         }
+        sums(M);
         GR_HIP(hipGetLastError());
     }
+    // Adam step on every parameter; the BF16 weight copies are refreshed in the same launch
     void adam(float lr) {
         step += 1;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+        gr::PackTable T{};
+        for (int l = 0; l < 4; ++l) {
+            T.woff[l] = (int)woff[l];
+            T.Kp[l] = Kp[l];
+            T.Np[l] = Np[l];
+            T.WT[l] = WT[l].p;
+            T.Wb[l] = Wb[l].p;
+        }
         hipLaunchKernelGGL(gr::k_adam, dim3(gr::blocks_for(n_params)), dim3(gr::kBlock), 0, stream, p.p, m.p, v.p, g.p,
-                           n_params, lr, b1, b2, 1e-8f, c1, c2);
-        pack(stream);
+                           n_params, lr, b1, b2, 1e-8f, c1, c2, T);
     }
     double host_loss() {
         double l = 0;
