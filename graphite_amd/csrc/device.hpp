@@ -22,6 +22,14 @@ constexpr int kBlock = 256;   // threads per block of the per-cell kernels (4 wa
 
 inline unsigned blocks_for(size_t n, unsigned b = kBlock) { return (unsigned)((n + b - 1) / b); }
 
+// Bijection workgroup id -> work item (0..T-1) such that XCD x (= w mod 8: the MI355X hands
+// workgroups to its 8 XCDs in turn) gets a contiguous range of items, whose shared inputs then
+// stay in that XCD's own L2.
+__device__ inline uint32_t xcd_tile(uint32_t w, uint32_t T) {
+    const uint32_t x = w & 7u, k = w >> 3;
+    return x * (T >> 3) + min(x, T & 7u) + k;
+}
+
 // Owning device allocation.
 template <typename T>
 struct DevArray {

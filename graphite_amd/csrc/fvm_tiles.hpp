@@ -30,11 +30,6 @@ static_assert(5 * kTileCells <= kBlock, "per-cell phases need 5 threads per cell
 constexpr int kTileRec = 16;
 static_assert(kTileCells + 1 <= 4 * (kTileRec - 3) && kTileSlots < 256, "tile record layout");
 
-// Bijection workgroup id -> tile such that XCD x (= w mod 8) gets a contiguous tile range.
-__device__ inline uint32_t xcd_tile(uint32_t w, uint32_t T) {
-    const uint32_t x = w & 7u, k = w >> 3;
-    return x * (T >> 3) + min(x, T & 7u) + k;
-}
 
 // Ghost state from the state of its cell, outward normal n (src/models/euler.cu:436-441,
 // :473-518).  Same arithmetic, in the same order, as k_euler_bc.

@@ -40,20 +40,118 @@ __device__ inline size_t strip_index(int r, int c, int C) { return ((size_t)(r >
 
 enum { kEpiTanh = 0, kEpiLinear = 1, kEpiDtanh = 2 };
 
-// Y[M x N] = epilogue(X[M x K] * B), B given as BT = B^T [N x K] row-major, K % 32 == 0.  Wave
-// tile 32 rows x 16*TN columns (2 x TN accumulators of the 16x16x32 MFMA: lane l holds
-// row/column l mod 16 of its operand and eight consecutive k of group l/16; accumulator element
-// j of lane l is row 4*(l/16)+j, column l mod 16), operands loaded straight from global memory
-// (the weights stay in L2): the rows four k-steps ahead of the MFMAs, the weights one; a
-// block's 4 waves take the N/(16*TN) column tiles of 4*16*TN/N row groups.  Epilogues:
+// tanh for values about to be rounded to BF16, within about 3 ulp of binary32 (a correctly
+// rounded tanh would still differ from it in the BF16 result of a few values in a million, as any
+// binary32 tanh does): below |x| = 0.55 the odd Taylor series of tanh to x^15 (Horner in x^2;
+// the next term is under 5e-8 |x| there), above it (1 - e) / (1 + e) with e = exp(-2|x|) <= 0.33
+// (hardware exp2 and reciprocal), where 1 - e loses no digits.
+__device__ inline float tanh_bf16(float x) {
+    const float a = fabsf(x), s = a * a;
+    float p = -929569.f / 638512875.f;
+    p = fmaf(p, s, 21844.f / 6081075.f);
+    p = fmaf(p, s, -1382.f / 155925.f);
+    p = fmaf(p, s, 62.f / 2835.f);
+    p = fmaf(p, s, -17.f / 315.f);
+    p = fmaf(p, s, 2.f / 15.f);
+    p = fmaf(p, s, -1.f / 3.f);
+    const float poly = fmaf(a * s, p, a);
+    const float e = __builtin_amdgcn_exp2f(-2.885390081777927f * a);   // 2 log2(e)
+    const float t = a < 0.55f ? poly : (1.f - e) * __builtin_amdgcn_rcpf(1.f + e);
+    return copysignf(t, x);
+}
+
+// Epilogue of one wave's 32 x 16*TN tile of a product's accumulators (rows row0.., columns
+// n0..; accumulator element j of lane l is row 4*(l/16)+j, column l mod 16 of its 16x16 block):
 //   kEpiTanh:   tanh(acc + bias) -> Y (BF16 row-major) and, if Ys is set, Ys (strip layout);
 //   kEpiLinear: acc + bias -> Y32 (FP32 row-major);
 //   kEpiDtanh:  acc * (1 - h^2) with h the layer output read from Hs (strip layout) -> Y (if
 //               set) and Ys; column sums of the FP32 values over the wave's rows (< M) into
 //               colpart[row0 / 32][N] (the bias gradient's first stage, fixed order).
-// BF16 outputs are staged in LDS so that global stores are 16-byte rows and strips.  Rows at
-// or past M repeat row M-1 (loads stay unconditional, so they pipeline): their row-major
-// entries are not stored, and their strip entries (finite) are masked by every reader.
+// BF16 outputs are staged in the wave's LDS tiles sT and sS so that global stores are 16-byte
+// rows and strips.  Rows at or past M (the products repeat row M-1 there, so their loads stay
+// unconditional and pipeline) are not stored row-major; their strip entries (finite) are masked
+// by every reader.  Every wave of the block calls it together (it synchronises the block).
+template <int EPI, int TN>
+__device__ __forceinline__ void gemm_epilogue(const f32x4 (&acc)[2][TN], int row0, int n0, int M, int N,
+                                              const float *__restrict__ bias, const bf16 *__restrict__ Hs,
+                                              bf16 *__restrict__ Y, bf16 *__restrict__ Ys, float *__restrict__ Y32,
+                                              float *__restrict__ colpart, bf16 (*sT)[16 * TN + 8],
+                                              bf16 (*sS)[16 * TN][8]) {
+    constexpr int TC = 16 * TN;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+    if constexpr (EPI == kEpiLinear) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int t = 0; t < TN; ++t) {
+                const int col = n0 + t * 16 + lr;
+                const float bb = bias[col];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int row = row0 + r * 16 + 4 * lg + j;
+                    if (row < M) Y32[(size_t)row * N + col] = acc[r][t][j] + bb;
+                }
+            }
+    } else {
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+            const int col = n0 + t * 16 + lr;
+            float csum = 0.f;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int R = row0 + r * 16 + 4 * lg;   // first of this lane's 4 rows (a multiple of 4)
+                float v[4];
+                if constexpr (EPI == kEpiTanh) {
+                    const float bb = bias[col];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = tanh_bf16(acc[r][t][j] + bb);
+                } else {
+                    const bf16x4 h4 = *(const bf16x4 *)&Hs[strip_index(R, col, N)];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float h = (float)h4[j];
+                        v[j] = acc[r][t][j] * (1.f - h * h);
+                        if (R + j < M) csum += v[j];
+                    }
+                }
+                bf16x4 o;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    o[j] = (bf16)v[j];
+                    sT[r * 16 + 4 * lg + j][t * 16 + lr] = o[j];
+                }
+                *(bf16x4 *)&sS[2 * r + (lg >> 1)][t * 16 + lr][4 * (lg & 1)] = o;
+            }
+            if constexpr (EPI == kEpiDtanh) {
+                // the four lane groups' sums, (g0 + g1) + (g2 + g3) in every lane
+                csum += __shfl_xor(csum, 16);
+                csum += __shfl_xor(csum, 32);
+                if (lg == 0) colpart[(size_t)(row0 / 32) * N + col] = csum;
+            }
+        }
+        __syncthreads();
+        if (Y) {
+            for (int e = lane; e < 32 * (TC / 8); e += 64) {
+                const int row = e / (TC / 8), c8 = e - row * (TC / 8);
+                if (row0 + row < M)
+                    *(bf16x8 *)&Y[(size_t)(row0 + row) * N + n0 + c8 * 8] = *(const bf16x8 *)&sT[row][c8 * 8];
+            }
+        }
+        if (Ys) {
+            for (int e = lane; e < 4 * TC; e += 64) {
+                const int b = e / TC, c = e - b * TC;
+                *(bf16x8 *)&Ys[strip_index(row0 + 8 * b, n0 + c, N)] = *(const bf16x8 *)&sS[b][c][0];
+            }
+        }
+    }
+}
+
+// Y[M x N] = epilogue(X[M x K] * B), B given as BT = B^T [N x K] row-major, K % 32 == 0.  Wave
+// tile 32 rows x 16*TN columns (2 x TN accumulators of the 16x16x32 MFMA: lane l holds
+// row/column l mod 16 of its operand and eight consecutive k of group l/16), operands loaded
+// straight from global memory (the weights stay in L2): the rows four k-steps ahead of the
+// MFMAs, the weights one; a block's 4 waves take the N/(16*TN) column tiles of 4*16*TN/N row
+// groups.
 template <int EPI, int TN, int K>
 __global__ __launch_bounds__(kBlock) void k_gemm(const bf16 *__restrict__ X, int ldx, int M,
                                                  const bf16 *__restrict__ BT, int N, const float *__restrict__ bias,
@@ -108,70 +206,80 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const bf16 *__restrict__ X, int
         }
         __builtin_amdgcn_sched_barrier(0);
     }
-    if constexpr (EPI == kEpiLinear) {
+    gemm_epilogue<EPI, TN>(acc, row0, n0, M, N, bias, Hs, Y, Ys, Y32, colpart, sT[w], sS[w]);
+}
+
+// Y[M x 256] = epilogue(X[M x K] * B) with the weights held in registers: wave w of a block owns
+// the output columns 64w..64w+63 and loads their weight fragments for all of K once, then the
+// block walks the 32-row tiles blockIdx.x, blockIdx.x + gridDim.x, ... (its four waves on the
+// same rows), so a weight is fetched once per block instead of once per wave and k-step.  Row
+// fragments are loaded two k-steps ahead of the MFMAs, across tile boundaries (the last tile
+// re-reads its own rows in place of a next one).  The k-steps and MFMAs of each output element
+// are those of k_gemm<EPI, 4, K>, in the same order: the same bits.
+template <int EPI, int K>
+__global__ __launch_bounds__(kBlock) void k_gemm_ws(const bf16 *__restrict__ X, int ldx, int M,
+                                                    const bf16 *__restrict__ BT, const float *__restrict__ bias,
+                                                    const bf16 *__restrict__ Hs, bf16 *__restrict__ Y,
+                                                    bf16 *__restrict__ Ys, float *__restrict__ colpart) {
+    static_assert(EPI != kEpiLinear, "FP32 outputs go through k_gemm");
+    constexpr int TN = 4, TC = 16 * TN, N = kMlpW;
+    constexpr int S = K / 32, D = S < 2 ? S : 2;   // k-steps; k-steps of A in flight (2: two waves per SIMD at K = 256)
+    static_assert((kBlock / 64) * TC == N, "the block's waves cover the columns");
+    __shared__ __attribute__((aligned(16))) bf16 sT[kBlock / 64][32][TC + 8];
+    __shared__ __attribute__((aligned(16))) bf16 sS[kBlock / 64][4][TC][8];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int n0 = w * TC, ntiles = (M + 31) / 32;
+    int tile = blockIdx.x;
+    if (tile >= ntiles) return;   // the whole block
+    bf16x8 fb[S][TN];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) fb[s][t] = *(const bf16x8 *)(BT + (size_t)(n0 + t * 16 + lr) * K + 32 * s + 8 * lg);
+    // row offsets (elements of X) of this lane's two rows in the current and the next tile;
+    // offsets rather than pointers, so that every load is known to address global memory
+    // (a pointer chosen between two tiles becomes generic, and generic loads are waited for one
+    // by one)
+    size_t xa[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) xa[r] = (size_t)min(tile * 32 + r * 16 + lr, M - 1) * ldx + 8 * lg;
+    bf16x8 fa[D][2];
+#pragma unroll
+    for (int s = 0; s < D; ++s)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) fa[s][r] = *(const bf16x8 *)(X + xa[r] + 32 * s);
+    for (;;) {
+        const int next = tile + (int)gridDim.x;
+        const bool more = next < ntiles;
+        size_t xn[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) xn[r] = (size_t)min((more ? next : tile) * 32 + r * 16 + lr, M - 1) * ldx + 8 * lg;
+        f32x4 acc[2][TN];
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
+            for (int t = 0; t < TN; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int q = s % D;
+#pragma unroll
             for (int t = 0; t < TN; ++t) {
-                const int col = n0 + t * 16 + lr;
-                const float bb = bias[col];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int row = row0 + r * 16 + 4 * lg + j;
-                    if (row < M) Y32[(size_t)row * N + col] = acc[r][t][j] + bb;
-                }
+                acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[q][0], fb[s][t], acc[0][t], 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[q][1], fb[s][t], acc[1][t], 0, 0, 0);
             }
-    } else {
 #pragma unroll
-        for (int t = 0; t < TN; ++t) {
-            const int col = n0 + t * 16 + lr;
-            float csum = 0.f;
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int R = row0 + r * 16 + 4 * lg;   // first of this lane's 4 rows (a multiple of 4)
-                float v[4];
-                if constexpr (EPI == kEpiTanh) {
-                    const float bb = bias[col];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) v[j] = tanhf(acc[r][t][j] + bb);
-                } else {
-                    const bf16x4 h4 = *(const bf16x4 *)&Hs[strip_index(R, col, N)];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const float h = (float)h4[j];
-                        v[j] = acc[r][t][j] * (1.f - h * h);
-                        if (R + j < M) csum += v[j];
-                    }
-                }
-                bf16x4 o;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    o[j] = (bf16)v[j];
-                    sT[w][r * 16 + 4 * lg + j][t * 16 + lr] = o[j];
-                }
-                *(bf16x4 *)&sS[w][2 * r + (lg >> 1)][t * 16 + lr][4 * (lg & 1)] = o;
-            }
-            if constexpr (EPI == kEpiDtanh) {
-                // the four lane groups' sums, (g0 + g1) + (g2 + g3) in every lane
-                csum += __shfl_xor(csum, 16);
-                csum += __shfl_xor(csum, 32);
-                if (lg == 0) colpart[(size_t)(row0 / 32) * N + col] = csum;
-            }
+            for (int r = 0; r < 2; ++r)
+                fa[q][r] = *(const bf16x8 *)(X + (s + D < S ? xa[r] + 32 * (s + D) : xn[r] + 32 * (s + D - S)));
+            __builtin_amdgcn_sched_barrier(0);
         }
-        __syncthreads();
-        if (Y) {
-            for (int e = lane; e < 32 * (TC / 8); e += 64) {
-                const int row = e / (TC / 8), c8 = e - row * (TC / 8);
-                if (row0 + row < M)
-                    *(bf16x8 *)&Y[(size_t)(row0 + row) * N + n0 + c8 * 8] = *(const bf16x8 *)&sT[w][row][c8 * 8];
-            }
-        }
-        if (Ys) {
-            for (int e = lane; e < 4 * TC; e += 64) {
-                const int b = e / TC, c = e - b * TC;
-                *(bf16x8 *)&Ys[strip_index(row0 + 8 * b, n0 + c, N)] = *(const bf16x8 *)&sS[w][b][c][0];
-            }
-        }
+        gemm_epilogue<EPI, TN>(acc, tile * 32, n0, M, N, bias, Hs, Y, Ys, nullptr, colpart, sT[w], sS[w]);
+        if (!more) break;
+        tile = next;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) xa[r] = xn[r];
+        __syncthreads();   // the next epilogue rewrites the staging tiles
     }
 }
 
@@ -210,34 +318,47 @@ struct WgradStep {
 // Partial weight gradient P[chunk][K][N] = A[rows of chunk]^T * D[rows of chunk] from the strip
 // layouts As [M/8][K][8] and Ds [M/8][N][8] (K % 64 == 0, N % (16*TN) == 0, per % 32 == 0).
 // Block (tile, chunk): a 64 x 16*TN tile of the gradient; its 4 waves take the chunk's 32-row
-// steps w, w+4, ..., each loading the next step's fragments before the current step's MFMAs,
-// and add their tiles through LDS in wave order (fixed order: deterministic).  Rows at or past M
-// are masked to zero in both operands.
-template <int TN>
+// steps w, w+4, ..., in order, each with the fragments of its next DQ steps in flight (a ring of
+// DQ step buffers: a wave's steps are a chain of dependent HBM reads, so the depth of the ring
+// divides the chain's latency), and add their tiles through LDS in wave order (fixed order:
+// deterministic).  Rows at or past M are masked to zero in both operands.
+template <int TN, int DQ>
 __global__ __launch_bounds__(kBlock) void k_wgrad(const bf16 *__restrict__ As, int K, const bf16 *__restrict__ Ds,
                                                   int N, int M, int per, float *__restrict__ P) {
     constexpr int TC = 16 * TN, W = kBlock / 64;
     __shared__ f32x4 s_acc[W - 1][4][TN][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
+    // (tile, chunk) in XCD order: the tiles of a chunk read the same strips, so they run on one
+    // XCD, where the strips stay in its L2 (in dispatch order the four column tiles of a row of
+    // tiles would land on four XCDs and fetch the strips four times from HBM)
+    const uint32_t v = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tile = (int)(v % gridDim.x), chunk = (int)(v / gridDim.x);
     const int tiles_n = N / TC;
-    const int i0 = (blockIdx.x / tiles_n) * 64, n0 = (blockIdx.x % tiles_n) * TC;
-    const int r_begin = blockIdx.y * per, r_end = min(M, r_begin + per);
+    const int i0 = (tile / tiles_n) * 64, n0 = (tile % tiles_n) * TC;
+    const int r_begin = chunk * per, r_end = min(M, r_begin + per);
     f32x4 acc[4][TN];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int t = 0; t < TN; ++t) acc[u][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // each step's loads issued one step ahead (r0 is wave-uniform)
-    WgradStep<TN> cur, nxt;
-    int r0 = r_begin + 32 * w;
-    if (r0 < r_end) cur.load(As, K, Ds, N, r0 + 8 * lg, i0, n0, lr);
-    for (; r0 < r_end; r0 += 32 * W) {
-        const int r1 = r0 + 32 * W;
-        if (r1 < r_end) nxt.load(As, K, Ds, N, r1 + 8 * lg, i0, n0, lr);
-        __builtin_amdgcn_sched_barrier(0);   // the next step's loads go out before these MFMAs
-        cur.mma(acc, r0, r0 + 8 * lg, r_end);
-        cur = nxt;
+    // step d of the ring holds rows r0 + 32 W d (row indices are wave-uniform); after its MFMAs
+    // it takes the step DQ further on.  The loads are unconditional (a step past the chunk loads
+    // the chunk's first rows again, in bounds, and is never multiplied), so the compiler can count
+    // them and wait for the oldest step only.
+    WgradStep<TN> st[DQ];
+    const int r_first = r_begin + 32 * w;
+    auto load_row = [&](int rr) { return (rr < r_end ? rr : r_begin) + 8 * lg; };
+#pragma unroll
+    for (int d = 0; d < DQ; ++d) st[d].load(As, K, Ds, N, load_row(r_first + 32 * W * d), i0, n0, lr);
+    for (int r0 = r_first; r0 < r_end; r0 += 32 * W * DQ) {
+#pragma unroll
+        for (int d = 0; d < DQ; ++d) {
+            const int rr = r0 + 32 * W * d;
+            if (rr < r_end) st[d].mma(acc, rr, rr + 8 * lg, r_end);
+            st[d].load(As, K, Ds, N, load_row(rr + 32 * W * DQ), i0, n0, lr);
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
     if (w > 0) {
 #pragma unroll
@@ -247,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const bf16 *__restrict__ As, i
     }
     __syncthreads();
     if (w == 0) {
-        float *Pc = P + (size_t)blockIdx.y * K * N;
+        float *Pc = P + (size_t)chunk * K * N;
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -281,10 +402,10 @@ struct ChunkSums {
     int start[kMaxJobs + 1];
     int n;
 };
-__global__ __launch_bounds__(kBlock) void k_sum_chunks(ChunkSums J) {
-    const int j = job_of(J.start, J.n, (int)blockIdx.x);
+__device__ inline void sum_chunks(const ChunkSums &J, int b) {
+    const int j = job_of(J.start, J.n, b);
     const ChunkSum q = J.job[j];
-    const int e = ((int)blockIdx.x - J.start[j]) * blockDim.x + threadIdx.x;
+    const int e = (b - J.start[j]) * blockDim.x + threadIdx.x;
     if (e >= q.n) return;
     float s = 0.f;
 #pragma unroll 8
@@ -309,14 +430,14 @@ struct ColSums {
     int start[kMaxJobs + 1];
     int n;
 };
-__global__ __launch_bounds__(kBlock) void k_colsum(ColSums J) {
+__device__ inline void colsum(const ColSums &J, int blk) {
     __shared__ float s_w[kBlock / 64][64];
-    const int j = job_of(J.start, J.n, (int)blockIdx.x);
+    const int j = job_of(J.start, J.n, blk);
     const ColSum q = J.job[j];
     const float *__restrict__ Z = q.Z;
     float *__restrict__ P = q.P;
     const int ldz = q.N, M = q.M, N = q.N, per = q.per;
-    const int nb = (N + 63) / 64, b = (int)blockIdx.x - J.start[j];
+    const int nb = (N + 63) / 64, b = blk - J.start[j];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = (b % nb) * 64 + lane, chunk = b / nb;
     const int r0 = chunk * per, r1 = min(M, r0 + per);
@@ -333,6 +454,32 @@ __global__ __launch_bounds__(kBlock) void k_colsum(ColSums J) {
         P[(size_t)chunk * N + n] = t;
     }
 }
+__global__ __launch_bounds__(kBlock) void k_colsum(ColSums J) { colsum(J, (int)blockIdx.x); }
+
+// The mean squared error of a batch: *dst = (sum of the n per-block partials, in a fixed order) /
+// denom.
+struct LossSum {
+    const double *partials;
+    int n;
+    double denom;
+    double *dst;
+};
+__device__ inline void loss_sum(const LossSum &L) {
+    double s = 0;
+    for (int i = threadIdx.x; i < L.n; i += blockDim.x) s += L.partials[i];
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) *L.dst = t / L.denom;
+}
+
+// The sums that need only the backward pass, in one launch: the weight-gradient chunk partials
+// of every layer (blocks 0 ..), the first pass of every layer's bias column sums (the next
+// c1.start[c1.n] blocks) and the loss (the last block).
+__global__ __launch_bounds__(kBlock) void k_grad_sums(ChunkSums cs, ColSums c1, LossSum L) {
+    const int b = (int)blockIdx.x, nc = cs.start[cs.n], n1 = c1.start[c1.n];
+    if (b < nc) sum_chunks(cs, b);
+    else if (b < nc + n1) colsum(c1, b - nc);
+    else loss_sum(L);
+}
 
 // FP32 host input [M x n_in] -> BF16 [M x kMlpIn], zero padded, and (if Xs is set) its strip
 // layout.
@@ -347,18 +494,25 @@ __global__ void k_pack_input(const float *__restrict__ x, int M, int n_in, bf16 
 }
 
 // Mini-batch of a resident training set: row r of the batch is row (a*(pos0+r) + b) mod Mall of
-// the set (a coprime to Mall, so one pass of pos over 0..Mall-1 visits every row once).
+// the set (a coprime to Mall, so one pass of pos over 0..Mall-1 visits every row once).  One
+// thread per (row, group of 8 columns): the source row (two 64-bit remainders) once per group,
+// one 16-byte row-major store; a wave holds 8 consecutive rows, so its strip stores fill whole
+// 16-byte strip entries.
 __global__ void k_gather_rows(const float *__restrict__ xall, const float *__restrict__ tall, uint64_t Mall, int n_in,
                               int n_out, uint64_t a, uint64_t b, uint64_t pos0, int M, bf16 *__restrict__ X,
                               bf16 *__restrict__ Xs, float *__restrict__ T) {
+    constexpr int G = kMlpIn / 8;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)M * kMlpIn) return;
-    const int r = (int)(e / kMlpIn), c = (int)(e % kMlpIn);
+    if (e >= (size_t)M * G) return;
+    const int r = (int)(e / G), c0 = 8 * (int)(e % G);
     const uint64_t src = (a * ((pos0 + r) % Mall) + b) % Mall;
-    const bf16 v = (bf16)(c < n_in ? xall[src * n_in + c] : 0.f);
-    X[e] = v;
-    Xs[strip_index(r, c, kMlpIn)] = v;
-    if (c < n_out) T[(size_t)r * n_out + c] = tall[src * n_out + c];
+    bf16x8 v;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (bf16)(c0 + k < n_in ? xall[src * n_in + c0 + k] : 0.f);
+    *(bf16x8 *)&X[(size_t)r * kMlpIn + c0] = v;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Xs[strip_index(r, c0 + k, kMlpIn)] = v[k];
+    for (int c = c0; c < c0 + 8 && c < n_out; ++c) T[(size_t)r * n_out + c] = tall[src * n_out + c];
 }
 
 // Mean squared error: dOut = 2 (out - t) / (M n_out) (BF16 row-major and in strips for the
@@ -383,15 +537,6 @@ __global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ out, c
     }
     const double bs = block_sum(se);
     if (threadIdx.x == 0) partials[blockIdx.x] = bs;
-}
-
-// dst = (sum of the n partials, in a fixed order) / denom
-__global__ __launch_bounds__(kBlock) void k_loss_final(const double *__restrict__ partials, int n, double denom,
-                                                       double *__restrict__ dst) {
-    double s = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
-    const double t = block_sum(s);
-    if (threadIdx.x == 0) *dst = t / denom;
 }
 
 // Where each layer's weights sit in the parameter vector and their BF16 copies.
@@ -451,6 +596,7 @@ struct gr_mlp {
     int Np[4] = {gr::kMlpW, gr::kMlpW, gr::kMlpW, gr::kMlpOut};
     size_t woff[4], boff[4], n_params = 0;   // offsets into the padded FP32 parameter vector
     hipStream_t stream = nullptr;
+    int cus = 256;   // compute units of the device
     gr::DevArray<float> p, m, v, g;
     gr::DevArray<gr::bf16> WT[4], Wb[4];
     int cap = 0;   // rows of the activation buffers (a multiple of 4096)
@@ -506,15 +652,19 @@ struct gr_mlp {
         const int rows_per_block = 32 * ((gr::kBlock / 64) / (N / (16 * TN)));
         return (unsigned)((M + rows_per_block - 1) / rows_per_block);
     }
+    // blocks of a weight-stationary product: kWsBlocksPerCu per compute unit, at most one per
+    // 32-row tile
+    static constexpr int kWsBlocksPerCu = 2;
+    unsigned ws_blocks(int M) const { return (unsigned)std::min((M + 31) / 32, kWsBlocksPerCu * cus); }
     // strips: also write the hidden layers' strip layouts (training)
     void forward(int M, hipStream_t s, bool strips) {
-        const dim3 blk(gr::kBlock), gh(gemm_blocks(M, gr::kMlpW, 4)), go(gemm_blocks(M, gr::kMlpOut, 2));
+        const dim3 blk(gr::kBlock), gh(ws_blocks(M)), go(gemm_blocks(M, gr::kMlpOut, 2));
         const gr::bf16 *in[3] = {X0.p, H[0].p, H[1].p};
         const int ld[3] = {gr::kMlpIn, gr::kMlpW, gr::kMlpW};
         for (int l = 0; l < 3; ++l)
-            hipLaunchKernelGGL(l == 0 ? (gr::k_gemm<gr::kEpiTanh, 4, gr::kMlpIn>) : (gr::k_gemm<gr::kEpiTanh, 4, gr::kMlpW>),
-                               gh, blk, 0, s, in[l], ld[l], M, WT[l].p, Np[l], p.p + boff[l], (const gr::bf16 *)nullptr,
-                               H[l].p, strips ? Hs[l].p : nullptr, (float *)nullptr, (float *)nullptr);
+            hipLaunchKernelGGL(l == 0 ? (gr::k_gemm_ws<gr::kEpiTanh, gr::kMlpIn>) : (gr::k_gemm_ws<gr::kEpiTanh, gr::kMlpW>),
+                               gh, blk, 0, s, in[l], ld[l], M, WT[l].p, p.p + boff[l], (const gr::bf16 *)nullptr,
+                               H[l].p, strips ? Hs[l].p : nullptr, (float *)nullptr);
         hipLaunchKernelGGL((gr::k_gemm<gr::kEpiLinear, 2, gr::kMlpW>), go, blk, 0, s, H[2].p, gr::kMlpW, M, WT[3].p,
                            Np[3], p.p + boff[3], (const gr::bf16 *)nullptr, (gr::bf16 *)nullptr, (gr::bf16 *)nullptr,
                            out.p, (float *)nullptr);
@@ -530,19 +680,21 @@ struct gr_mlp {
     // weight gradient of layer l from the strips of its input (As) and output gradient (Ds) into
     // its chunk partials: row chunks of multiples of 32 rows, at most kWgradChunks, up to 512
     // blocks (summed in order by sums())
+    static constexpr int kWgradDepth = 3;   // row steps in flight per wave
     template <int TN>
     void wgrad(int l, const gr::bf16 *As, const gr::bf16 *Ds, int M) {
         const int tiles = (Kp[l] / 64) * (Np[l] / (16 * TN));
         const int want = std::max(1, std::min({(M + 127) / 128, 512 / tiles, gr::kWgradChunks}));
         const int per = ((M + want - 1) / want + 31) / 32 * 32;
         wchunks[l] = (M + per - 1) / per;
-        hipLaunchKernelGGL((gr::k_wgrad<TN>), dim3(tiles, wchunks[l]), dim3(gr::kBlock), 0, stream, As, Kp[l], Ds,
-                           Np[l], M, per, part.p + l * kPartLayer);
+        hipLaunchKernelGGL((gr::k_wgrad<TN, kWgradDepth>), dim3(tiles, wchunks[l]), dim3(gr::kBlock), 0, stream, As,
+                           Kp[l], Ds, Np[l], M, per, part.p + l * kPartLayer);
     }
-    // The ordered sums of all four layers, three launches: weight-gradient chunk partials, then
-    // the two passes of the bias column sums (layer 4 over the FP32 output gradient d32, layers
-    // 1-3 over the per-32-row sums the input-gradient products left in colpart).
-    void sums(int M) {
+    // The ordered sums of all four layers and the loss, two launches: the weight-gradient chunk
+    // partials, the first pass of the bias column sums (layer 4 over the FP32 output gradient
+    // d32, layers 1-3 over the per-32-row sums the input-gradient products left in colpart) and
+    // the loss together, then the bias sums' second pass.
+    void sums(int M, const gr::LossSum &loss) {
         gr::ChunkSums cs{};
         cs.n = 4;
         for (int l = 0, b = 0; l < 4; ++l) {
@@ -552,7 +704,6 @@ struct gr_mlp {
             b += (int)gr::blocks_for(n);
             cs.start[l + 1] = b;
         }
-        hipLaunchKernelGGL(gr::k_sum_chunks, dim3(cs.start[4]), dim3(gr::kBlock), 0, stream, cs);
         gr::ColSums c1{}, c2{};
         c1.n = c2.n = 4;
         const int hrows = (M + 31) / 32;
@@ -570,7 +721,8 @@ struct gr_mlp {
             b2 += nb;
             c2.start[l + 1] = b2;
         }
-        hipLaunchKernelGGL(gr::k_colsum, dim3(c1.start[4]), dim3(gr::kBlock), 0, stream, c1);
+        hipLaunchKernelGGL(gr::k_grad_sums, dim3(cs.start[4] + c1.start[4] + 1), dim3(gr::kBlock), 0, stream, cs, c1,
+                           loss);
         hipLaunchKernelGGL(gr::k_colsum, dim3(c2.start[4]), dim3(gr::kBlock), 0, stream, c2);
     }
     // loss (into *loss_dst on the device) and gradients for the targets in tgt (the forward pass
@@ -578,10 +730,9 @@ struct gr_mlp {
     void backward(int M, double *loss_dst) {
         const size_t no = (size_t)M * gr::kMlpOut;
         const unsigned nb = gr::blocks_for(no);
-        const dim3 blk(gr::kBlock), gh(gemm_blocks(M, gr::kMlpW, 4));
+        const dim3 blk(gr::kBlock), gh(ws_blocks(M));
         hipLaunchKernelGGL(gr::k_mse, dim3(nb), blk, 0, stream, out.p, tgt.p, M, (int)n_out, d16[0].p, d16s[0].p,
                            d32.p, lpart.p);
-        hipLaunchKernelGGL(gr::k_loss_final, dim3(1), blk, 0, stream, lpart.p, (int)nb, (double)M * n_out, loss_dst);
         // layer 4 (linear): dW4 = H3^T dOut
         wgrad<2>(3, Hs[2].p, d16s[0].p, M);
         // layers 3..1: dZ_l = (dZ_{l+1} W_{l+1}^T) * (1 - H_l^2) in the input-gradient product's
@@ -589,14 +740,14 @@ struct gr_mlp {
         int cur = 0;
         for (int l = 2; l >= 0; --l) {
             const int nxt = 1 - cur;
-            hipLaunchKernelGGL(l == 2 ? (gr::k_gemm<gr::kEpiDtanh, 4, gr::kMlpOut>) : (gr::k_gemm<gr::kEpiDtanh, 4, gr::kMlpW>),
-                               gh, blk, 0, stream, d16[cur].p, Np[l + 1], M, Wb[l + 1].p, Kp[l + 1],
-                               (const float *)nullptr, (const gr::bf16 *)Hs[l].p, l > 0 ? d16[nxt].p : nullptr,
-                               d16s[nxt].p, (float *)nullptr, colpart.p + (size_t)l * (cap / 32) * gr::kMlpW);
+            hipLaunchKernelGGL(l == 2 ? (gr::k_gemm_ws<gr::kEpiDtanh, gr::kMlpOut>) : (gr::k_gemm_ws<gr::kEpiDtanh, gr::kMlpW>),
+                               gh, blk, 0, stream, d16[cur].p, Np[l + 1], M, Wb[l + 1].p, (const float *)nullptr,
+                               (const gr::bf16 *)Hs[l].p, l > 0 ? d16[nxt].p : nullptr, d16s[nxt].p,
+                               colpart.p + (size_t)l * (cap / 32) * gr::kMlpW);
             wgrad<4>(l, l == 0 ? X0s.p : Hs[l - 1].p, d16s[nxt].p, M);
             cur = nxt;
         }
-        sums(M);
+        sums(M, gr::LossSum{lpart.p, (int)nb, (double)M * n_out, loss_dst});
         GR_HIP(hipGetLastError());
     }
     // Adam step on every parameter; the BF16 weight copies are refreshed in the same launch
@@ -670,6 +821,9 @@ int gr_mlp_create(uint32_t n_in, uint32_t n_out, uint64_t seed, gr_mlp **out) {
         try {
             n->n_in = n_in;
             n->n_out = n_out;
+            int dev = 0;
+            GR_HIP(hipGetDevice(&dev));
+            GR_HIP(hipDeviceGetAttribute(&n->cus, hipDeviceAttributeMultiprocessorCount, dev));
             GR_HIP(hipStreamCreate(&n->stream));
             size_t off = 0;
             for (int l = 0; l < 4; ++l) {
@@ -829,7 +983,7 @@ int gr_mlp_train(gr_mlp *n, int batch, int steps, float lr, float lr_final, uint
                 } while (gcd64(a, Mall) != 1);
                 b = rng() % Mall;
             }
-            hipLaunchKernelGGL(gr::k_gather_rows, dim3(gr::blocks_for((size_t)batch * gr::kMlpIn)), dim3(gr::kBlock), 0,
+            hipLaunchKernelGGL(gr::k_gather_rows, dim3(gr::blocks_for((size_t)batch * (gr::kMlpIn / 8))), dim3(gr::kBlock), 0,
                                n->stream, n->xall.p, n->tall.p, Mall, (int)n->n_in, (int)n->n_out, a, b,
                                within * (uint64_t)batch, batch, n->X0.p, n->X0s.p, n->tgt.p);
             n->forward(batch, n->stream, true);
