@@ -417,6 +417,43 @@ def wing_solver(gr, m, dtype, cfl, tune_iters=10):
     return sv
 
 
+def shard_leg(gr, e, m, c2o, steps, parts=8):
+    """The wing cut into `parts` contiguous ranges of its Hilbert order, as `parts` GPUs would own
+    it: each range's cells with its boundary faces and a halo region for the faces to the other
+    ranges (ghosts without a boundary condition, read from the state array where an exchange
+    would write them).  Each range timed alone in mixed precision; the slowest sets the
+    iteration time of a partitioned run (before any exchange)."""
+    from graphite_amd.partition import shard_elements
+    log(f"bench: swept wing in {parts} shards")
+    n = m.n_cells
+    bounds = [n * k // parts for k in range(parts + 1)]
+    mach = wing_mach()
+    rows = []
+    for k in range(parts):
+        sm = gr.Mesh.build(shard_elements(e, m, c2o, bounds[k], bounds[k + 1]))
+        sm.renumber_hilbert()
+        sm.compute_metrics()
+        ss = sm.sizes()
+        halo = int(np.diff(sm.connectivity()["ghost_cell_starts"])[sm.region_id("halo")]) \
+            if "halo" in sm.region_names() else 0
+        sv = gr.Solver(sm, dtype=gr.GR_F32, cfl=0.5)
+        for name, kind in WING_BCS:
+            if name in sm.region_names():
+                sv.set_bc(name, gr.GR_BC_FARFIELD if kind == "farfield" else gr.GR_BC_SLIP, mach, P_INF, T_INF)
+        sv.init_freestream(mach, P_INF, T_INF)
+        tile = sv.autotune(10)
+        sv.iterate(11)
+        dt = timed_steps(sv, steps)
+        prof = sv.profile(20)
+        rows.append({"cells": ss.n_cells, "halo_faces": halo, "ghosts": ss.n_ghosts, "tile_cells": tile,
+                     "ms_per_step": dt / steps * 1e3, "grad_lim_ms": prof["grad_lim"],
+                     "flux_update_ms": prof["flux_update"]})
+        del sv, sm
+    slowest = max(r["ms_per_step"] for r in rows)
+    return {"parts": parts, "partition": "contiguous ranges of the Hilbert-curve cell order, one halo layer",
+            "shards": rows, "slowest_ms_per_step": slowest}
+
+
 def kernel_table(prof, kb, traffic_tag=None):
     return {k: {"ms": prof[k], "bytes": kb[k],
                 "GBps": kb[k] / (prof[k] * 1e-3) / 1e9 if prof[k] > 0 else None,
@@ -487,6 +524,7 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=100)
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-nsi", action="store_true")
+    ap.add_argument("--no-shards", action="store_true", help="skip timing the wing's 8 shards")
     ap.add_argument("--only-headline", action="store_true", help="the swept-wing headline alone (profiling)")
     ap.add_argument("--only-nsi", action="store_true", help="the cavity leg alone (profiling)")
     ap.add_argument("--only-mlp", action="store_true", help="the warm-start network's kernels alone (profiling)")
@@ -525,7 +563,7 @@ def main():
     log("bench: building the swept-wing mesh")
     e = gr.Elements.wing(scale=WING_SCALE)
     m = gr.Mesh.build(e)
-    m.renumber_hilbert()
+    c2o, _ = m.renumber_hilbert()
     m.compute_metrics()
     s = m.sizes()
     mach = wing_mach()
@@ -598,6 +636,12 @@ def main():
                                    "normalisation": "residual of iteration 0 (reference semantics)",
                                    "lift_drag": {"C_L": cl, "C_D": cd, "s_ref": s_ref}}
             del svc
+        if not args.no_shards:
+            sh = shard_leg(gr, e, m, c2o, max(20, args.steps // 4))
+            full = dt / args.steps * 1e3
+            sh["whole_mesh_ms_per_step"] = full
+            sh["kernel_speedup_at_8"] = full / sh["slowest_ms_per_step"]
+            wing["shards_1_of_8"] = sh
         out["swept_wing"] = wing
         out["stream_copy_GBps"] = gr.stream_copy_gbps(1 << 30, 20)
         if not args.no_ogrid:
