@@ -219,6 +219,7 @@ __device__ inline double sum_parts(const double *__restrict__ p, int nb) {
 // The CG kernels run blocks of kCgBlock threads: a quarter of the partials a 256-thread grid
 // leaves, which every block of the next CG launch reads again.
 constexpr int kCgBlock = 1024;
+constexpr int kCgRow = 6;   // operator row entries held in registers by the CG iteration
 
 // Sums of three values over a block of kCgBlock threads (valid in thread 0): wave sums by a
 // shuffle tree, then the wave sums in wave order (fixed order: equal inputs give equal bits).
@@ -344,11 +345,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(const double *__restrict__ 
 }
 
 // Iteration k (slot = k & 1: static in a replayed graph of an even number of iterations).  Every
-// block first sums the previous launch's partials itself, in the same fixed order as every other
+// block sums the previous launch's partials itself, in the same fixed order as every other
 // block, so all blocks take the same decisions with no one-block reduction kernel and no
 // device-scope fence (block 0 records the scalars): stop when |r| < tol |b| or k >= cg_max
 // (k >= 1), or when the alpha denominator is 0.  Then per cell: p, s, x, r, u and w = A u (the
-// neighbours' u by cg_next_u), and the partials of the next gamma, delta and r.r.
+// neighbours' u by cg_next_u), and the partials of the next gamma, delta and r.r.  The cell's
+// and its neighbours' values do not depend on this iteration's alpha and beta (each cell writes
+// only its own p and x; r, w and s are read from the other buffer of their pair), so they are
+// loaded before the partial sums, whose latency they then share: the first kCgRow entries of the
+// row into registers (a hexahedron's 6 faces, the longest row of the operator), any further
+// ones afterwards, in the same order.
 template <typename S>
 __global__ __launch_bounds__(kCgBlock) void k_cg_iter(uint32_t N, const uint32_t *__restrict__ start,
                                                     const uint32_t *__restrict__ col, const S *__restrict__ coef,
@@ -362,6 +368,31 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_iter(uint32_t N, const uint32_t
     if (cg->done) return;
     __shared__ double s_alpha, s_beta;
     __shared__ int s_stop, s_first;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double di = 0, nd[kCgRow];
+    S ri = S(0), wi = S(0), p_old = S(0), s_old = S(0), xi = S(0), nr[kCgRow], nw[kCgRow], ns[kCgRow], nc[kCgRow];
+    uint32_t k0 = 0, k1 = 0;
+    if (i < N) {
+        di = diag[i];
+        ri = r_in[i];
+        wi = w_in[i];
+        p_old = p[i];
+        s_old = s_in[i];
+        xi = x[i];
+        k0 = start[i];
+        k1 = start[i + 1];
+#pragma unroll
+        for (int e = 0; e < kCgRow; ++e) {
+            if (k0 + e < k1) {
+                const uint32_t j = col[k0 + e];
+                nc[e] = coef[k0 + e];
+                nr[e] = r_in[j];
+                nw[e] = w_in[j];
+                ns[e] = s_in[j];
+                nd[e] = diag[j];
+            }
+        }
+    }
     double gam, del, rr;
     cg_sum_parts3(q_in, nb, gam, del, rr);
     if (threadIdx.x == 0) {
@@ -399,19 +430,19 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_iter(uint32_t N, const uint32_t
     if (s_stop) return;
     const S al = (S)s_alpha, be = (S)s_beta;
     const bool first = s_first != 0;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double g = 0, d = 0, rr1 = 0;
     if (i < N) {
-        const double di = diag[i];
-        const S ri = r_in[i], wi = w_in[i];
         const S ui = cg_precond<S>(ri, di);
-        const S pi = first ? ui : ui + be * p[i];
-        const S si = first ? wi : wi + be * s_in[i];
-        x[i] += al * pi;
+        const S pi = first ? ui : ui + be * p_old;
+        const S si = first ? wi : wi + be * s_old;
+        x[i] = xi + al * pi;
         const S r1 = ri - al * si;
         const S u1 = cg_precond<S>(r1, di);
         S w1 = S(0);
-        for (uint32_t k = start[i]; k < start[i + 1]; ++k) {
+#pragma unroll
+        for (int e = 0; e < kCgRow; ++e)
+            if (k0 + e < k1) w1 += nc[e] * (u1 - cg_next_u<S>(nr[e], nw[e], first ? S(0) : ns[e], nd[e], al, be, first));
+        for (uint32_t k = k0 + kCgRow; k < k1; ++k) {
             const uint32_t j = col[k];
             w1 += coef[k] * (u1 - cg_next_u<S>(r_in[j], w_in[j], first ? S(0) : s_in[j], diag[j], al, be, first));
         }
