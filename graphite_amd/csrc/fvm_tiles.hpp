@@ -318,6 +318,12 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
                                                          const uint32_t *__restrict__ tiles, uint32_t ntiles,
                                                          int kind, int corr, SolverCtl *ctl,
                                                          const double *__restrict__ partials, double *hist) {
+    // the tile record is read before the stop flag (nothing is written before the test), so
+    // the two loads overlap instead of following each other
+    const uint32_t *tr = tiles + (size_t)xcd_tile(blockIdx.x, ntiles) * kTileRec;
+    const uint32_t c0 = tr[0], q0 = tr[1];
+    const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
+    const uint8_t *starts = reinterpret_cast<const uint8_t *>(tr + 3);
     if (ctl) {
         __shared__ int s_stop;
         if (blockIdx.x == 0 && ctl->pending) {   // uniform across the block
@@ -349,10 +355,6 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     auto max_at = [&](int c, int sb) -> S & {
         return kTerms ? s_buf[5 + c][sb + 1] : s_buf[5 + (2 * c + 1) % 4][sb + (2 * c + 1) / 4];
     };
-    const uint32_t *tr = tiles + (size_t)xcd_tile(blockIdx.x, ntiles) * kTileRec;
-    const uint32_t c0 = tr[0], q0 = tr[1];
-    const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
-    const uint8_t *starts = reinterpret_cast<const uint8_t *>(tr + 3);
     const int t = threadIdx.x;
     // loads of the per-(component, cell) phase B (the cell's LSQ matrix and value), issued first
     // so that their latency hides under the slot gathers of phase A
@@ -501,6 +503,12 @@ template <typename S, bool CORR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 4 : 8, 8))) void k_flux_tile(MeshView m, SolverArgs<S> a, BcTable bc,
                                                       const uint32_t *__restrict__ tiles, uint32_t ntiles, double cfl,
                                                       double *partials, SolverCtl *ctl) {
+    // the tile record is read before the stop flag (nothing is written before the test), so
+    // the two loads overlap instead of following each other
+    const uint32_t *tr = tiles + (size_t)xcd_tile(blockIdx.x, ntiles) * kTileRec;
+    const uint32_t c0 = tr[0], q0 = tr[1];
+    const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
+    const uint8_t *starts = reinterpret_cast<const uint8_t *>(tr + 3);
     if (ctl) {
         if (ctl->stop) return;
         if (blockIdx.x == 0 && threadIdx.x == 0) ctl->pending = 1;   // partials for the next launch to sum
@@ -509,10 +517,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
     __shared__ S s_c[5][kTileSlots];   // central flux term ((F0+F1)*ds)*0.5
     __shared__ S s_d[5][kTileSlots];   // dissipation term ((eig*(U1-U0))*0.5)*ds
     __shared__ S s_e[kTileSlots];      // eig*ds
-    const uint32_t *tr = tiles + (size_t)xcd_tile(blockIdx.x, ntiles) * kTileRec;
-    const uint32_t c0 = tr[0], q0 = tr[1];
-    const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
-    const uint8_t *starts = reinterpret_cast<const uint8_t *>(tr + 3);
     const int t = threadIdx.x;
     const bool in_b = t < 5 * ncell;   // one thread per (component, cell) in the update
     const int cc_ = in_b ? t / ncell : 0, cl = in_b ? t - cc_ * ncell : 0;
