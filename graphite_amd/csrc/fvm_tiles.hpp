@@ -369,6 +369,22 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         for (int k = 0; k < 6; ++k) bmat[k] = a.lsq[k * a.lsq_stride + ic];
         bu = a.u[(size_t)ic * kUR + bc_c];
     }
+    // What a slot reads by its own index (its record, the neighbour's state or the ghost's face
+    // normal): in binary64 it is loaded before the barrier that the slot -> cell lookup needs,
+    // so that its latency runs alongside the lookup's (wing: 164 -> 158 us); in binary32 that
+    // was slower (79.3 -> 81.2 us), and the loads follow the cell's own state after the barrier.
+    constexpr bool kEarly = sizeof(S) == 8;
+    S dw[kGradRec], uj[5], gn[4];   // dw: d = x_j - x_i, 1/|d|^2, fc - x_i (phase C), j
+    int r = -1;                     // the ghost neighbour's BC region (-1: state from the array)
+    if constexpr (kEarly) {
+        if (t < nslot) {
+            load_vec<S, kGradRec>(a.grec + (size_t)(q0 + t) * kGradRec, dw);
+            const uint32_t j = bits_id<S>(dw[7]);
+            if (j >= m.N) r = bc_region(bc, j - m.N);
+            if (r < 0) load_state<S>(a.u, j, uj);
+            else load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, gn);
+        }
+    }
     for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = starts[k];
     __syncthreads();
 
@@ -377,23 +393,24 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     S ui[5], d0[3] = {S(0), S(0), S(0)};
     if (t < nslot) {
         lc = tile_cell_of(s_start, ncell, (uint32_t)t);
-        const uint32_t i = c0 + lc, q = q0 + t;
+        const uint32_t i = c0 + lc;
         load_state<S>(a.u, i, ui);
-        S dw[kGradRec];   // d = x_j - x_i, 1/|d|^2, fc - x_i (phase C), j
-        load_vec<S, kGradRec>(a.grec + (size_t)q * kGradRec, dw);
-        const uint32_t j = bits_id<S>(dw[7]);
+        if constexpr (kEarly) {
+            if (r >= 0) ghost_state<S>(bc.r[r], gn, ui, uj);
+        } else {
+            load_vec<S, kGradRec>(a.grec + (size_t)(q0 + t) * kGradRec, dw);
+            const uint32_t j = bits_id<S>(dw[7]);
+            r = j >= m.N ? bc_region(bc, j - m.N) : -1;
+            if (r < 0) {
+                load_state<S>(a.u, j, uj);
+            } else {
+                load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, gn);
+                ghost_state<S>(bc.r[r], gn, ui, uj);
+            }
+        }
         d0[0] = dw[4];
         d0[1] = dw[5];
         d0[2] = dw[6];
-        S uj[5];
-        const int r = j >= m.N ? bc_region(bc, j - m.N) : -1;
-        if (r < 0) {
-            load_state<S>(a.u, j, uj);
-        } else {
-            S n[4];
-            load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, n);
-            ghost_state<S>(bc.r[r], n, ui, uj);
-        }
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const S du = uj[c] - ui[c];
@@ -520,15 +537,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
     const int t = threadIdx.x;
     const bool in_b = t < 5 * ncell;   // one thread per (component, cell) in the update
     const int cc_ = in_b ? t / ncell : 0, cl = in_b ? t - cc_ * ncell : 0;
+    // What a slot reads by its own index (neighbour, record, the neighbour's state): in binary64
+    // it is loaded before the barrier that the slot -> cell lookup needs, so that its latency
+    // runs alongside the lookup's (wing: 221.6 -> 219.0 us); in binary32 that was slower (96 ->
+    // 102 us, with the neighbour's gradient record as well), and the loads stay where they were.
+    constexpr bool kEarly = sizeof(S) == 8;
+    uint32_t j = 0;
+    int jr = -1;   // the ghost neighbour's BC region (-1: state read from the array)
+    S geo[kFluxRec], uj[5];
+    if constexpr (kEarly) {
+        if (t < nslot) {
+            j = m.cc[q0 + t];
+            load_vec<S, kFluxRec, 2 * sizeof(S) == 16 ? 16 : 8>(a.frec + (size_t)(q0 + t) * kFluxRec, geo);
+            if (j >= m.N) jr = bc_region(bc, j - m.N);
+            if (jr < 0) load_state<S>(a.u, j, uj);
+        }
+    }
     for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = starts[k];
     __syncthreads();
 
     if (t < nslot) {
         const int lc = tile_cell_of(s_start, ncell, (uint32_t)t);
         const uint32_t i = c0 + lc, q = q0 + t;
-        const uint32_t j = m.cc[q];
-        S geo[kFluxRec];
-        load_vec<S, kFluxRec, 2 * sizeof(S) == 16 ? 16 : 8>(a.frec + (size_t)q * kFluxRec, geo);
+        if constexpr (!kEarly) {
+            j = m.cc[q];
+            load_vec<S, kFluxRec, 2 * sizeof(S) == 16 ? 16 : 8>(a.frec + (size_t)q * kFluxRec, geo);
+        }
         const S n[3] = {geo[0], geo[1], geo[2]};
         const S ds = geo[3];
         const S d0[3] = {geo[4], geo[5], geo[6]};
@@ -538,16 +572,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
         S r0, ru0[3], e0, r1, ru1[3], e1;
         recon_at<S, CORR>(a.gl, i, ui, d0, r0, ru0, e0);
         if (j < m.N) {
-            S uj[5];
-            load_state<S>(a.u, j, uj);
+            if constexpr (!kEarly) load_state<S>(a.u, j, uj);
             recon_at<S, CORR>(a.gl, j, uj, d1, r1, ru1, e1);
         } else {   // ghost: its state from the BC, gradient and limiter of cell i
             S ug[5];
-            const int r = bc_region(bc, j - m.N);
-            if (r < 0) {
-                load_state<S>(a.u, j, ug);
+            if constexpr (!kEarly) jr = bc_region(bc, j - m.N);
+            if (jr < 0) {
+                if constexpr (kEarly) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) ug[c] = uj[c];
+                } else {
+                    load_state<S>(a.u, j, ug);
+                }
             } else {
-                ghost_state<S>(bc.r[r], n, ui, ug);
+                ghost_state<S>(bc.r[jr], n, ui, ug);
             }
             recon_at<S, CORR>(a.gl, i, ug, d1, r1, ru1, e1);   // cell i's gradient and limiter
         }
