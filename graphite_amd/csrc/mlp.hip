@@ -599,7 +599,11 @@ struct gr_mlp {
     int cus = 256;   // compute units of the device
     gr::DevArray<float> p, m, v, g;
     gr::DevArray<gr::bf16> WT[4], Wb[4];
-    int cap = 0;   // rows of the activation buffers (a multiple of 4096)
+    // Row capacities (multiples of 4096): the forward pass's buffers (X0, H, out), the host
+    // input staging (xin) and the training-only buffers (strips, targets, gradients, partials),
+    // each grown only by what uses it, so that evaluating chunks of 2^18 rows does not grow the
+    // training buffers to that size.
+    int cap = 0, xcap = 0, tcap = 0;
     gr::DevArray<float> xin, tgt, out, d32, part, cpart, colpart;
     // activations row-major and in strips (X0s, Hs), output-gradient rows ping-pong (d16[2],
     // row-major and d16s[2] in strips)
@@ -615,27 +619,35 @@ struct gr_mlp {
             (void)hipStreamDestroy(stream);
         }
     }
+    static int rows_cap(int M) { return ((M + 4095) / 4096) * 4096; }
     void ensure(int M) {
         if (M <= cap) return;
-        cap = ((M + 4095) / 4096) * 4096;
-        xin.alloc((size_t)cap * gr::kMlpIn);
-        tgt.alloc((size_t)cap * gr::kMlpOut);
+        cap = rows_cap(M);
         out.alloc((size_t)cap * gr::kMlpOut);
         X0.alloc((size_t)cap * gr::kMlpIn);
-        X0s.alloc((size_t)cap * gr::kMlpIn);
-        for (int l = 0; l < 3; ++l) {
-            H[l].alloc((size_t)cap * gr::kMlpW);
-            Hs[l].alloc((size_t)cap * gr::kMlpW);
-        }
+        for (int l = 0; l < 3; ++l) H[l].alloc((size_t)cap * gr::kMlpW);
+    }
+    void ensure_host_input(int M) {
+        if (M <= xcap) return;
+        xcap = rows_cap(M);
+        xin.alloc((size_t)xcap * gr::kMlpIn);
+    }
+    void ensure_train(int M) {
+        ensure(M);
+        if (M <= tcap) return;
+        tcap = rows_cap(M);
+        tgt.alloc((size_t)tcap * gr::kMlpOut);
+        X0s.alloc((size_t)tcap * gr::kMlpIn);
+        for (int l = 0; l < 3; ++l) Hs[l].alloc((size_t)tcap * gr::kMlpW);
         for (int k = 0; k < 2; ++k) {
-            d16[k].alloc((size_t)cap * gr::kMlpW);
-            d16s[k].alloc((size_t)cap * gr::kMlpW);
+            d16[k].alloc((size_t)tcap * gr::kMlpW);
+            d16s[k].alloc((size_t)tcap * gr::kMlpW);
         }
-        d32.alloc((size_t)cap * gr::kMlpOut);
-        colpart.alloc((size_t)3 * (cap / 32) * gr::kMlpW);
+        d32.alloc((size_t)tcap * gr::kMlpOut);
+        colpart.alloc((size_t)3 * (tcap / 32) * gr::kMlpW);
         part.alloc(4 * kPartLayer);
         cpart.alloc(4 * kCpartLayer);
-        lpart.alloc((size_t)cap * gr::kMlpOut / gr::kBlock + 1);
+        lpart.alloc((size_t)tcap * gr::kMlpOut / gr::kBlock + 1);
     }
     void pack(hipStream_t s) {
         for (int l = 0; l < 4; ++l)
@@ -644,6 +656,7 @@ struct gr_mlp {
     }
     // strips: also write the input's strip layout (training)
     void upload_x(const float *x, int M, bool strips) {
+        ensure_host_input(M);
         GR_HIP(hipMemcpyAsync(xin.p, x, (size_t)M * n_in * sizeof(float), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(gr::k_pack_input, dim3(gr::blocks_for((size_t)M * gr::kMlpIn)), dim3(gr::kBlock), 0, stream,
                            xin.p, M, (int)n_in, X0.p, strips ? X0s.p : nullptr);
@@ -710,7 +723,7 @@ struct gr_mlp {
         for (int l = 0, b1 = 0, b2 = 0; l < 4; ++l) {
             const int N = Np[l], rows = l == 3 ? M : hrows, per = colsum_per(rows);
             const int chunks = (rows + per - 1) / per, nb = (N + 63) / 64;
-            const float *Z = l == 3 ? d32.p : colpart.p + (size_t)l * (cap / 32) * gr::kMlpW;
+            const float *Z = l == 3 ? d32.p : colpart.p + (size_t)l * (tcap / 32) * gr::kMlpW;
             float *P = cpart.p + l * kCpartLayer;
             c1.job[l] = gr::ColSum{Z, P, rows, N, per};
             c1.start[l] = b1;
@@ -743,7 +756,7 @@ struct gr_mlp {
             hipLaunchKernelGGL(l == 2 ? (gr::k_gemm_ws<gr::kEpiDtanh, gr::kMlpOut>) : (gr::k_gemm_ws<gr::kEpiDtanh, gr::kMlpW>),
                                gh, blk, 0, stream, d16[cur].p, Np[l + 1], M, Wb[l + 1].p, (const float *)nullptr,
                                (const gr::bf16 *)Hs[l].p, l > 0 ? d16[nxt].p : nullptr, d16s[nxt].p,
-                               colpart.p + (size_t)l * (cap / 32) * gr::kMlpW);
+                               colpart.p + (size_t)l * (tcap / 32) * gr::kMlpW);
             wgrad<4>(l, l == 0 ? X0s.p : Hs[l - 1].p, d16s[nxt].p, M);
             cur = nxt;
         }
@@ -919,7 +932,7 @@ int gr_mlp_loss_grad(gr_mlp *n, const float *x, const float *t, int M, double *l
         check_dims(n, x, M);
         require(t && loss, "null argument");
         require(M <= (1 << 20), "at most 2^20 rows per gradient step");
-        n->ensure(M);
+        n->ensure_train(M);
         n->upload_x(x, M, true);
         GR_HIP(hipMemcpyAsync(n->tgt.p, t, (size_t)M * n->n_out * sizeof(float), hipMemcpyHostToDevice, n->stream));
         n->forward(M, n->stream, true);
@@ -938,7 +951,7 @@ int gr_mlp_train_step(gr_mlp *n, const float *x, const float *t, int M, float lr
         require(t && loss, "null argument");
         require(M <= (1 << 20), "at most 2^20 rows per training step");
         require(lr > 0, "learning rate must be positive");
-        n->ensure(M);
+        n->ensure_train(M);
         n->upload_x(x, M, true);
         GR_HIP(hipMemcpyAsync(n->tgt.p, t, (size_t)M * n->n_out * sizeof(float), hipMemcpyHostToDevice, n->stream));
         n->forward(M, n->stream, true);
@@ -969,7 +982,7 @@ int gr_mlp_train(gr_mlp *n, int batch, int steps, float lr, float lr_final, uint
         require(batch >= 1 && batch <= (1 << 20) && (uint64_t)batch <= n->n_rows, "batch must be in 1..min(2^20, rows)");
         require(steps >= 1, "steps must be positive");
         require(lr > 0 && lr_final > 0, "learning rates must be positive");
-        n->ensure(batch);
+        n->ensure_train(batch);
         gr::DevArray<double> ldev;
         ldev.alloc((size_t)steps);
         const uint64_t Mall = n->n_rows, per_epoch = Mall / (uint64_t)batch;
