@@ -213,8 +213,12 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const bf16 *__restrict__ X, int
 // the output columns 64w..64w+63 and loads their weight fragments for all of K once, then the
 // block walks the 32-row tiles blockIdx.x, blockIdx.x + gridDim.x, ... (its four waves on the
 // same rows), so a weight is fetched once per block instead of once per wave and k-step.  Row
-// fragments are loaded two k-steps ahead of the MFMAs, across tile boundaries (the last tile
-// re-reads its own rows in place of a next one).  The k-steps and MFMAs of each output element
+// tiles go through LDS, double-buffered: each thread issues its share of the next tile's rows
+// (16-byte chunks, coalesced along the rows) before the current tile's MFMAs and writes it to the
+// other buffer after the epilogue, so the rows' latency hides under a whole tile of work (with
+// fragments loaded two k-steps ahead from HBM it was exposed every tile).  The buffer holds the
+// tile fragment-major -- chunk (k-step s, row block r, lane l) at (2 s + r) * 64 + l -- so a
+// wave's fragment read is one contiguous 1 KB run.  The k-steps and MFMAs of each output element
 // are those of k_gemm<EPI, 4, K>, in the same order: the same bits.
 template <int EPI, int K>
 __global__ __launch_bounds__(kBlock) void k_gemm_ws(const bf16 *__restrict__ X, int ldx, int M,
@@ -223,8 +227,12 @@ __global__ __launch_bounds__(kBlock) void k_gemm_ws(const bf16 *__restrict__ X, 
                                                     bf16 *__restrict__ Ys, float *__restrict__ colpart) {
     static_assert(EPI != kEpiLinear, "FP32 outputs go through k_gemm");
     constexpr int TN = 4, TC = 16 * TN, N = kMlpW;
-    constexpr int S = K / 32, D = S < 2 ? S : 2;   // k-steps; k-steps of A in flight (2: two waves per SIMD at K = 256)
+    constexpr int S = K / 32;                          // k-steps
+    constexpr int RC = K / 8;                          // 16-byte chunks per row
+    constexpr int CH = 32 * RC;                        // chunks per row tile
+    constexpr int CPT = (CH + kBlock - 1) / kBlock;    // chunks per thread
     static_assert((kBlock / 64) * TC == N, "the block's waves cover the columns");
+    __shared__ bf16x8 sA[2][2 * S * 64];
     __shared__ __attribute__((aligned(16))) bf16 sT[kBlock / 64][32][TC + 8];
     __shared__ __attribute__((aligned(16))) bf16 sS[kBlock / 64][4][TC][8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -237,49 +245,56 @@ __global__ __launch_bounds__(kBlock) void k_gemm_ws(const bf16 *__restrict__ X, 
     for (int s = 0; s < S; ++s)
 #pragma unroll
         for (int t = 0; t < TN; ++t) fb[s][t] = *(const bf16x8 *)(BT + (size_t)(n0 + t * 16 + lr) * K + 32 * s + 8 * lg);
-    // row offsets (elements of X) of this lane's two rows in the current and the next tile;
-    // offsets rather than pointers, so that every load is known to address global memory
-    // (a pointer chosen between two tiles becomes generic, and generic loads are waited for one
-    // by one)
-    size_t xa[2];
+    // this thread's chunks of a row tile: chunk q = threadIdx.x + kBlock i is row q / RC (rows at
+    // or past M repeat row M-1), k-chunk q % RC; its place in the fragment-major buffer
+    int dst[CPT];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) xa[r] = (size_t)min(tile * 32 + r * 16 + lr, M - 1) * ldx + 8 * lg;
-    bf16x8 fa[D][2];
+    for (int i = 0; i < CPT; ++i) {
+        const int q = (int)threadIdx.x + kBlock * i, R = q / RC, c = q % RC;
+        dst[i] = (2 * (c >> 2) + (R >> 4)) * 64 + (R & 15) + 16 * (c & 3);
+    }
+    auto fetch = [&](int tl, bf16x8 (&v)[CPT]) {
 #pragma unroll
-    for (int s = 0; s < D; ++s)
+        for (int i = 0; i < CPT; ++i) {
+            const int q = (int)threadIdx.x + kBlock * i;
+            if (CH % kBlock == 0 || q < CH)
+                v[i] = *(const bf16x8 *)(X + (size_t)min(tl * 32 + q / RC, M - 1) * ldx + 8 * (q % RC));
+        }
+    };
+    auto stage = [&](int b, const bf16x8 (&v)[CPT]) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) fa[s][r] = *(const bf16x8 *)(X + xa[r] + 32 * s);
+        for (int i = 0; i < CPT; ++i)
+            if (CH % kBlock == 0 || (int)threadIdx.x + kBlock * i < CH) sA[b][dst[i]] = v[i];
+    };
+    bf16x8 pre[CPT];
+    fetch(tile, pre);
+    stage(0, pre);
+    __syncthreads();
+    int buf = 0;
     for (;;) {
         const int next = tile + (int)gridDim.x;
         const bool more = next < ntiles;
-        size_t xn[2];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) xn[r] = (size_t)min((more ? next : tile) * 32 + r * 16 + lr, M - 1) * ldx + 8 * lg;
+        fetch(more ? next : tile, pre);   // the last tile's own rows again when there is no next one
         f32x4 acc[2][TN];
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
             for (int t = 0; t < TN; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const int q = s % D;
+            const bf16x8 a0 = sA[buf][(2 * s) * 64 + lane], a1 = sA[buf][(2 * s + 1) * 64 + lane];
 #pragma unroll
             for (int t = 0; t < TN; ++t) {
-                acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[q][0], fb[s][t], acc[0][t], 0, 0, 0);
-                acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[q][1], fb[s][t], acc[1][t], 0, 0, 0);
+                acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, fb[s][t], acc[0][t], 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, fb[s][t], acc[1][t], 0, 0, 0);
             }
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-                fa[q][r] = *(const bf16x8 *)(X + (s + D < S ? xa[r] + 32 * (s + D) : xn[r] + 32 * (s + D - S)));
-            __builtin_amdgcn_sched_barrier(0);
         }
         gemm_epilogue<EPI, TN>(acc, tile * 32, n0, M, N, bias, Hs, Y, Ys, nullptr, colpart, sT[w], sS[w]);
         if (!more) break;
+        stage(buf ^ 1, pre);
+        __syncthreads();   // the next tile's rows are in place; the staging tiles are free again
+        buf ^= 1;
         tile = next;
-#pragma unroll
-        for (int r = 0; r < 2; ++r) xa[r] = xn[r];
-        __syncthreads();   // the next epilogue rewrites the staging tiles
     }
 }
 
