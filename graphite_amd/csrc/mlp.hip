@@ -38,7 +38,15 @@ constexpr int kWgradChunks = 32;   // most row chunks (blocks along rows) of a w
 // operand fragments is one 16-byte load from this layout, with no transposition.
 __device__ inline size_t strip_index(int r, int c, int C) { return ((size_t)(r >> 3) * C + c) * 8 + (r & 7); }
 
-enum { kEpiTanh = 0, kEpiLinear = 1, kEpiDtanh = 2 };
+enum { kEpiTanh = 0, kEpiLinear = 1, kEpiDtanh = 2, kEpiMse = 3 };
+
+// Targets and outputs of the output layer's training epilogue (kEpiMse of k_gemm).
+struct MseArgs {
+    const float *t;   // targets [M][n_out]
+    int n_out;
+    float *d32;       // dOut, FP32 row-major [M][kMlpOut]
+    double *lpart;    // per-block partial sums of the squared error
+};
 
 // tanh for values about to be rounded to BF16, within about 3 ulp of binary32 (a correctly
 // rounded tanh would still differ from it in the BF16 result of a few values in a million, as any
@@ -151,13 +159,17 @@ __device__ __forceinline__ void gemm_epilogue(const f32x4 (&acc)[2][TN], int row
 // row/column l mod 16 of its operand and eight consecutive k of group l/16), operands loaded
 // straight from global memory (the weights stay in L2): the rows four k-steps ahead of the
 // MFMAs, the weights one; a block's 4 waves take the N/(16*TN) column tiles of 4*16*TN/N row
-// groups.
+// groups.  kEpiMse (the output layer in training, N = kMlpOut): o = acc + bias is compared with
+// the targets, dOut = 2 (o - t) / (M n_out) (0 in padded columns) goes to Y (BF16 row-major),
+// Ys (strips) and mse.d32, and the block's sum of (o - t)^2 to mse.lpart[blockIdx.x]; rows at or
+// past M write nothing.  o is the binary32 value the inference epilogue stores, so the gradient
+// equals the one formed from stored outputs.
 template <int EPI, int TN, int K>
 __global__ __launch_bounds__(kBlock) void k_gemm(const bf16 *__restrict__ X, int ldx, int M,
                                                  const bf16 *__restrict__ BT, int N, const float *__restrict__ bias,
                                                  const bf16 *__restrict__ Hs, bf16 *__restrict__ Y,
                                                  bf16 *__restrict__ Ys, float *__restrict__ Y32,
-                                                 float *__restrict__ colpart) {
+                                                 float *__restrict__ colpart, MseArgs mse) {
     constexpr int TC = 16 * TN;
     constexpr int S = K / 32, D = S < 4 ? S : 4;   // k-steps; k-steps of A in flight
     __shared__ __attribute__((aligned(16))) bf16 sT[kBlock / 64][32][TC + 8];   // row-major tile of each wave
@@ -206,7 +218,37 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const bf16 *__restrict__ X, int
         }
         __builtin_amdgcn_sched_barrier(0);
     }
-    gemm_epilogue<EPI, TN>(acc, row0, n0, M, N, bias, Hs, Y, Ys, Y32, colpart, sT[w], sS[w]);
+    if constexpr (EPI == kEpiMse) {
+        double se = 0;
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int t = 0; t < TN; ++t) {
+                const int col = n0 + t * 16 + lr;
+                const float bb = bias[col];
+                const int R = row0 + r * 16 + 4 * lg;   // first of this lane's 4 rows (a multiple of 4)
+                bf16x4 o;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float g = 0.f;
+                    if (R + j < M && col < mse.n_out) {
+                        const float diff = (acc[r][t][j] + bb) - mse.t[(size_t)(R + j) * mse.n_out + col];
+                        se += (double)diff * (double)diff;
+                        g = 2.f * diff / ((float)M * (float)mse.n_out);
+                    }
+                    o[j] = (bf16)g;
+                    if (R + j < M) {
+                        Y[(size_t)(R + j) * N + col] = o[j];
+                        mse.d32[(size_t)(R + j) * N + col] = g;
+                    }
+                }
+                if (R < M) *(bf16x4 *)&Ys[strip_index(R, col, N)] = o;
+            }
+        const double bs = block_sum(se);
+        if (threadIdx.x == 0) mse.lpart[blockIdx.x] = bs;
+    } else {
+        gemm_epilogue<EPI, TN>(acc, row0, n0, M, N, bias, Hs, Y, Ys, Y32, colpart, sT[w], sS[w]);
+    }
 }
 
 // Y[M x 256] = epilogue(X[M x K] * B) with the weights held in registers: wave w of a block owns
@@ -530,30 +572,6 @@ __global__ void k_gather_rows(const float *__restrict__ xall, const float *__res
     for (int c = c0; c < c0 + 8 && c < n_out; ++c) T[(size_t)r * n_out + c] = tall[src * n_out + c];
 }
 
-// Mean squared error: dOut = 2 (out - t) / (M n_out) (BF16 row-major and in strips for the
-// GEMMs, FP32 for the bias gradient), padded columns 0; per-block partial sums of the squared
-// error.
-__global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ out, const float *__restrict__ t, int M,
-                                                int n_out, bf16 *__restrict__ d16, bf16 *__restrict__ d16s,
-                                                float *__restrict__ d32, double *__restrict__ partials) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double se = 0;
-    if (e < (size_t)M * kMlpOut) {
-        const int r = (int)(e / kMlpOut), c = (int)(e % kMlpOut);
-        float g = 0.f;
-        if (c < n_out) {
-            const float diff = out[e] - t[(size_t)r * n_out + c];
-            se = (double)diff * (double)diff;
-            g = 2.f * diff / ((float)M * (float)n_out);
-        }
-        d16[e] = (bf16)g;
-        d16s[strip_index(r, c, kMlpOut)] = (bf16)g;
-        d32[e] = g;
-    }
-    const double bs = block_sum(se);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
-}
-
 // Where each layer's weights sit in the parameter vector and their BF16 copies.
 struct PackTable {
     int woff[4], Kp[4], Np[4];
@@ -693,9 +711,14 @@ struct gr_mlp {
             hipLaunchKernelGGL(l == 0 ? (gr::k_gemm_ws<gr::kEpiTanh, gr::kMlpIn>) : (gr::k_gemm_ws<gr::kEpiTanh, gr::kMlpW>),
                                gh, blk, 0, s, in[l], ld[l], M, WT[l].p, p.p + boff[l], (const gr::bf16 *)nullptr,
                                H[l].p, strips ? Hs[l].p : nullptr, (float *)nullptr);
-        hipLaunchKernelGGL((gr::k_gemm<gr::kEpiLinear, 2, gr::kMlpW>), go, blk, 0, s, H[2].p, gr::kMlpW, M, WT[3].p,
-                           Np[3], p.p + boff[3], (const gr::bf16 *)nullptr, (gr::bf16 *)nullptr, (gr::bf16 *)nullptr,
-                           out.p, (float *)nullptr);
+        if (strips)   // training: the output layer's epilogue forms the loss gradient (k_gemm kEpiMse)
+            hipLaunchKernelGGL((gr::k_gemm<gr::kEpiMse, 2, gr::kMlpW>), go, blk, 0, s, H[2].p, gr::kMlpW, M, WT[3].p,
+                               Np[3], p.p + boff[3], (const gr::bf16 *)nullptr, d16[0].p, d16s[0].p, (float *)nullptr,
+                               (float *)nullptr, gr::MseArgs{tgt.p, (int)n_out, d32.p, lpart.p});
+        else
+            hipLaunchKernelGGL((gr::k_gemm<gr::kEpiLinear, 2, gr::kMlpW>), go, blk, 0, s, H[2].p, gr::kMlpW, M, WT[3].p,
+                               Np[3], p.p + boff[3], (const gr::bf16 *)nullptr, (gr::bf16 *)nullptr, (gr::bf16 *)nullptr,
+                               out.p, (float *)nullptr, gr::MseArgs{});
     }
     // Partial buffers per layer l: weight-gradient chunk partials at part + l * kPartLayer,
     // first-pass bias column sums at cpart + l * kCpartLayer.
@@ -753,14 +776,12 @@ struct gr_mlp {
                            loss);
         hipLaunchKernelGGL(gr::k_colsum, dim3(c2.start[4]), dim3(gr::kBlock), 0, stream, c2);
     }
-    // loss (into *loss_dst on the device) and gradients for the targets in tgt (the forward pass
-    // must have written the strips); no host sync
+    // loss (into *loss_dst on the device) and gradients for the targets in tgt (the training
+    // forward pass must have run: it writes the strips and, in the output layer's epilogue, the
+    // output gradient and the squared-error partials); no host sync
     void backward(int M, double *loss_dst) {
-        const size_t no = (size_t)M * gr::kMlpOut;
-        const unsigned nb = gr::blocks_for(no);
+        const unsigned nb = gemm_blocks(M, gr::kMlpOut, 2);   // squared-error partials of the output layer
         const dim3 blk(gr::kBlock), gh(ws_blocks(M));
-        hipLaunchKernelGGL(gr::k_mse, dim3(nb), blk, 0, stream, out.p, tgt.p, M, (int)n_out, d16[0].p, d16s[0].p,
-                           d32.p, lpart.p);
         // layer 4 (linear): dW4 = H3^T dOut
         wgrad<2>(3, Hs[2].p, d16s[0].p, M);
         // layers 3..1: dZ_l = (dZ_{l+1} W_{l+1}^T) * (1 - H_l^2) in the input-gradient product's
