@@ -382,38 +382,33 @@ __device__ inline FarField far_field(double p_inf, double T_inf) {
     return f;
 }
 
-// Raw network inputs of rows row0..row0+rows-1; with nrm set, normalised and written as
-// bfloat16 into X [rows][64] (zero padded); else binary32 into X [rows][6].
+// Raw network inputs of rows row0..row0+rows-1, one thread per row: with nrm set, normalised and
+// written as bfloat16 into the first 8 columns of X [rows][64] (the 6 inputs, then 2 zeros; the
+// columns after them are zero already and left alone); else binary32 into X [rows][6].
 template <bool NORM>
 __global__ void k_warm_features(MeshView m, const double *__restrict__ wd, const double *__restrict__ inst_mach,
                                 uint32_t n_inst, size_t row0, int rows, gr_warm_norm nrm, void *X) {
-    constexpr int W = NORM ? 64 : 6;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)rows * W) return;
-    const int c = (int)(e % W);
-    const size_t r = row0 + e / W;
+    if (e >= (size_t)rows) return;
+    const size_t r = row0 + e;
     const uint32_t i = (uint32_t)(r / n_inst), k = (uint32_t)(r % n_inst);
-    double f = 0;
-    if (c < 6) {
-        const double mx = inst_mach[3 * k], my = inst_mach[3 * k + 1], mz = inst_mach[3 * k + 2];
-        // centre coordinates compressed about the origin: (x - o) / (1 + |x - o|) resolves the
-        // body's neighbourhood and keeps the far field bounded
-        const double ex = m.cx[i] - nrm.origin[0], ey = m.cy[i] - nrm.origin[1], ez = m.cz[i] - nrm.origin[2];
-        const double q = 1.0 + dsqrt((ex * ex + ey * ey) + ez * ez);
-        switch (c) {
-        case 0: f = ex / q; break;
-        case 1: f = ey / q; break;
-        case 2: f = ez / q; break;
-        case 3: f = log(wd[i] + 1e-3); break;
-        case 4: f = dsqrt((mx * mx + my * my) + mz * mz); break;
-        default: f = atan2(my, mx); break;
-        }
-    }
-    if (NORM) {
-        const float v = c < 6 ? (float)((f - nrm.in_mean[c]) / nrm.in_std[c]) : 0.f;
-        ((__bf16 *)X)[e] = (__bf16)v;
+    const double mx = inst_mach[3 * k], my = inst_mach[3 * k + 1], mz = inst_mach[3 * k + 2];
+    // centre coordinates compressed about the origin: (x - o) / (1 + |x - o|) resolves the
+    // body's neighbourhood and keeps the far field bounded
+    const double ex = m.cx[i] - nrm.origin[0], ey = m.cy[i] - nrm.origin[1], ez = m.cz[i] - nrm.origin[2];
+    const double q = 1.0 + dsqrt((ex * ex + ey * ey) + ez * ez);
+    const double f[6] = {ex / q, ey / q, ez / q, log(wd[i] + 1e-3), dsqrt((mx * mx + my * my) + mz * mz),
+                         atan2(my, mx)};
+    if constexpr (NORM) {
+        typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+        bf16x8 v;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[c] = (__bf16)(float)((f[c] - nrm.in_mean[c]) / nrm.in_std[c]);
+        v[6] = v[7] = (__bf16)0.f;
+        *(bf16x8 *)((__bf16 *)X + e * kMlpInPad) = v;
     } else {
-        ((float *)X)[e] = (float)f;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ((float *)X)[e * 6 + c] = (float)f[c];
     }
 }
 
@@ -1248,7 +1243,7 @@ int gr_batch_warm_rows(gr_batch *b, const double *wall_distance, const float ori
         wd.upload(wall_distance, N, m->stream);
         gr_warm_norm none{};
         for (int k = 0; k < 3; ++k) none.origin[k] = origin[k];
-        hipLaunchKernelGGL(gr::k_warm_features<false>, dim3(gr::blocks_for(rows * 6)), dim3(gr::kBlock), 0, m->stream,
+        hipLaunchKernelGGL(gr::k_warm_features<false>, dim3(gr::blocks_for(rows)), dim3(gr::kBlock), 0, m->stream,
                            m->view(), wd.p, (const double *)b->inst_mach.p, b->n_inst, (size_t)0, (int)rows, none,
                            (void *)X.p);
         if (b->cfg.dtype == GR_F64)
@@ -1290,7 +1285,7 @@ int gr_batch_warm_start(gr_batch *b, gr_mlp *n, const double *wall_distance, con
         const gr_warm_norm nrm = *norm;
         const double *mach = (const double *)b->inst_mach.p;
         const auto fill = [&](void *X, size_t row0, int rows) {
-            hipLaunchKernelGGL(gr::k_warm_features<true>, dim3(gr::blocks_for((size_t)rows * gr::kMlpInPad)),
+            hipLaunchKernelGGL(gr::k_warm_features<true>, dim3(gr::blocks_for((size_t)rows)),
                                dim3(gr::kBlock), 0, m->stream, m->view(), wd.p, mach, I, row0, rows, nrm, X);
         };
         const auto consume = [&](const float *Y, size_t row0, int rows) {

@@ -110,7 +110,9 @@ int gr_internal_wall_force(gr_mesh *m, const void *u, int dtype, size_t stride, 
 
 // Device-side evaluation of the warm-start network (mlp.hip): rows are processed in chunks on
 // stream s; fill(X, row0, m) writes the bfloat16 inputs of rows row0..row0+m-1 into X
-// ([m][kMlpInPad], zero padded), consume(Y, row0, m) reads the binary32 outputs ([m][kMlpOutPad]).
+// ([m][kMlpInPad]: the 8-column groups that hold inputs, padding within them zero; the columns
+// after those groups are zero on entry and must stay zero), consume(Y, row0, m) reads the
+// binary32 outputs ([m][kMlpOutPad]).
 struct gr_mlp;
 namespace gr {
 constexpr int kMlpInPad = 64;

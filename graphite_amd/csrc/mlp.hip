@@ -552,23 +552,24 @@ __global__ void k_pack_input(const float *__restrict__ x, int M, int n_in, bf16 
 
 // Mini-batch of a resident training set: row r of the batch is row (a*(pos0+r) + b) mod Mall of
 // the set (a coprime to Mall, so one pass of pos over 0..Mall-1 visits every row once).  One
-// thread per (row, group of 8 columns): the source row (two 64-bit remainders) once per group,
-// one 16-byte row-major store; a wave holds 8 consecutive rows, so its strip stores fill whole
-// 16-byte strip entries.
+// thread per (row, group of 8 columns) for the G groups that hold inputs or targets: the source
+// row (two 64-bit remainders) once per group, one 16-byte row-major store; input columns past
+// the groups holding inputs are zero already and not written.
 __global__ void k_gather_rows(const float *__restrict__ xall, const float *__restrict__ tall, uint64_t Mall, int n_in,
-                              int n_out, uint64_t a, uint64_t b, uint64_t pos0, int M, bf16 *__restrict__ X,
+                              int n_out, uint64_t a, uint64_t b, uint64_t pos0, int M, int G, bf16 *__restrict__ X,
                               bf16 *__restrict__ Xs, float *__restrict__ T) {
-    constexpr int G = kMlpIn / 8;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)M * G) return;
     const int r = (int)(e / G), c0 = 8 * (int)(e % G);
     const uint64_t src = (a * ((pos0 + r) % Mall) + b) % Mall;
-    bf16x8 v;
+    if (c0 < n_in) {
+        bf16x8 v;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = (bf16)(c0 + k < n_in ? xall[src * n_in + c0 + k] : 0.f);
-    *(bf16x8 *)&X[(size_t)r * kMlpIn + c0] = v;
+        for (int k = 0; k < 8; ++k) v[k] = (bf16)(c0 + k < n_in ? xall[src * n_in + c0 + k] : 0.f);
+        *(bf16x8 *)&X[(size_t)r * kMlpIn + c0] = v;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) Xs[strip_index(r, c0 + k, kMlpIn)] = v[k];
+        for (int k = 0; k < 8; ++k) Xs[strip_index(r, c0 + k, kMlpIn)] = v[k];
+    }
     for (int c = c0; c < c0 + 8 && c < n_out; ++c) T[(size_t)r * n_out + c] = tall[src * n_out + c];
 }
 
@@ -657,7 +658,11 @@ struct gr_mlp {
         if (M <= cap) return;
         cap = rows_cap(M);
         out.alloc((size_t)cap * gr::kMlpOut);
+        // input columns past the 8-column groups that hold inputs stay zero: cleared here, and
+        // the writers (k_pack_input, k_gather_rows, the device-side fill) never write them non-zero
         X0.alloc((size_t)cap * gr::kMlpIn);
+        GR_HIP(hipMemsetAsync(X0.p, 0, X0.n * sizeof(gr::bf16), stream));
+        GR_HIP(hipStreamSynchronize(stream));   // the device-side fill writes X0 from another stream
         for (int l = 0; l < 3; ++l) H[l].alloc((size_t)cap * gr::kMlpW);
     }
     void ensure_host_input(int M) {
@@ -671,6 +676,7 @@ struct gr_mlp {
         tcap = rows_cap(M);
         tgt.alloc((size_t)tcap * gr::kMlpOut);
         X0s.alloc((size_t)tcap * gr::kMlpIn);
+        GR_HIP(hipMemsetAsync(X0s.p, 0, X0s.n * sizeof(gr::bf16), stream));
         for (int l = 0; l < 3; ++l) Hs[l].alloc((size_t)tcap * gr::kMlpW);
         for (int k = 0; k < 2; ++k) {
             d16[k].alloc((size_t)tcap * gr::kMlpW);
@@ -1022,6 +1028,8 @@ int gr_mlp_train(gr_mlp *n, int batch, int steps, float lr, float lr_final, uint
         gr::DevArray<double> ldev;
         ldev.alloc((size_t)steps);
         const uint64_t Mall = n->n_rows, per_epoch = Mall / (uint64_t)batch;
+        // 8-column groups holding inputs or targets (the gather's threads per row)
+        const int G = (int)std::max((n->n_in + 7) / 8, (n->n_out + 7) / 8);
         uint64_t a = 1, b = 0;
         for (int s = 0; s < steps; ++s) {
             const uint64_t epoch = (uint64_t)s / per_epoch, within = (uint64_t)s % per_epoch;
@@ -1032,9 +1040,9 @@ int gr_mlp_train(gr_mlp *n, int batch, int steps, float lr, float lr_final, uint
                 } while (gcd64(a, Mall) != 1);
                 b = rng() % Mall;
             }
-            hipLaunchKernelGGL(gr::k_gather_rows, dim3(gr::blocks_for((size_t)batch * (gr::kMlpIn / 8))), dim3(gr::kBlock), 0,
+            hipLaunchKernelGGL(gr::k_gather_rows, dim3(gr::blocks_for((size_t)batch * G)), dim3(gr::kBlock), 0,
                                n->stream, n->xall.p, n->tall.p, Mall, (int)n->n_in, (int)n->n_out, a, b,
-                               within * (uint64_t)batch, batch, n->X0.p, n->X0s.p, n->tgt.p);
+                               within * (uint64_t)batch, batch, G, n->X0.p, n->X0s.p, n->tgt.p);
             n->forward(batch, n->stream, true);
             n->backward(batch, ldev.p + s);
             const double cosf_ = 0.5 * (1.0 + std::cos(M_PI * (double)s / (double)steps));
