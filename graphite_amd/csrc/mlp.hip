@@ -251,7 +251,9 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const bf16 *__restrict__ X, int
     }
 }
 
-// Y[M x 256] = epilogue(X[M x K] * B) with the weights held in registers: wave w of a block owns
+// Y[M x 256] = epilogue(X[M x K] * B) (B given as BT = B^T [256 x K], rows ldb >= K apart: a
+// product over the first K columns of wider rows of X and BT when the rest are zero) with the
+// weights held in registers: wave w of a block owns
 // the output columns 64w..64w+63 and loads their weight fragments for all of K once, then the
 // block walks the 32-row tiles blockIdx.x, blockIdx.x + gridDim.x, ... (its four waves on the
 // same rows), so a weight is fetched once per block instead of once per wave and k-step.  Row
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const bf16 *__restrict__ X, int
 // are those of k_gemm<EPI, 4, K>, in the same order: the same bits.
 template <int EPI, int K>
 __global__ __launch_bounds__(kBlock) void k_gemm_ws(const bf16 *__restrict__ X, int ldx, int M,
-                                                    const bf16 *__restrict__ BT, const float *__restrict__ bias,
+                                                    const bf16 *__restrict__ BT, int ldb, const float *__restrict__ bias,
                                                     const bf16 *__restrict__ Hs, bf16 *__restrict__ Y,
                                                     bf16 *__restrict__ Ys, float *__restrict__ colpart) {
     static_assert(EPI != kEpiLinear, "FP32 outputs go through k_gemm");
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_ws(const bf16 *__restrict__ X, 
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
-        for (int t = 0; t < TN; ++t) fb[s][t] = *(const bf16x8 *)(BT + (size_t)(n0 + t * 16 + lr) * K + 32 * s + 8 * lg);
+        for (int t = 0; t < TN; ++t) fb[s][t] = *(const bf16x8 *)(BT + (size_t)(n0 + t * 16 + lr) * ldb + 32 * s + 8 * lg);
     // this thread's chunks of a row tile: chunk q = threadIdx.x + kBlock i is row q / RC (rows at
     // or past M repeat row M-1), k-chunk q % RC; its place in the fragment-major buffer
     int dst[CPT];
@@ -713,10 +715,14 @@ struct gr_mlp {
         const dim3 blk(gr::kBlock), gh(ws_blocks(M)), go(gemm_blocks(M, gr::kMlpOut, 2));
         const gr::bf16 *in[3] = {X0.p, H[0].p, H[1].p};
         const int ld[3] = {gr::kMlpIn, gr::kMlpW, gr::kMlpW};
-        for (int l = 0; l < 3; ++l)
-            hipLaunchKernelGGL(l == 0 ? (gr::k_gemm_ws<gr::kEpiTanh, gr::kMlpIn>) : (gr::k_gemm_ws<gr::kEpiTanh, gr::kMlpW>),
-                               gh, blk, 0, s, in[l], ld[l], M, WT[l].p, p.p + boff[l], (const gr::bf16 *)nullptr,
-                               H[l].p, strips ? Hs[l].p : nullptr, (float *)nullptr);
+        for (int l = 0; l < 3; ++l) {
+            // layer 1 over the first 32 input columns when the inputs fit there (the padding
+            // columns of X0 and rows of W1 are zero: the second k-step would add only zeros)
+            const auto kern = l > 0 ? gr::k_gemm_ws<gr::kEpiTanh, gr::kMlpW>
+                                    : n_in <= 32 ? gr::k_gemm_ws<gr::kEpiTanh, 32> : gr::k_gemm_ws<gr::kEpiTanh, gr::kMlpIn>;
+            hipLaunchKernelGGL(kern, gh, blk, 0, s, in[l], ld[l], M, WT[l].p, Kp[l], p.p + boff[l],
+                               (const gr::bf16 *)nullptr, H[l].p, strips ? Hs[l].p : nullptr, (float *)nullptr);
+        }
         if (strips)   // training: the output layer's epilogue forms the loss gradient (k_gemm kEpiMse)
             hipLaunchKernelGGL((gr::k_gemm<gr::kEpiMse, 2, gr::kMlpW>), go, blk, 0, s, H[2].p, gr::kMlpW, M, WT[3].p,
                                Np[3], p.p + boff[3], (const gr::bf16 *)nullptr, d16[0].p, d16s[0].p, (float *)nullptr,
@@ -796,7 +802,7 @@ struct gr_mlp {
         for (int l = 2; l >= 0; --l) {
             const int nxt = 1 - cur;
             hipLaunchKernelGGL(l == 2 ? (gr::k_gemm_ws<gr::kEpiDtanh, gr::kMlpOut>) : (gr::k_gemm_ws<gr::kEpiDtanh, gr::kMlpW>),
-                               gh, blk, 0, stream, d16[cur].p, Np[l + 1], M, Wb[l + 1].p, (const float *)nullptr,
+                               gh, blk, 0, stream, d16[cur].p, Np[l + 1], M, Wb[l + 1].p, Np[l + 1], (const float *)nullptr,
                                (const gr::bf16 *)Hs[l].p, l > 0 ? d16[nxt].p : nullptr, d16s[nxt].p,
                                colpart.p + (size_t)l * (tcap / 32) * gr::kMlpW);
             wgrad<4>(l, l == 0 ? X0s.p : Hs[l - 1].p, d16s[nxt].p, M);
