@@ -92,6 +92,40 @@ def test_training_reduces_loss_and_is_deterministic(gr):
     assert losses[-1] < 0.1 * losses[0]
 
 
+def test_wide_network_matches_oracle_and_both_training_paths_agree(gr, orc):
+    """40 inputs and 20 outputs: the first layer over all 64 padded input columns (the narrow
+    networks take a 32-column product), five 8-column input groups in the resident-set gather."""
+    n_in, n_out = 40, 20
+    rng = np.random.default_rng(31)
+    p = zero_padding(random_params(rng, 1.0), n_in, n_out).astype(np.float32)
+    net = gr.Mlp(n_in, n_out, seed=3)
+    net.set_params(p)
+    p64 = p.astype(np.float64)
+    x = rng.uniform(-1, 1, (777, n_in)).astype(np.float32)
+    y = net.forward(x)
+    assert rel(y, orc.mlp_forward(p64, x.astype(np.float64), n_out, emulate_bf16=True)) < 1e-4
+    M = 4096
+    xa = rng.uniform(-1, 1, (M, n_in)).astype(np.float32)
+    ta = rng.normal(size=(M, n_out)).astype(np.float32)
+    loss, g = net.loss_grad(xa, ta)
+    le, ge = orc.mlp_loss_grad(p64, xa.astype(np.float64), ta.astype(np.float64), emulate_bf16=True)
+    assert loss == pytest.approx(le, rel=1e-5)
+    assert rel(g, ge) < 2e-3
+    # the resident set's gather feeds the same rows (one batch = the whole set, permuted) as a
+    # host batch: the same first-step loss, and after one Adam step the same second-step loss
+    # up to the order of the row sums
+    a = gr.Mlp(n_in, n_out, seed=3)
+    a.set_params(p)
+    a.set_data(xa, ta)
+    la = a.train(M, 2, 1e-3, 1e-3, seed=5)
+    b = gr.Mlp(n_in, n_out, seed=3)
+    b.set_params(p)
+    lb = [b.train_step(xa, ta, 1e-3) for _ in range(2)]
+    assert la[0] == pytest.approx(lb[0], rel=1e-9)
+    assert la[1] == pytest.approx(lb[1], rel=1e-4)
+    assert la[1] < la[0]
+
+
 def test_invalid_arguments(gr):
     with pytest.raises(gr.GraphiteError):
         gr.Mlp(0, 3)
