@@ -286,11 +286,12 @@ def warm_forces(gr, args, m, net, norm, test, s_ref):
 
 def mlp_leg(gr, args):
     """The warm-start network's BF16-MFMA kernels on their own: training steps (batch 32,768 rows
-    of a resident synthetic set of 2^21 rows) and inference of 2^21 rows, FLOP rates from the
-    padded layer shapes (in 64, hidden 256 x 3, out 32: 2 x (64*256 + 2*256*256 + 256*32) =
-    311,296 FLOP per row forward; a training step is a forward pass, the input-gradient products
-    of layers 4..2 and the weight-gradient products of all four layers: 3x forward less the
-    first layer's input gradient)."""
+    of a resident synthetic set of 2^21 rows) and inference of 2^21 rows, FLOP rates of the
+    products the MFMAs execute (padded shapes; the first layer's forward product over 32 input
+    columns, hidden 256 x 3, out 32: 2 x (32*256 + 2*256*256 + 256*32) = 294,912 FLOP per row
+    forward; a training step adds the input-gradient products of layers 4..2,
+    2 x (32*256 + 2*256*256), and the weight-gradient products of all four layers,
+    2 x (64*256 + 2*256*256 + 256*32): 884,736 FLOP per row)."""
     log("bench: MLP leg")
     rng = np.random.default_rng(0)
     rows, batch = 1 << 21, 32768
@@ -303,8 +304,8 @@ def mlp_leg(gr, args):
     t0 = time.perf_counter()
     net.train(batch, steps, 1e-3, 1e-4)                   # returns after the losses are copied back
     dt = time.perf_counter() - t0
-    fwd_row = 2 * (64 * 256 + 2 * 256 * 256 + 256 * 32)
-    train_row = 3 * fwd_row - 2 * 64 * 256
+    fwd_row = 2 * (32 * 256 + 2 * 256 * 256 + 256 * 32)
+    train_row = fwd_row + 2 * (32 * 256 + 2 * 256 * 256) + 2 * (64 * 256 + 2 * 256 * 256 + 256 * 32)
     t1 = time.perf_counter()
     net.forward(x)
     dt_inf = time.perf_counter() - t1
