@@ -455,12 +455,21 @@ def shard_leg(gr, e, m, c2o, steps, parts=8):
             "shards": rows, "slowest_ms_per_step": slowest}
 
 
-def kernel_table(prof, kb, traffic_tag=None):
-    return {k: {"ms": prof[k], "bytes": kb[k],
-                "GBps": kb[k] / (prof[k] * 1e-3) / 1e9 if prof[k] > 0 else None,
-                "frac_of_peak": kb[k] / (prof[k] * 1e-3) / 1e9 / HBM_PEAK_GBPS if prof[k] > 0 else None,
-                "traffic": measured_traffic(f"{traffic_tag}:{k}") if traffic_tag else None}
-            for k in prof}
+def kernel_table(prof, kb, traffic_tag=None, lb=None):
+    """Per kernel: mean launch (ms), algorithmic bytes and rate, measured HBM traffic; with lb
+    (the tile solver's layout bytes: its per-slot records and padded state records, each element
+    once) also those and the measured traffic over them, the share of data fetched twice."""
+    out = {}
+    for k in prof:
+        tr = measured_traffic(f"{traffic_tag}:{k}") if traffic_tag else None
+        out[k] = {"ms": prof[k], "bytes": kb[k],
+                  "GBps": kb[k] / (prof[k] * 1e-3) / 1e9 if prof[k] > 0 else None,
+                  "frac_of_peak": kb[k] / (prof[k] * 1e-3) / 1e9 / HBM_PEAK_GBPS if prof[k] > 0 else None,
+                  "traffic": tr}
+        if lb is not None:
+            out[k]["layout_bytes"] = lb[k]
+            out[k]["traffic_over_layout"] = tr / lb[k] if tr and lb[k] > 0 else None
+    return out
 
 
 def field_errors(a, b, n):
@@ -492,7 +501,7 @@ def ogrid_leg(gr, args):
         kb = sv.kernel_bytes()
         su = 8 if dtype == gr.GR_F64 else 4
         res[name] = {"value": s.n_cells * args.steps / dt, "unit": "cell-updates/s", "ms_per_step": dt / args.steps * 1e3,
-                     "tile_cells": sv.tile_cells, "kernels": kernel_table(prof, kb, f"ogrid_{name}"),
+                     "tile_cells": sv.tile_cells, "kernels": kernel_table(prof, kb, f"ogrid_{name}", sv.layout_bytes()),
                      "iteration_bytes_reference_phases": iteration_bytes(s, su),
                      "iteration_GBps_reference_phases": s.n_cells * args.steps / dt * iteration_bytes(s, su) / s.n_cells / 1e9}
     if not args.no_converge:
@@ -607,7 +616,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(f"wing_mixed:{dom}"),
                      "algorithmic_bytes_per_launch": kb[dom], "avg_launch_ms": prof[dom]},
-        "kernels": kernel_table(prof, kb, "wing_mixed"),
+        "kernels": kernel_table(prof, kb, "wing_mixed", sv.layout_bytes()),
         "iteration_bytes_reference_phases": iteration_bytes(s, 4),
         "iteration_GBps_reference_phases": value * iteration_bytes(s, 4) / s.n_cells / 1e9,
     }
@@ -620,7 +629,8 @@ def main():
         dt64 = timed_steps(sv64, steps64)
         wing["f64"] = {"value": s.n_cells * steps64 / dt64, "unit": "cell-updates/s", "ms_per_step": dt64 / steps64 * 1e3,
                        "tile_cells": sv64.tile_cells,
-                       "kernels": kernel_table(sv64.profile(20), sv64.kernel_bytes(), "wing_f64")}
+                       "kernels": kernel_table(sv64.profile(20), sv64.kernel_bytes(), "wing_f64",
+                                               sv64.layout_bytes())}
         del sv64
         if not args.no_converge:
             log("bench: swept wing to a 1e-3 residual drop (mixed precision, cold start)")

@@ -480,6 +480,13 @@ class Solver:
         _check(load().gr_solver_kernel_bytes(self._h, _p(b, C.c_double)))
         return dict(zip(self.KERNELS, b.tolist()))
 
+    def layout_bytes(self) -> dict:
+        """Bytes of the arrays one launch of each kernel reads or writes in this library's
+        layout (per-slot records, padded state records), each element once."""
+        b = np.zeros(4)
+        _check(load().gr_solver_layout_bytes(self._h, _p(b, C.c_double)))
+        return dict(zip(self.KERNELS, b.tolist()))
+
 
 class Batch:
     """Up to 64 Euler instances (own far-field Mach vectors) on one mesh, advanced together."""

@@ -128,6 +128,7 @@ SIGNATURES = {
     "gr_solver_profile": (_i, [_vp, _i, _dp]),
     "gr_solver_autotune": (_i, [_vp, _i, C.POINTER(C.c_uint32)]),
     "gr_solver_kernel_bytes": (_i, [_vp, _dp]),
+    "gr_solver_layout_bytes": (_i, [_vp, _dp]),
     "gr_stream_copy_probe": (_i, [C.c_size_t, _i, _dp]),
     "gr_batch_create": (_i, [_vp, _u32, C.POINTER(SolverConfig), C.POINTER(_vp)]),
     "gr_batch_free": (None, [_vp]),

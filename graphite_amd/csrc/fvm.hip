@@ -637,6 +637,12 @@ struct gr_solver {
         const size_t w = 16 / sizeof(S);
         return (slots * gr::kFluxRec + w - 1) / w * w;
     }
+    // gradient records, rounded up to whole 16-byte words (the ghost records after them are
+    // read with 16-byte loads)
+    template <typename S> static size_t grad_span(size_t slots) {
+        const size_t w = 16 / sizeof(S);
+        return (slots * gr::kGradRec + w - 1) / w * w;
+    }
     template <typename S> gr::SolverArgs<S> sargs(int b) const {
         gr::SolverArgs<S> a{};
         a.u = rec<S>(b);
@@ -645,7 +651,7 @@ struct gr_solver {
         const size_t slots = mesh->h.cell_faces_starts[mesh->h.n_cells];
         a.frec = (const S *)slot_geo.p;
         a.grec = a.frec + flux_span<S>(slots);
-        a.gnrm = a.grec + slots * gr::kGradRec;
+        a.gnrm = a.grec + grad_span<S>(slots);
         if constexpr (sizeof(S) == 8) a.lsq = (const S *)mesh->mat.p;
         else a.lsq = (const S *)lsq32.p;
         a.lsq_stride = std::max<size_t>(64, mesh->mstride());
@@ -737,7 +743,7 @@ struct gr_solver {
     template <typename S> void build_slot_geometry() {
         gr_mesh *m = mesh;
         const size_t slots = m->h.cell_faces_starts[m->h.n_cells];
-        slot_geo.alloc(std::max<size_t>(1, (flux_span<S>(slots) + slots * gr::kGradRec + (size_t)m->h.n_ghosts * gr::kGhostRec) *
+        slot_geo.alloc(std::max<size_t>(1, (flux_span<S>(slots) + grad_span<S>(slots) + (size_t)m->h.n_ghosts * gr::kGhostRec) *
                                               sizeof(S)));
         if (sizeof(S) == 4) lsq32.alloc(6 * std::max<size_t>(64, m->mstride()));
         const gr::SolverArgs<S> a = sargs<S>(0);
@@ -826,6 +832,30 @@ struct gr_solver {
         // flux+update: U, gradient, limiter of cells, centres of cells and ghosts, volumes,
         // face centre+normal+area, CSR; new U written.
         out[2] = su * (25 * N) + sg * (3 * (N + G) + N + 7 * F) + 4.0 * (N + 1 + 2 * slots) + su * (5 * N);
+        out[3] = 8.0 * ntiles;
+    }
+    // Bytes of the arrays each kernel reads or writes in this library's layout, each element
+    // once: a gradient record (kGradRec) and a flux record (kFluxRec) per face slot in place of
+    // the gathered cell and face metrics, state records padded to kUR values, kGL gradient +
+    // limiter values per cell, the tile records.  HBM traffic above these is data read twice.
+    void layout_bytes(double out[4]) const {
+        const double N = mesh->h.n_cells;
+        const double slots = mesh->h.cell_faces_starts[mesh->h.n_cells];
+        double g_bc = 0, g_state = 0;   // ghosts of regions with a BC (ghost records) / without (state records)
+        for (uint32_t r = 0; r < mesh->h.n_regions; ++r)
+            (bcs[r].set ? g_bc : g_state) += mesh->h.ghost_cell_starts[r + 1] - mesh->h.ghost_cell_starts[r];
+        const double su = (double)cfg.dtype, ur = su * gr::kUR;
+        const double gl = su * (cfg.dtype == GR_F64 ? gr::kGL<double> : gr::kGL<float>);
+        const double tiles = 4.0 * gr::kTileRec * ntiles;
+        out[0] = 0.0;
+        // tile records, neighbour ids, cell and ghost states, gradient records, ghost records, LSQ
+        // matrices (6 in the solver's precision); gradient + limiter written
+        out[1] = tiles + 4.0 * slots + ur * (N + g_state) + su * gr::kGradRec * slots + su * gr::kGhostRec * g_bc +
+                 su * 6 * N + gl * N;
+        // tile records, neighbour ids, flux records, cell and ghost states, gradient + limiter,
+        // volumes; new state records and the tile partials written
+        out[2] = tiles + 4.0 * slots + su * gr::kFluxRec * slots + ur * (N + g_state) + gl * N + 8.0 * N + ur * N +
+                 8.0 * ntiles;
         out[3] = 8.0 * ntiles;
     }
     hipGraphExec_t graph(int b) {
@@ -1115,6 +1145,13 @@ int gr_solver_kernel_bytes(const gr_solver *s, double bytes[4]) {
     return guarded([&] {
         require(s && bytes, "null argument");
         s->kernel_bytes(bytes);
+    });
+}
+
+int gr_solver_layout_bytes(const gr_solver *s, double bytes[4]) {
+    return guarded([&] {
+        require(s && bytes, "null argument");
+        s->layout_bytes(bytes);
     });
 }
 

@@ -163,23 +163,13 @@ __host__ __device__ constexpr uint64_t row_limiters_packed(int corr) {
 //   flux record:     n = outer_normal(n_f, fc - x_i) (3), ds = A_f / V_i, d0 = fc - x_i (3),
 //                    d1 = fc - x_j (3): 80 bytes in binary64 (16-byte loads), 40 in binary32
 //                    (8-byte loads)
-//   gradient record: d = x_j - x_i (3), 1/|d|^2, d0 (3), the neighbour id j (its bits)
+//   gradient record: d = x_j - x_i (3), d0 (3): 48 bytes in binary64 (16-byte loads), 24 in
+//                    binary32 (8-byte loads); the kernel forms 1/|d|^2 from d with the
+//                    expression that set-up would store, and reads the neighbour from cc
 //   ghost record:    n of the ghost's face seen from its cell (3), padding (fused BCs)
 constexpr int kFluxRec = 10;
-constexpr int kGradRec = 8;
+constexpr int kGradRec = 6;
 constexpr int kGhostRec = 4;
-
-// An index stored in (read from) a value slot of a record: its bits, never used in arithmetic.
-template <typename S>
-__device__ inline S id_bits(uint32_t j) {
-    if constexpr (sizeof(S) == 8) return __longlong_as_double((long long)j);
-    else return __uint_as_float(j);
-}
-template <typename S>
-__device__ inline uint32_t bits_id(S v) {
-    if constexpr (sizeof(S) == 8) return (uint32_t)__double_as_longlong(v);
-    else return __float_as_uint(v);
-}
 
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_rec, S *grad_rec, S *ghost_rec) {
@@ -205,10 +195,8 @@ __global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_re
         S *gq = grad_rec + (size_t)q * kGradRec;
 #pragma unroll
         for (int k = 0; k < 3; ++k) gq[k] = d[k];
-        gq[3] = fdiv(S(1), (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) gq[4 + k] = d0[k];
-        gq[7] = id_bits<S>(j);
+        for (int k = 0; k < 3; ++k) gq[3 + k] = d0[k];
         if (j >= m.N) {   // a ghost's only face slot belongs to this cell
             S *g = ghost_rec + (size_t)(j - m.N) * kGhostRec;
 #pragma unroll
@@ -374,12 +362,13 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     // so that its latency runs alongside the lookup's (wing: 164 -> 158 us); in binary32 that
     // was slower (79.3 -> 81.2 us), and the loads follow the cell's own state after the barrier.
     constexpr bool kEarly = sizeof(S) == 8;
-    S dw[kGradRec], uj[5], gn[4];   // dw: d = x_j - x_i, 1/|d|^2, fc - x_i (phase C), j
+    constexpr int kGradWB = sizeof(S) == 8 ? 16 : 8;
+    S dw[kGradRec], uj[5], gn[4];   // dw: d = x_j - x_i, fc - x_i (phase C)
     int r = -1;                     // the ghost neighbour's BC region (-1: state from the array)
     if constexpr (kEarly) {
         if (t < nslot) {
-            load_vec<S, kGradRec>(a.grec + (size_t)(q0 + t) * kGradRec, dw);
-            const uint32_t j = bits_id<S>(dw[7]);
+            const uint32_t j = m.cc[q0 + t];
+            load_vec<S, kGradRec, kGradWB>(a.grec + (size_t)(q0 + t) * kGradRec, dw);
             if (j >= m.N) r = bc_region(bc, j - m.N);
             if (r < 0) load_state<S>(a.u, j, uj);
             else load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, gn);
@@ -398,8 +387,8 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         if constexpr (kEarly) {
             if (r >= 0) ghost_state<S>(bc.r[r], gn, ui, uj);
         } else {
-            load_vec<S, kGradRec>(a.grec + (size_t)(q0 + t) * kGradRec, dw);
-            const uint32_t j = bits_id<S>(dw[7]);
+            const uint32_t j = m.cc[q0 + t];
+            load_vec<S, kGradRec, kGradWB>(a.grec + (size_t)(q0 + t) * kGradRec, dw);
             r = j >= m.N ? bc_region(bc, j - m.N) : -1;
             if (r < 0) {
                 load_state<S>(a.u, j, uj);
@@ -408,15 +397,17 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
                 ghost_state<S>(bc.r[r], gn, ui, uj);
             }
         }
-        d0[0] = dw[4];
-        d0[1] = dw[5];
-        d0[2] = dw[6];
+        d0[0] = dw[3];
+        d0[1] = dw[4];
+        d0[2] = dw[5];
+        // the LSQ weight 1/|d|^2, products summed in x, y, z order
+        const S w2 = fdiv(S(1), (dw[0] * dw[0] + dw[1] * dw[1]) + dw[2] * dw[2]);
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const S du = uj[c] - ui[c];
             if constexpr (kTerms) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) s_buf[3 * c + k][t] = (dw[k] * du) * dw[3];
+                for (int k = 0; k < 3; ++k) s_buf[3 * c + k][t] = (dw[k] * du) * w2;
             } else {
                 s_buf[c][t] = du;
             }
@@ -424,7 +415,8 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
         }
         if constexpr (!kTerms) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) s_buf[5 + k][t] = dw[k];
+            for (int k = 0; k < 3; ++k) s_buf[5 + k][t] = dw[k];
+            s_buf[8][t] = w2;
         }
     }
     __syncthreads();

@@ -460,3 +460,39 @@ def test_ogrid_c2_mixed_precision_error_at_1e3(gr):
     assert abs(out[gr.GR_F64][0] - out[gr.GR_F32][0]) <= 20
     for k, (mx, l2) in errs.items():
         assert mx < 1e-2 and l2 < 1e-3, (k, mx, l2)
+
+
+def test_kernel_and_layout_bytes_count_the_arrays(gr, orc, c1):
+    """The byte counts the benchmark divides by: algorithmic (the reference's arrays, each
+    element once) and layout (the arrays the tile kernels read and write: per-slot gradient
+    records of 6 values, flux records of 10 and neighbour ids, state records of 8, gradient + limiter records
+    of 20 in binary64 and 16 in binary32, tile records of 16 words, ghost records of 4 for the
+    regions with a boundary condition), counted here from the mesh's connectivity with one
+    region left without a boundary condition (its ghost states are read as records)."""
+    m, om, _ = c1
+    mach = (0.5, 0.1, 0.0)
+    state = perturbed_state(om.cell_centers, mach=mach)
+    names = m.region_names()
+    kinds = ["farfield", "farfield", "slip", "slip", None, "farfield"]
+    bcs = [(nm, k, mach) for nm, k in zip(names, kinds) if k]
+    con = m.connectivity()
+    N, sz = m.n_cells, m.sizes()
+    G, F = sz.n_ghosts, sz.n_faces
+    slots = int(con["cell_faces_starts"][N])
+    per_region = np.diff(con["ghost_cell_starts"].astype(np.int64))
+    g_state = int(sum(n for n, k in zip(per_region, kinds) if k is None))
+    g_bc = G - g_state
+    assert g_state > 0 and g_bc > 0
+    for dtype, su, gl in ((gr.GR_F64, 8, 20), (gr.GR_F32, 4, 16)):
+        sv, _ = _solver_run(gr, m, dtype, bcs, state, 1)
+        kb, lb = sv.kernel_bytes(), sv.layout_bytes()
+        ntiles = kb["residual_reduce"] / 8
+        assert ntiles == int(ntiles) and ntiles >= N / 51
+        assert kb["grad_lim"] == su * 5 * N + 8 * (3 * (N + G) + 6 * N + 3 * F + 3 * G) + 4 * (N + 1 + 2 * slots) + su * 20 * N
+        assert kb["flux_update"] == su * 25 * N + 8 * (3 * (N + G) + N + 7 * F) + 4 * (N + 1 + 2 * slots) + su * 5 * N
+        tiles = 64 * ntiles
+        assert lb["grad_lim"] == (tiles + 4 * slots + su * 8 * (N + g_state) + su * 6 * slots + su * 4 * g_bc
+                                  + su * 6 * N + su * gl * N)
+        assert lb["flux_update"] == (tiles + 4 * slots + su * 10 * slots + su * 8 * (N + g_state) + su * gl * N
+                                     + 8 * N + su * 8 * N + 8 * ntiles)
+        assert lb["bc"] == 0 and lb["residual_reduce"] == kb["residual_reduce"]
