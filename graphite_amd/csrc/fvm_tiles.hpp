@@ -206,18 +206,30 @@ __global__ __launch_bounds__(kBlock) void k_slot_geometry(MeshView m, S *flux_re
     }
 }
 
-// The solver's state as one record per cell and ghost: rho, rho*u (3), rho*E, then padding to
-// kUR values (64 bytes in binary64, 32 in binary32), so a neighbour's state is one aligned
-// segment of a cache line where five arrays would touch five lines.
-constexpr int kUR = 8;
+// The solver's state as one record per cell and ghost: rho, rho*u (3), rho*E and one value of
+// padding (48 bytes in binary64, read with 16-byte loads; 24 in binary32, 8-byte loads), so a
+// neighbour's state is one contiguous run where five arrays would touch five cache lines.
+constexpr int kUR = 6;
 
 template <typename S>
 __device__ inline void load_state(const S *__restrict__ u, uint32_t i, S v[5]) {
-    constexpr int W = sizeof(S) == 8 ? 6 : 8;
-    S w[W];
-    load_vec<S, W>(u + (size_t)i * kUR, w);
+    const S *p = u + (size_t)i * kUR;
+    if constexpr (sizeof(S) == 8) {
+        S w[kUR];
+        load_vec<S, kUR, 16>(p, w);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) v[c] = w[c];
+        for (int c = 0; c < 5; ++c) v[c] = w[c];
+    } else {
+        // a 24-byte record starts 8-byte aligned: one 16-byte load (global loads of whole dwords
+        // need only dword alignment on this target) and one 4-byte load, two instructions per
+        // gathered neighbour where three 8-byte loads would take three
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        typedef f4 f4a8 __attribute__((aligned(8)));
+        const f4a8 a = *reinterpret_cast<const f4a8 *>(p);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = a[c];
+        v[4] = p[4];
+    }
 }
 
 // Ghost states of buffer u from its interior cells, for regions with a boundary condition (the
@@ -243,7 +255,7 @@ template <typename S>
 __global__ __launch_bounds__(kBlock) void k_fill_states(S *u, size_t n, S v0, S v1, S v2, S v3, S v4) {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n * kUR) return;
-    const int r = (int)(k & (kUR - 1));
+    const int r = (int)(k % kUR);
     u[k] = r == 0 ? v0 : r == 1 ? v1 : r == 2 ? v2 : r == 3 ? v3 : r == 4 ? v4 : S(0);
 }
 
@@ -629,7 +641,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
     __syncthreads();
     // the tile's new state records, contiguous in memory, written coalesced
     for (int k = t; k < kUR * ncell; k += blockDim.x) {
-        const int r = k & (kUR - 1);
+        const int r = k % kUR;
         a.un[(size_t)c0 * kUR + k] = r < 5 ? s_c[r][k / kUR] : S(0);
     }
     double e_loc = 0;

@@ -353,7 +353,7 @@ int gr_solver_autotune(gr_solver *s, int n_iter, uint32_t *tile_cells);
  * counted once), same order as gr_solver_profile. */
 int gr_solver_kernel_bytes(const gr_solver *s, double bytes[4]);
 /* Bytes of the arrays one launch of each kernel reads or writes as this library lays them out
- * (per-slot gradient and flux records in place of gathered metrics, state records padded to 8
+ * (per-slot gradient and flux records in place of gathered metrics, state records padded to 6
  * values, gradient + limiter records of 20 values in binary64 and 16 in binary32, tile records),
  * each element counted once; same order as gr_solver_profile.  Measured HBM traffic above these
  * is data fetched twice. */

@@ -465,7 +465,7 @@ def test_ogrid_c2_mixed_precision_error_at_1e3(gr):
 def test_kernel_and_layout_bytes_count_the_arrays(gr, orc, c1):
     """The byte counts the benchmark divides by: algorithmic (the reference's arrays, each
     element once) and layout (the arrays the tile kernels read and write: per-slot gradient
-    records of 6 values, flux records of 10 and neighbour ids, state records of 8, gradient + limiter records
+    records of 6 values, flux records of 10 and neighbour ids, state records of 6, gradient + limiter records
     of 20 in binary64 and 16 in binary32, tile records of 16 words, ghost records of 4 for the
     regions with a boundary condition), counted here from the mesh's connectivity with one
     region left without a boundary condition (its ghost states are read as records)."""
@@ -491,8 +491,8 @@ def test_kernel_and_layout_bytes_count_the_arrays(gr, orc, c1):
         assert kb["grad_lim"] == su * 5 * N + 8 * (3 * (N + G) + 6 * N + 3 * F + 3 * G) + 4 * (N + 1 + 2 * slots) + su * 20 * N
         assert kb["flux_update"] == su * 25 * N + 8 * (3 * (N + G) + N + 7 * F) + 4 * (N + 1 + 2 * slots) + su * 5 * N
         tiles = 64 * ntiles
-        assert lb["grad_lim"] == (tiles + 4 * slots + su * 8 * (N + g_state) + su * 6 * slots + su * 4 * g_bc
+        assert lb["grad_lim"] == (tiles + 4 * slots + su * 6 * (N + g_state) + su * 6 * slots + su * 4 * g_bc
                                   + su * 6 * N + su * gl * N)
-        assert lb["flux_update"] == (tiles + 4 * slots + su * 10 * slots + su * 8 * (N + g_state) + su * gl * N
-                                     + 8 * N + su * 8 * N + 8 * ntiles)
+        assert lb["flux_update"] == (tiles + 4 * slots + su * 10 * slots + su * 6 * (N + g_state) + su * gl * N
+                                     + 8 * N + su * 6 * N + 8 * ntiles)
         assert lb["bc"] == 0 and lb["residual_reduce"] == kb["residual_reduce"]
