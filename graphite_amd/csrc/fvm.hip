@@ -612,7 +612,7 @@ struct gr_solver {
     gr::SolverCtl *ctl_host = nullptr;   // pinned: [0] upload, [1],[2] polling
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
-    int graph_iters = 10;   // iterations per captured graph (even)
+    int graph_iters = 20;   // iterations per captured graph (even)
     int last_iters = 0;
 
     ~gr_solver() {

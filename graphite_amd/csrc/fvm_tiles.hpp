@@ -541,32 +541,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
     const int t = threadIdx.x;
     const bool in_b = t < 5 * ncell;   // one thread per (component, cell) in the update
     const int cc_ = in_b ? t / ncell : 0, cl = in_b ? t - cc_ * ncell : 0;
-    // What a slot reads by its own index (neighbour, record, the neighbour's state): in binary64
-    // it is loaded before the barrier that the slot -> cell lookup needs, so that its latency
-    // runs alongside the lookup's (wing: 221.6 -> 219.0 us); in binary32 that was slower (96 ->
-    // 102 us, with the neighbour's gradient record as well), and the loads stay where they were.
-    constexpr bool kEarly = sizeof(S) == 8;
+    // What a slot reads by its own index (neighbour, record, the neighbour's state) is loaded
+    // before the barrier that the slot -> cell lookup needs, so that its latency runs alongside
+    // the lookup's (wing: binary32 95.3 -> 90.3 us, binary64 221.6 -> 219.0 us; the neighbour's
+    // gradient record as well made binary32 slower, 96 -> 102 us)
     uint32_t j = 0;
     int jr = -1;   // the ghost neighbour's BC region (-1: state read from the array)
     S geo[kFluxRec], uj[5];
-    if constexpr (kEarly) {
-        if (t < nslot) {
-            j = m.cc[q0 + t];
-            load_vec<S, kFluxRec, 2 * sizeof(S) == 16 ? 16 : 8>(a.frec + (size_t)(q0 + t) * kFluxRec, geo);
-            if (j >= m.N) jr = bc_region(bc, j - m.N);
-            if (jr < 0) load_state<S>(a.u, j, uj);
-        }
+    if (t < nslot) {
+        j = m.cc[q0 + t];
+        load_vec<S, kFluxRec, 2 * sizeof(S) == 16 ? 16 : 8>(a.frec + (size_t)(q0 + t) * kFluxRec, geo);
+        if (j >= m.N) jr = bc_region(bc, j - m.N);
+        if (jr < 0) load_state<S>(a.u, j, uj);
     }
     for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = starts[k];
     __syncthreads();
 
     if (t < nslot) {
         const int lc = tile_cell_of(s_start, ncell, (uint32_t)t);
-        const uint32_t i = c0 + lc, q = q0 + t;
-        if constexpr (!kEarly) {
-            j = m.cc[q];
-            load_vec<S, kFluxRec, 2 * sizeof(S) == 16 ? 16 : 8>(a.frec + (size_t)q * kFluxRec, geo);
-        }
+        const uint32_t i = c0 + lc;
         const S n[3] = {geo[0], geo[1], geo[2]};
         const S ds = geo[3];
         const S d0[3] = {geo[4], geo[5], geo[6]};
@@ -576,18 +569,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
         S r0, ru0[3], e0, r1, ru1[3], e1;
         recon_at<S, CORR>(a.gl, i, ui, d0, r0, ru0, e0);
         if (j < m.N) {
-            if constexpr (!kEarly) load_state<S>(a.u, j, uj);
             recon_at<S, CORR>(a.gl, j, uj, d1, r1, ru1, e1);
         } else {   // ghost: its state from the BC, gradient and limiter of cell i
             S ug[5];
-            if constexpr (!kEarly) jr = bc_region(bc, j - m.N);
             if (jr < 0) {
-                if constexpr (kEarly) {
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) ug[c] = uj[c];
-                } else {
-                    load_state<S>(a.u, j, ug);
-                }
+                for (int c = 0; c < 5; ++c) ug[c] = uj[c];
             } else {
                 ghost_state<S>(bc.r[jr], n, ui, ug);
             }
