@@ -324,6 +324,11 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     const uint32_t c0 = tr[0], q0 = tr[1];
     const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
     const uint8_t *starts = reinterpret_cast<const uint8_t *>(tr + 3);
+    // the tile's slot offsets into LDS before any other load is issued: the loads after this
+    // need the tile record, whose cache line holds the offsets, so the wait costs nothing, and
+    // it does not drain them (loads complete in order, so a wait issued after them would)
+    __shared__ uint32_t s_start[kTileCells + 1];
+    if (threadIdx.x <= (unsigned)ncell) s_start[threadIdx.x] = starts[threadIdx.x];
     if (ctl) {
         __shared__ int s_stop;
         if (blockIdx.x == 0 && ctl->pending) {   // uniform across the block
@@ -346,7 +351,6 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     // after phase D its limiters sit in row kUj+c of column sb.
     constexpr bool kTerms = sizeof(S) == 4;
     constexpr int kUj = kTerms ? 15 : 9;
-    __shared__ uint32_t s_start[kTileCells + 1];
     __shared__ S s_buf[kUj + 5][kTileSlots];
     auto grad_at = [&](int c, int k, int sb) -> S & { return kTerms ? s_buf[3 * c + k][sb] : s_buf[c][sb + k]; };
     auto min_at = [&](int c, int sb) -> S & {
@@ -386,7 +390,6 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
             else load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, gn);
         }
     }
-    for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = starts[k];
     __syncthreads();
 
     // A: per slot u_j and either its gradient terms or u_j - u_i with the slot's offset and weight
@@ -530,11 +533,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
     const uint32_t c0 = tr[0], q0 = tr[1];
     const int ncell = (int)(tr[2] & 0xFFFFu), nslot = (int)(tr[2] >> 16);
     const uint8_t *starts = reinterpret_cast<const uint8_t *>(tr + 3);
+    // the tile's slot offsets into LDS before any other load is issued: the loads after this
+    // need the tile record, whose cache line holds the offsets, so the wait costs nothing, and
+    // it does not drain them (loads complete in order, so a wait issued after them would)
+    __shared__ uint32_t s_start[kTileCells + 1];
+    if (threadIdx.x <= (unsigned)ncell) s_start[threadIdx.x] = starts[threadIdx.x];
     if (ctl) {
         if (ctl->stop) return;
         if (blockIdx.x == 0 && threadIdx.x == 0) ctl->pending = 1;   // partials for the next launch to sum
     }
-    __shared__ uint32_t s_start[kTileCells + 1];
     __shared__ S s_c[5][kTileSlots];   // central flux term ((F0+F1)*ds)*0.5
     __shared__ S s_d[5][kTileSlots];   // dissipation term ((eig*(U1-U0))*0.5)*ds
     __shared__ S s_e[kTileSlots];      // eig*ds
@@ -554,7 +561,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
         if (j >= m.N) jr = bc_region(bc, j - m.N);
         if (jr < 0) load_state<S>(a.u, j, uj);
     }
-    for (int k = t; k <= ncell; k += blockDim.x) s_start[k] = starts[k];
     __syncthreads();
 
     if (t < nslot) {
