@@ -551,7 +551,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(S
     // What a slot reads by its own index (neighbour, record, the neighbour's state) is loaded
     // before the barrier that the slot -> cell lookup needs, so that its latency runs alongside
     // the lookup's (wing: binary32 95.3 -> 90.3 us, binary64 221.6 -> 219.0 us; the neighbour's
-    // gradient record as well made binary32 slower, 96 -> 102 us)
+    // gradient record as well left binary32 no faster, 0.1675 -> 0.1680 ms per iteration)
     uint32_t j = 0;
     int jr = -1;   // the ghost neighbour's BC region (-1: state read from the array)
     S geo[kFluxRec], uj[5];
