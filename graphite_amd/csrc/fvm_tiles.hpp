@@ -376,7 +376,10 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     // What a slot reads by its own index (its record, the neighbour's state or the ghost's face
     // normal): in binary64 it is loaded before the barrier that the slot -> cell lookup needs,
     // so that its latency runs alongside the lookup's (wing: 164 -> 158 us); in binary32 that
-    // was slower (79.3 -> 81.2 us), and the loads follow the cell's own state after the barrier.
+    // was slower (0.1745 -> 0.1769 ms per iteration), and the loads come right after the barrier,
+    // before the lookup and the cell's own state (0.1679 -> 0.1660 ms against after them: loads
+    // complete in order, so the neighbour's state, which waits for its index, no longer waits
+    // for the cell's state as well).
     constexpr bool kEarly = sizeof(S) == 8;
     constexpr int kGradWB = sizeof(S) == 8 ? 16 : 8;
     S dw[kGradRec], uj[5], gn[4];   // dw: d = x_j - x_i, fc - x_i (phase C)
@@ -396,22 +399,17 @@ __global__ __launch_bounds__(kBlock) void k_gradlim_tile(MeshView m, SolverArgs<
     int lc = 0;
     S ui[5], d0[3] = {S(0), S(0), S(0)};
     if (t < nslot) {
-        lc = tile_cell_of(s_start, ncell, (uint32_t)t);
-        const uint32_t i = c0 + lc;
-        load_state<S>(a.u, i, ui);
-        if constexpr (kEarly) {
-            if (r >= 0) ghost_state<S>(bc.r[r], gn, ui, uj);
-        } else {
+        if constexpr (!kEarly) {   // the neighbour's loads first: they need no lookup
             const uint32_t j = m.cc[q0 + t];
             load_vec<S, kGradRec, kGradWB>(a.grec + (size_t)(q0 + t) * kGradRec, dw);
             r = j >= m.N ? bc_region(bc, j - m.N) : -1;
-            if (r < 0) {
-                load_state<S>(a.u, j, uj);
-            } else {
-                load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, gn);
-                ghost_state<S>(bc.r[r], gn, ui, uj);
-            }
+            if (r < 0) load_state<S>(a.u, j, uj);
+            else load_vec<S, 4>(a.gnrm + (size_t)(j - m.N) * kGhostRec, gn);
         }
+        lc = tile_cell_of(s_start, ncell, (uint32_t)t);
+        const uint32_t i = c0 + lc;
+        load_state<S>(a.u, i, ui);
+        if (r >= 0) ghost_state<S>(bc.r[r], gn, ui, uj);
         d0[0] = dw[3];
         d0[1] = dw[4];
         d0[2] = dw[5];
