@@ -46,6 +46,9 @@ struct BatchArgs {
     S *g;         // 15 blocks
     S *l;         // 5 blocks
     size_t stride;
+    const S *frec;   // per-slot flux, gradient and ghost records (k_slot_geometry): the metrics
+    const S *grec;   // every instance of a cell reads, cast as the kernels cast them, by slot
+    const S *gnrm;   // index instead of gathered by neighbour and face
 };
 
 // Cell of this wave (wave-uniform, in a scalar register).  Blocks are mapped XCD-aware: the
@@ -77,7 +80,6 @@ __global__ __launch_bounds__(CPB *kBatchLanes) void k_batch_gradlim(MeshView m, 
     if (i >= m.N) return;
     const uint32_t k = threadIdx.x & 63;
     const size_t st = a.stride, me = (size_t)i * kBatchLanes + k;
-    const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
     S ui[5], umin[5], umax[5], g[15];
 #pragma unroll
     for (int c = 0; c < 5; ++c) ui[c] = umin[c] = umax[c] = a.u[c * st + me];
@@ -86,17 +88,18 @@ __global__ __launch_bounds__(CPB *kBatchLanes) void k_batch_gradlim(MeshView m, 
     const uint32_t s = m.cfs[i], e = m.cfs[i + 1];
     for (uint32_t q = s; q < e; ++q) {
         const uint32_t j = m.cc[q];
+        // d = x_j - x_i from the slot's gradient record; the ghost's face normal from its record
+        const S d[3] = {a.grec[(size_t)q * kGradRec], a.grec[(size_t)q * kGradRec + 1], a.grec[(size_t)q * kGradRec + 2]};
         S uj[5];
         const int r = j >= m.N ? bc_region(bc, j - m.N) : -1;
         if (r < 0) {
 #pragma unroll
             for (int c = 0; c < 5; ++c) uj[c] = a.u[c * st + (size_t)j * kBatchLanes + k];
         } else {
-            S n[3];
-            outward_normal<S>(m, m.cf[q], xi, yi, zi, n);
+            const S *gn = a.gnrm + (size_t)(j - m.N) * kGhostRec;
+            const S n[3] = {gn[0], gn[1], gn[2]};
             ghost_state<S>(inst_region(bc.r[r], inst_mach, k), n, ui, uj);
         }
-        const S d[3] = {(S)(m.cx[j] - xi), (S)(m.cy[j] - yi), (S)(m.cz[j] - zi)};
         const S w2 = fdiv(S(1), (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
@@ -116,8 +119,8 @@ __global__ __launch_bounds__(CPB *kBatchLanes) void k_batch_gradlim(MeshView m, 
     }
     S lim[5] = {S(2), S(2), S(2), S(2), S(2)};
     for (uint32_t q = s; q < e; ++q) {
-        const uint32_t f = m.cf[q];
-        const S dx = (S)(m.fcx[f] - xi), dy = (S)(m.fcy[f] - yi), dz = (S)(m.fcz[f] - zi);
+        const S *gq = a.grec + (size_t)q * kGradRec + 3;   // fc - x_i
+        const S dx = gq[0], dy = gq[1], dz = gq[2];
 #pragma unroll
         for (int c = 0; c < 5; ++c)
             lim[c] = fmin(lim[c], limiter_face<S>(ui[c], g[3 * c], g[3 * c + 1], g[3 * c + 2], umin[c], umax[c], dx,
@@ -188,18 +191,17 @@ __device__ __forceinline__ void batch_flux(const MeshView &m, const BatchArgs<S>
         } else {
 #pragma unroll
             for (int q = 0; q < 15; ++q) gi[q] = a.g[q * st + me];
-            const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
             const double V = m.cv[i];
             S acc[5] = {0, 0, 0, 0, 0}, dts = 0;
             const uint32_t s = m.cfs[i], e = m.cfs[i + 1];
             for (uint32_t q = s; q < e; ++q) {
-                const uint32_t f = m.cf[q], j = m.cc[q];
-                const double fcx = m.fcx[f], fcy = m.fcy[f], fcz = m.fcz[f];
-                S n[3];
-                outward_normal<S>(m, f, xi, yi, zi, n);
-                const S ds = geo_ratio<S>(m.fa[f], V);
-                const S d0[3] = {(S)(fcx - xi), (S)(fcy - yi), (S)(fcz - zi)};
-                const S d1[3] = {(S)(fcx - m.cx[j]), (S)(fcy - m.cy[j]), (S)(fcz - m.cz[j])};
+                const uint32_t j = m.cc[q];
+                // outward normal, A_f / V_i, fc - x_i, fc - x_j: the slot's flux record
+                const S *fr = a.frec + (size_t)q * kFluxRec;
+                const S n[3] = {fr[0], fr[1], fr[2]};
+                const S ds = fr[3];
+                const S d0[3] = {fr[4], fr[5], fr[6]};
+                const S d1[3] = {fr[7], fr[8], fr[9]};
                 S r0, ru0[3], e0, r1, ru1[3], e1;
                 if constexpr (sizeof(S) == 8) reconstruct<S, CORR>(ui, gi, li, d0, r0, ru0, e0);
                 else reconstruct_pre<S, CORR>(ui, gi, d0, r0, ru0, e0);
@@ -515,6 +517,8 @@ struct gr_batch {
     uint32_t n_inst = 0;
     size_t stride = 0;   // elements per component block: pad64(N+G) * 64
     gr::DevArray<char> ubuf[2], grad, lim;
+    gr::DevArray<char> slot_geo;      // per-slot flux and gradient records, ghost records
+    size_t grec_off = 0, gnrm_off = 0;   // their offsets in values of the batch's precision
     gr::DevArray<double> inst_mach;   // 3 per lane
     std::vector<double> mach_h;       // 3 per instance
     double p_inf = 1.0, T_inf = 1.0 / 1.4;
@@ -572,7 +576,25 @@ struct gr_batch {
         return ((size_t)n_cells + kMinCpb - 1) / kMinCpb + kMaxGroups;
     }
     template <typename S> gr::BatchArgs<S> args(int b) const {
-        return gr::BatchArgs<S>{(const S *)ubuf[b].p, (S *)ubuf[1 - b].p, (S *)grad.p, (S *)lim.p, stride};
+        const S *fr = (const S *)slot_geo.p;
+        return gr::BatchArgs<S>{(const S *)ubuf[b].p, (S *)ubuf[1 - b].p, (S *)grad.p, (S *)lim.p, stride,
+                                fr, fr + grec_off, fr + gnrm_off};
+    }
+    // Per-slot records of the tile kernels (k_slot_geometry), read by the batch kernels in place
+    // of gathering the cell and face metrics face by face: the same values, cast the same way.
+    template <typename S> void build_slot_geometry() {
+        gr_mesh *m = mesh;
+        const size_t slots = m->h.cell_faces_starts[m->h.n_cells];
+        const size_t w = 16 / sizeof(S);
+        grec_off = (slots * gr::kFluxRec + w - 1) / w * w;
+        gnrm_off = grec_off + (slots * gr::kGradRec + w - 1) / w * w;
+        slot_geo.alloc(std::max<size_t>(1, (gnrm_off + (size_t)m->h.n_ghosts * gr::kGhostRec) * sizeof(S)));
+        if (m->h.n_cells) {
+            S *fr = (S *)slot_geo.p;
+            hipLaunchKernelGGL(gr::k_slot_geometry<S>, dim3(gr::blocks_for(m->h.n_cells)), dim3(gr::kBlock), 0, m->stream,
+                               m->view(), fr, fr + grec_off, fr + gnrm_off);
+        }
+        GR_HIP(hipGetLastError());
     }
     gr::BcTable table() const {
         gr::BcTable t{};
@@ -899,6 +921,8 @@ int gr_batch_create(gr_mesh *m, uint32_t n_inst, const gr_solver_config *cfg, gr
             b->lim.alloc(5 * blk);
             GR_HIP(hipMemsetAsync(b->grad.p, 0, b->grad.n, m->stream));
             GR_HIP(hipMemsetAsync(b->lim.p, 0, b->lim.n, m->stream));
+            if (cfg->dtype == GR_F64) b->build_slot_geometry<double>();
+            else b->build_slot_geometry<float>();
             b->inst_mach.alloc(3 * gr::kBatchLanes);
             b->mach_h.assign(3 * n_inst, 0.0);
             b->region_kind.assign(m->h.n_regions, -1);
