@@ -381,11 +381,20 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_iter(uint32_t N, const uint32_t
         xi = x[i];
         k0 = start[i];
         k1 = start[i + 1];
+        // the row's indices first, then the values at them: loads complete in order, so a wait
+        // for one neighbour's index inside a per-neighbour loop would also wait for the values
+        // of the neighbours before it.  Entries past the row's end repeat its last (never used).
+        if (k1 > k0) {
+            uint32_t jn[kCgRow];
 #pragma unroll
-        for (int e = 0; e < kCgRow; ++e) {
-            if (k0 + e < k1) {
-                const uint32_t j = col[k0 + e];
-                nc[e] = coef[k0 + e];
+            for (int e = 0; e < kCgRow; ++e) {
+                const uint32_t ke = min(k0 + (uint32_t)e, k1 - 1);
+                jn[e] = col[ke];
+                nc[e] = coef[ke];
+            }
+#pragma unroll
+            for (int e = 0; e < kCgRow; ++e) {
+                const uint32_t j = jn[e];
                 nr[e] = r_in[j];
                 nw[e] = w_in[j];
                 ns[e] = s_in[j];
